@@ -1,0 +1,199 @@
+/*
+ * mafx.h - C-ABI of the MI355X-native magnum.af effective-field / integrator hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns an int status
+ * (MAFX_OK == 0).  On failure mafx_last_error() returns a thread-local message.  No C++
+ * exception crosses this boundary.
+ *
+ * Layout of every vector field (m, H, k, gradients): structure of arrays, double or float,
+ *     index(c, x, y, z) = x + nx * (y + ny * (z + nz * c)),   c in {0,1,2}
+ * which is bit-for-bit the column-major af::array [nx, ny, nz, 3] the reference passes around
+ * (magnumaf/include/magnumaf/mesh.hpp:34, "dims_v").  Scalar fields (Ms_field, rkky values)
+ * use index(x, y, z) = x + nx * (y + ny * z).
+ *
+ * Pointers named *_dev are device (HBM) pointers; pointers named *_host are host pointers
+ * that are read during the call only.  A context owns all of its device scratch memory and
+ * launches on one HIP stream (the null stream unless mafx_ctx_set_stream is called).
+ * A context is externally synchronised: one host thread at a time.
+ */
+#ifndef MAFX_H
+#define MAFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MAFX_OK 0
+#define MAFX_ERR_ARG 1      /* invalid argument (sizes, null pointers, unsupported combination) */
+#define MAFX_ERR_DEVICE 2   /* a HIP runtime call failed */
+#define MAFX_ERR_NUMERIC 3  /* non-finite value in a reduction / line search failure */
+#define MAFX_ERR_STATE 4    /* object used in the wrong state */
+
+/* Mesh: magnumaf/include/magnumaf/mesh.hpp:11-20 (nx, ny, nz cells of dx x dy x dz metres). */
+typedef struct mafx_mesh {
+    int64_t nx, ny, nz;
+    double dx, dy, dz;
+} mafx_mesh;
+
+/* Arithmetic of the field evaluation.  MAFX_F64 is the plain FP64 path (the reference's
+ * arithmetic).  MAFX_F32_FIELD runs the demag FFT, tensor product and stencils in FP32 with
+ * FP32 accumulation; the state m, the RK combinations, error norm, renormalisation and all
+ * energy / dot-product sums stay FP64. */
+enum mafx_precision { MAFX_F64 = 0, MAFX_F32_FIELD = 1 };
+
+/* Field terms.  Each kind replaces one reference FieldTerm subclass; the cited file holds
+ * the impl_H_in_Apm / impl_E_in_J the GPU evaluation reproduces. */
+enum mafx_term_kind {
+    MAFX_DEMAG = 1,          /* DemagField(mesh)            field_terms/micro/demag_field.cpp:211-264 */
+    MAFX_EXCHANGE = 2,       /* ExchangeField(A)            field_terms/micro/exchange_field.cpp:14-60,115-157 */
+    MAFX_DMI = 3,            /* DmiField(D, axis)           field_terms/micro/dmi_field.cpp:41-110 */
+    MAFX_UNIAXIAL = 4,       /* UniaxialAnisotropyField     field_terms/micro/uniaxial_anisotropy_field.cpp:58-89 */
+    MAFX_ZEEMAN = 5,         /* ExternalField(H)            field_terms/micro/external_field.cpp:28-48 */
+    MAFX_RKKY = 6,           /* RKKYExchangeField           field_terms/micro/rkky_exchange_field.cpp:19-29,195-337 */
+    MAFX_ATOM_DIPOLE = 11,   /* AtomisticDipoleDipoleField  field_terms/atom/atomistic_dipole_dipole_field.cpp:14-113 */
+    MAFX_ATOM_EXCHANGE = 12, /* AtomisticExchangeField(J)   field_terms/atom/atomistic_exchange_field.cpp:8-40 */
+    MAFX_ATOM_DMI = 13,      /* AtomisticDmiField(D, axis)  field_terms/atom/atomistic_dmi_field.cpp:18-49 */
+    MAFX_ATOM_UNIAXIAL = 14, /* AtomisticUniaxialAnisotropyField  field_terms/atom/atomistic_uniaxial_anisotropy_field.cpp:24-34 */
+    MAFX_ATOM_ZEEMAN = 15    /* AtomisticExternalField(H)   field_terms/atom/atomistic_external_field.cpp:5-11 */
+};
+
+/* One field term.  Unused members are ignored.
+ *   value:   A [J/m] (EXCHANGE), D [J/m^2] (DMI), Ku1 [J/m^3] (UNIAXIAL),
+ *            J_atom / D_atom / K_atom [J] (ATOM_*)
+ *   axis:    DMI interface normal or anisotropy axis; normalised by the library as the
+ *            reference constructors do (dmi_field.cpp:43-47, uniaxial_anisotropy_field.cpp:11-12)
+ *   field:   homogeneous external field [A/m] (ZEEMAN / ATOM_ZEEMAN) when array0 is NULL
+ *   array0:  ZEEMAN / ATOM_ZEEMAN: host vector field [3*N] in A/m (overrides `field`);
+ *            RKKY: host scalar field of RKKY constants [N] (J/m, the reference's rkky_values)
+ *   array1:  RKKY: host scalar field of exchange constants A [N] (J/m)
+ *   index0:  RKKY: host scalar field of uint32 layer indices [N], or NULL for all zero
+ */
+typedef struct mafx_term_desc {
+    int32_t kind;
+    int32_t reserved;
+    double value;
+    double axis[3];
+    double field[3];
+    const double* array0;
+    const double* array1;
+    const uint32_t* index0;
+} mafx_term_desc;
+
+typedef struct mafx_ctx mafx_ctx;
+typedef struct mafx_llg mafx_llg;
+
+/* ------------------------------------------------------------------ errors / device info */
+const char* mafx_last_error(void);
+/* Number of visible devices (0 without a GPU).  Does not create a context. */
+int mafx_device_count(int* count);
+
+/* ------------------------------------------------------------------ context */
+/* Creates the evaluation context for one mesh, one material (Ms scalar, or Ms_field when
+ * Ms_field_host != NULL, State ctor semantics of state.cpp:71-111), one term list and one
+ * precision.  Uploads parameters, builds the demag / dipole tensor on the GPU in FP64
+ * (demag_field.cpp:104-147), plans the FFTs and allocates the workspace.  Replaces the
+ * construction of the reference's terms plus State::Ms/Ms_field. */
+int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_host,
+                    const mafx_term_desc* terms, int nterms, int precision, mafx_ctx** out);
+int mafx_ctx_destroy(mafx_ctx* ctx);
+/* Launch on an existing HIP stream (hipStream_t passed as void*); NULL selects the null stream. */
+int mafx_ctx_set_stream(mafx_ctx* ctx, void* stream);
+int mafx_ctx_sync(mafx_ctx* ctx);
+/* Device bytes held by the context (tensor, FFT workspace, parameters). */
+int mafx_ctx_device_bytes(const mafx_ctx* ctx, int64_t* bytes);
+
+/* ------------------------------------------------------------------ field and energy */
+/* H_eff = sum of the selected terms, in term order (field_term.hpp:116-123), A/m.
+ * term_mask bit i selects terms[i]; ~0u selects all.  h_dev is double* for MAFX_F64 and
+ * double* or float* for MAFX_F32_FIELD according to h_is_f32.  Replaces
+ * fieldterm::Heff_in_Apm and FieldTerm::H_in_Apm. */
+int mafx_heff(mafx_ctx* ctx, const double* m_dev, uint32_t term_mask, void* h_dev, int h_is_f32);
+/* Energy of every selected term (J) and their sum, each with the reference's formula:
+ * micro -mu0/2 Ms dV sum(h.m) (micro_term.hpp:18-30), exchange with the Neumann-corrected
+ * field (exchange_field.cpp:14-60), Zeeman without the 1/2 (external_field.cpp:38-48),
+ * atomistic -mu0/2 mu_s sum(h.m) (atom_term.hpp:12-15, atomistic_external_field.cpp:5-11).
+ * E_terms_host has nterms entries (0 for unselected terms).  Replaces
+ * FieldTerm::Energy_in_J and fieldterm::Eeff_in_J. */
+int mafx_energy(mafx_ctx* ctx, const double* m_dev, uint32_t term_mask, double* E_terms_host,
+                double* E_total_host);
+
+/* ------------------------------------------------------------------ vector helpers (FP64) */
+/* k = dt * LLG(alpha, m, h): -gamma/(1+alpha^2) [m x h + alpha m x (m x h)]
+ * (equations.hpp:19-33, llg_integrator.cpp:18-38); h is FP64. */
+int mafx_llg_rhs(mafx_ctx* ctx, double alpha, double dt, const double* m_dev, const double* h_dev,
+                 double* k_dev);
+/* sum(a*b) over 3N entries, FP64 fixed-order tree (util.cpp:30-32 full_inner_product). */
+int mafx_dot(mafx_ctx* ctx, const double* a_dev, const double* b_dev, double* out_host);
+/* max |a| over 3N entries (math.cpp:6-8 max_4d_abs). */
+int mafx_max_abs(mafx_ctx* ctx, const double* a_dev, double* out_host);
+/* In-place normalisation (util.cpp:34 normalize_even_zero_vectors when zero_safe == 0,
+ * util.cpp:37-42 normalize_handle_zero_vectors when zero_safe != 0). */
+int mafx_normalize(mafx_ctx* ctx, double* m_dev, int zero_safe);
+/* <m> over cells with |m| != 0 (state.cpp:32-45). */
+int mafx_mean_m(mafx_ctx* ctx, const double* m_dev, double out_host[3]);
+
+/* ------------------------------------------------------------------ adaptive RK / LLG */
+/* LLGIntegrator(alpha, terms, scheme, Controller(hmin, hmax, atol, rtol), dissipation_term_only)
+ * (llg_integrator.hpp:26-49, controller.hpp:11-17).  scheme: "DP45" or "RKF45"
+ * (adaptive_runge_kutta.cpp:75-173).  The integrator evaluates all terms of ctx. */
+int mafx_llg_create(mafx_ctx* ctx, double alpha, const char* scheme, double hmin, double hmax,
+                    double atol, double rtol, int dissipation_term_only, mafx_llg** out);
+int mafx_llg_destroy(mafx_llg* llg);
+int mafx_llg_set_alpha(mafx_llg* llg, double alpha);
+/* One accepted step of AdaptiveRungeKutta::step (adaptive_runge_kutta.cpp:39-71): repeats
+ * attempts until the controller accepts, then t += h, m += dm, renormalise.  m_dev is
+ * updated in place, *t_inout advanced. */
+int mafx_llg_step(mafx_llg* llg, double* m_dev, double* t_inout);
+/* Runs mafx_llg_step until *t_inout >= t_end or nsteps_max steps were taken. */
+int mafx_llg_run_until(mafx_llg* llg, double* m_dev, double* t_inout, double t_end,
+                       int64_t nsteps_max, int64_t* nsteps_done);
+/* LLGIntegrator::integrate_dense (llg_integrator.cpp:68-104): integrates time_in_s and writes
+ * (t, <mx>, <my>, <mz>) rows at every write_every_dt, the first row at the start time; the
+ * output sample comes from a backwards RKF45 step from the overshooting state.
+ * out_host holds max_rows*4 doubles; *rows_written is set. */
+int mafx_llg_integrate_dense(mafx_llg* llg, double* m_dev, double* t_inout, double time_in_s,
+                             double write_every_dt, double* out_host, int64_t max_rows,
+                             int64_t* rows_written);
+
+typedef struct mafx_llg_stats {
+    double h;                 /* next step size (AdaptiveRungeKutta::h_) */
+    double h_last;            /* step size of the last accepted step */
+    double err_last;          /* error norm of the last accepted attempt */
+    int64_t accepted, rejected, heff_evals;
+    int64_t counter_hmin, counter_hmax, counter_maxscale, counter_minscale;
+} mafx_llg_stats;
+int mafx_llg_get_stats(const mafx_llg* llg, mafx_llg_stats* out);
+/* The last accepted attempt's error estimate sequence lives on the host; this restores
+ * controller state for an exact resume (h, errold, reject flag). */
+int mafx_llg_set_controller_state(mafx_llg* llg, double h, double errold, int reject);
+int mafx_llg_get_controller_state(const mafx_llg* llg, double* h, double* errold, int* reject);
+
+/* ------------------------------------------------------------------ LBFGS */
+/* LBFGS_Minimizer(terms, tolerance, maxIter, verbose)::Minimize (lbfgs_minimizer.cpp:55-198):
+ * L-BFGS (history 5) with the More-Thuente line search on the sphere (cvsrch/cstep,
+ * :213-505).  m_dev is relaxed in place; returns the final energy in *E_out.
+ * torque_tol > 0 adds a stop when max|m x H| / max|H| <= torque_tol (checked after each
+ * iteration; the reference has no such stop, torque_tol <= 0 reproduces it exactly). */
+typedef struct mafx_lbfgs_stats {
+    int64_t iterations, energy_evals;
+    double final_energy, final_torque_rel;
+    int32_t stop_reason; /* 0 maxiter, 1 gradient, 2 triple test, 3 torque, 4 no descent */
+} mafx_lbfgs_stats;
+int mafx_lbfgs_minimize(mafx_ctx* ctx, double* m_dev, double tolerance, int64_t max_iter,
+                        int verbose, double torque_tol, double* E_out, mafx_lbfgs_stats* stats);
+
+/* ------------------------------------------------------------------ profiling */
+/* When enabled, every kernel launch is bracketed by HIP events on the context stream and
+ * its duration accumulated per kernel name. */
+int mafx_prof_enable(mafx_ctx* ctx, int on);
+int mafx_prof_reset(mafx_ctx* ctx);
+/* Writes "name count total_ms bytes_per_launch\n" lines into buf (NUL-terminated). */
+int mafx_prof_report(mafx_ctx* ctx, char* buf, int64_t buflen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAFX_H */

@@ -1,0 +1,439 @@
+"""Pins the FP64 CPU oracle against the reference's own known answers and against closed forms.
+
+Every test cites the reference test or source it reproduces.  These run on the CPU only.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+MU0 = O.MU0
+
+
+# ---------------------------------------------------------------- FFT vs numpy and brute force
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 12, 16, 20, 22, 24, 50, 64, 143, 200, 256, 2048])
+def test_fft_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = np.fft.fft(x)
+    got = O.fft(x, -1)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * max(1.0, np.max(np.abs(ref)))
+    back = O.fft(got, +1) / n
+    assert np.max(np.abs(back - x)) <= 1e-14 * max(1.0, np.max(np.abs(x)))
+
+
+def test_fft_brute_force_dft():
+    n = 30
+    x = np.random.default_rng(3).standard_normal(n) + 0j
+    k = np.arange(n)
+    W = np.exp(-2j * np.pi * np.outer(k, k) / n)
+    assert np.max(np.abs(O.fft(x) - W @ x)) < 1e-12
+
+
+# ---------------------------------------------------------------- constants (python/test/constants.py:8-10)
+def test_constants():
+    assert MU0 == 4e-7 * math.pi
+    gamma = 1.760859644e11 * MU0
+    assert gamma == 221276.14886372554
+
+
+# ---------------------------------------------------------------- demag
+def _uniform_x(nx, ny, nz):
+    m = np.zeros((nx, ny, nz, 3))
+    m[..., 0] = 1
+    return m
+
+
+def test_demag_energy_homogeneous_cube():
+    # magnumaf/test/micro_demag.cpp:6-29: E = mu0 Ms^2 V / 6 within 5.3e-8
+    x = y = z = 1e-9
+    nx, ny, nz = 8, 10, 12
+    s = O.Sim((nx, ny, nz, x / nx, y / ny, z / nz), 1e5, _uniform_x(nx, ny, nz), [O.term(O.DEMAG)])
+    E, _ = s.energy()
+    analytic = 1.0 / 6.0 * x * y * z * 1e5**2 * MU0
+    assert abs(E - analytic) <= (E + analytic) / 2 * 5.3e-8
+
+
+def test_demag_energy_homogeneous_cube_in_airbox():
+    # magnumaf/test/micro_demag.cpp:31-55 (zero-|m| cells build Ms_field, state.cpp:71-89)
+    x = y = z = 1e-9
+    nx, ny, nz = 12, 10, 8
+    m = np.zeros((2 * nx, 2 * ny, 2 * nz, 3))
+    m[nx // 2:2 * nx - nx // 2, ny // 2:2 * ny - ny // 2, nz // 2:2 * nz - nz // 2, 0] = 1
+    s = O.Sim((2 * nx, 2 * ny, 2 * nz, x / nx, y / ny, z / nz), 1e5, m, [O.term(O.DEMAG)])
+    assert s.has_ms_field
+    E, _ = s.energy()
+    analytic = 1.0 / 6.0 * x * y * z * 1e5**2 * MU0
+    assert abs(E - analytic) <= (E + analytic) / 2 * 5.3e-8
+
+
+def test_demag_energy_cube_python():
+    # python/test/micro_demag.py:10-16 (assertAlmostEqual: 7 decimals on ~1e-22 J is trivially met;
+    # the stricter 5.3e-8 relative bound of the C++ test is used)
+    nx, dx = 10, 1e-10
+    s = O.Sim((nx, nx, nx, dx, dx, dx), 1e5, _uniform_x(nx, nx, nx), [O.term(O.DEMAG)])
+    E, _ = s.energy()
+    analytic = 1.0 / 6.0 * (nx * dx) ** 3 * 1e5**2 * MU0
+    assert abs(E - analytic) <= analytic * 5.3e-8
+
+
+def _brute_force_demag(mesh, Ms, m_flat):
+    """H_i = sum_j N(i - j) M_j with the real-space tensor, no FFT."""
+    nx, ny, nz = mesh[:3]
+    N = O.demag_tensor(mesh)
+    n0, n1 = 2 * nx, 2 * ny
+    n2 = 1 if nz == 1 else 2 * nz
+    M = (Ms * m_flat).reshape(3, nz, ny, nx)
+    comp = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+    H = np.zeros((3, nz, ny, nx))
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                for zj in range(nz):
+                    for yj in range(ny):
+                        for xj in range(nx):
+                            i0, i1, i2 = (x - xj) % n0, (y - yj) % n1, (z - zj) % n2
+                            for a in range(3):
+                                H[a, z, y, x] += sum(N[comp[a][b], i2, i1, i0] * M[b, zj, yj, xj] for b in range(3))
+    return H.ravel()
+
+
+@pytest.mark.parametrize("mesh", [(5, 4, 3, 1e-9, 1.2e-9, 0.8e-9), (7, 3, 1, 2e-9, 1e-9, 3e-9)])
+def test_demag_fft_equals_direct_sum(mesh):
+    rng = np.random.default_rng(11)
+    nx, ny, nz = mesh[:3]
+    m = rng.standard_normal((nx, ny, nz, 3))
+    s = O.Sim(mesh, 8e5, m, [O.term(O.DEMAG)])
+    h = s.heff_flat()
+    ref = _brute_force_demag(mesh, 8e5, s.m_flat())
+    assert np.max(np.abs(h - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_newell_self_and_trace():
+    # Self demag of a cube: Nxx = Nyy = Nzz = -1/3 (sign: h = N * M); off-diagonal 0.
+    d = 2e-9
+    assert abs(O.lib().oracle_newell_nxx(0, 0, 0, d, d, d) + 1.0 / 3.0) < 1e-13
+    assert abs(O.lib().oracle_newell_nxy(0, 0, 0, d, d, d)) < 1e-13
+    # Trace vanishes away from the source cell (div of a dipole field), within cancellation.
+    for (i, j, k) in [(1, 0, 0), (2, 1, 0), (3, 2, 1)]:
+        xx = O.lib().oracle_newell_nxx(i, j, k, d, d, d)
+        yy = O.lib().oracle_newell_nxx(j, k, i, d, d, d)
+        zz = O.lib().oracle_newell_nxx(k, i, j, d, d, d)
+        assert abs(xx + yy + zz) < 1e-10 * max(abs(xx), abs(yy), abs(zz))
+    # Far field approaches the point dipole: N_xx = V (3 x^2 / r^5 - 1 / r^3) / (4 pi)
+    i, j, k = 12, 5, 3
+    x, y, z = i * d, j * d, k * d
+    r = math.sqrt(x * x + y * y + z * z)
+    dip = d**3 * (3 * x * x / r**5 - 1 / r**3) / (4 * math.pi)
+    assert abs(O.lib().oracle_newell_nxx(i, j, k, d, d, d) - dip) < 2e-3 * abs(dip)
+    dxy = d**3 * (3 * x * y / r**5) / (4 * math.pi)
+    assert abs(O.lib().oracle_newell_nxy(i, j, k, d, d, d) - dxy) < 2e-3 * abs(dxy)
+
+
+def test_newell_symmetry():
+    # Nxx even in every axis, Nxy odd in x and y, even in z, up to the cancellation rounding of
+    # the 27-term second difference (already ~1e-12 relative at offsets of a few cells)
+    d = (1e-9, 1.3e-9, 0.7e-9)
+    nxx = O.lib().oracle_newell_nxx
+    nxy = O.lib().oracle_newell_nxy
+    for (i, j, k) in [(1, 2, 3), (4, 1, 2)]:
+        ref = nxx(i, j, k, *d)
+        for s in [(-i, j, k), (i, -j, k), (i, j, -k), (-i, -j, -k)]:
+            assert abs(nxx(*s, *d) - ref) <= 1e-10 * abs(ref)
+        ref = nxy(i, j, k, *d)
+        assert abs(nxy(-i, j, k, *d) + ref) <= 1e-10 * abs(ref)
+        assert abs(nxy(i, -j, k, *d) + ref) <= 1e-10 * abs(ref)
+        assert abs(nxy(i, j, -k, *d) - ref) <= 1e-10 * abs(ref)
+
+
+# ---------------------------------------------------------------- exchange
+def test_exchange_uniform_field_and_energy():
+    # convolve_old keeps the self term: interior h = 0 for uniform m, boundary h = -m/d^2 * 2A/(mu0 Ms)
+    nx, ny, nz, d = 6, 5, 4, 1e-9
+    A, Ms = 1.3e-11, 8e5
+    s = O.Sim((nx, ny, nz, d, d, d), Ms, _uniform_x(nx, ny, nz), [O.term(O.EXCHANGE, A)])
+    h = s.heff()
+    c = 2 * A / (MU0 * Ms)
+    assert np.max(np.abs(h[1:-1, 1:-1, 1:-1])) < 1e-6 * c / d**2
+    assert abs(h[0, 2, 2, 0] - (-c / d**2)) < 1e-9 * c / d**2
+    assert abs(h[0, 0, 0, 0] - (-3 * c / d**2)) < 1e-9 * c / d**2
+    # Neumann-corrected energy of a uniform state is zero (exchange_field.cpp:14-60)
+    E, _ = s.energy()
+    assert abs(E) < 1e-30
+
+
+def test_exchange_linear_profile_interior_zero():
+    nx, ny, nz, d = 8, 3, 3, 2e-9
+    m = np.zeros((nx, ny, nz, 3))
+    m[..., 0] = 1.0
+    m[..., 1] = 0.01 * np.arange(nx)[:, None, None]
+    s = O.Sim((nx, ny, nz, d, d, d), 1e6, m, [O.term(O.EXCHANGE, 1e-11)])
+    # m is renormalised, so test the convolution on the stored m directly
+    mm = s.m()
+    h = s.heff()
+    c = 2 * 1e-11 / (MU0 * 1e6)
+    lap = (mm[2:, 1, 1] - 2 * mm[1:-1, 1, 1] + mm[:-2, 1, 1]) / d**2
+    assert np.max(np.abs(h[1:-1, 1, 1] - c * lap)) < 1e-9 * c / d**2
+
+
+def test_exchange_single_layer_self_term():
+    # nz == 1: zero padding in z leaves -2 m / dz^2 in h (no neighbours)
+    nx, ny, d, dz = 3, 3, 1e-9, 3e-9
+    s = O.Sim((nx, ny, 1, d, d, dz), 1e6, _uniform_x(nx, ny, 1), [O.term(O.EXCHANGE, 1e-11)])
+    h = s.heff()
+    c = 2 * 1e-11 / (MU0 * 1e6)
+    assert abs(h[1, 1, 0, 0] - c * (-2 / dz**2)) < 1e-9 * c / dz**2
+
+
+# ---------------------------------------------------------------- anisotropy (python/test/micro_anisotropy.py:36-54)
+def test_uniaxial_field_values():
+    mesh = (1, 1, 1, 1, 1, 1)
+    s = O.Sim(mesh, 1.0, np.array([1.0, 0, 0]).reshape(1, 1, 1, 3), [O.term(O.UNIAXIAL, 1.0, axis=(1, 0, 0))])
+    assert s.heff()[0, 0, 0, 0] == 2 / MU0
+    s = O.Sim(mesh, 1.0, np.array([0, 1.0, 0]).reshape(1, 1, 1, 3), [O.term(O.UNIAXIAL, 1.0, axis=(0, -1, 0))])
+    h = s.heff()[0, 0, 0]
+    assert h[0] == 0 and h[1] == 2 / MU0 and h[2] == 0
+    # E = -Ku (u.m)^2 V
+    E, _ = s.energy()
+    assert abs(E - (-1.0)) < 1e-15
+
+
+# ---------------------------------------------------------------- Zeeman (python/test/external_field.py:8-30)
+def test_zeeman_field_and_energy():
+    mesh = (3, 1, 1, 1e-10, 1e-10, 1e-10)
+    z = np.zeros((3, 1, 1, 3))
+    z[0, 0, 0, 0] = 1
+    z[1, 0, 0, 1] = 1
+    z[2, 0, 0, 2] = 1
+    m = np.zeros((3, 1, 1, 3))
+    m[..., 0] = 1
+    s = O.Sim(mesh, 2.0, m, [O.term(O.ZEEMAN, array0=O.to_flat(z))])
+    assert np.array_equal(s.heff(), z)
+    E, _ = s.energy()
+    assert abs(E - (-MU0 * 2.0 * 1.0 * 1e-30)) < 1e-45  # no 1/2 (external_field.cpp:38-48)
+
+
+# ---------------------------------------------------------------- interface DMI (micro): closed form on a 3-cell chain
+def test_dmi_three_cell_chain():
+    d = 1e-9
+    D, Ms = 3e-3, 1e6
+    m = np.zeros((3, 1, 1, 3))
+    m[0, 0, 0] = [0, 0, 1]
+    m[1, 0, 0] = [1, 0, 0]
+    m[2, 0, 0] = [0, 0, -1]
+    s = O.Sim((3, 1, 1, d, d, d), Ms, m, [O.term(O.DMI, D, axis=(0, 0, 1))])
+    h = s.heff()
+    c = 2 * D / (MU0 * Ms)
+    # cell 1: div m = (mx[2]-mx[0])/2d = 0; d(n.m)/dx = (mz[2]-mz[0])/2d = -1/d
+    assert np.allclose(h[1, 0, 0], [c / d, 0, 0], rtol=1e-15, atol=0)
+    # cell 0: div m = (mx[1] - 0)/2d = 1/(2d) -> n*div = (0,0,1/(2d)); grad(n.m)_x = (mz[1] - 0)/2d = 0
+    assert np.allclose(h[0, 0, 0], [0, 0, c / (2 * d)], rtol=1e-15, atol=0)
+    # cell 2: div m = (0 - mx[1])/2d = -1/(2d); grad(n.m)_x = (0 - mz[1])/2d = 0
+    assert np.allclose(h[2, 0, 0], [0, 0, -c / (2 * d)], rtol=1e-15, atol=0)
+
+
+def test_dmi_uniform_interior_zero():
+    m = np.zeros((5, 5, 1, 3))
+    m[..., 2] = 1
+    s = O.Sim((5, 5, 1, 1e-9, 1e-9, 1e-9), 1e6, m, [O.term(O.DMI, 3e-3)])
+    assert np.max(np.abs(s.heff()[1:-1, 1:-1])) == 0
+
+
+# ---------------------------------------------------------------- atomistic (python/test/atomistic_*.py)
+def _two(m0, m1, shape=(2, 1, 1)):
+    m = np.zeros(shape + (3,))
+    m.reshape(-1, 3)[0] = m0
+    m.reshape(-1, 3)[1] = m1
+    return m
+
+
+@pytest.mark.parametrize("m1,E,h0,h1", [
+    ((0, 0, 1), -1.0, (0, 0, 1), (0, 0, 1)),
+    ((1, 0, 0), 0.0, (1, 0, 0), (0, 0, 1)),
+    ((0, 0, -1), 1.0, (0, 0, -1), (0, 0, 1)),
+])
+def test_atomistic_exchange(m1, E, h0, h1):
+    # python/test/atomistic_exchange.py:16-97
+    p, J, dx = 1e-12, 1e-12, 2.715e-10
+    s = O.Sim((2, 1, 1, dx, dx, dx), p, _two((0, 0, 1), m1), [O.term(O.ATOM_EXCHANGE, J)])
+    assert abs(s.energy()[0] - E * J) < 1e-7 * J
+    h = s.heff()
+    c = J / MU0 / p
+    assert np.allclose(h[0, 0, 0], np.array(h0) * c, rtol=1e-15)
+    assert np.allclose(h[1, 0, 0], np.array(h1) * c, rtol=1e-15)
+
+
+def test_atomistic_dmi():
+    # python/test/atomistic_dmi.py:18-105
+    p, D, dx = 1e-20, 1e20, 1.1
+    c = D / MU0 / p
+    s = O.Sim((2, 1, 1, dx, dx, dx), p, _two((0, 0, 1), (0, 0, 1)), [O.term(O.ATOM_DMI, D, axis=(0, 0, 1))])
+    assert abs(s.energy()[0]) < 1e-7
+    h = s.heff()
+    assert abs((h[0, 0, 0, 0] + c) / h[0, 0, 0, 0]) < 1e-15 and h[0, 0, 0, 1] == 0 and h[0, 0, 0, 2] == 0
+    assert abs((h[1, 0, 0, 0] - c) / h[1, 0, 0, 0]) < 1e-15
+    s = O.Sim((2, 1, 1, dx, dx, dx), p, _two((0, 0, 1), (1, 0, 0)), [O.term(O.ATOM_DMI, D, axis=(0, 0, 1))])
+    assert abs(s.energy()[0] - (-D)) < 1e-7 * D
+    h = s.heff()
+    assert abs((h[0, 0, 0, 2] - c) / h[0, 0, 0, 2]) < 1e-15 and h[0, 0, 0, 0] == 0
+    assert abs((h[1, 0, 0, 0] - c) / h[1, 0, 0, 0]) < 1e-15
+    s = O.Sim((1, 1, 2, dx, dx, dx), p, _two((0, 0, 1), (1, 0, 0), (1, 1, 2)),
+              [O.term(O.ATOM_DMI, D, axis=(0, 0, 1))])
+    assert abs(s.energy()[0]) < 1e-7
+    assert np.max(np.abs(s.heff())) < 1e-7
+
+
+@pytest.mark.parametrize("m1,E,h1", [((0, 0, 1), -2.0, (0, 0, 2)), ((1, 0, 0), -1.0, (0, 0, 0))])
+def test_atomistic_anisotropy(m1, E, h1):
+    # python/test/atomistic_anisotropy.py:16-60
+    p, k, dx = 1e-20, 1e-20, 2.715e-10
+    s = O.Sim((2, 1, 1, dx, dx, dx), p, _two((0, 0, 1), m1), [O.term(O.ATOM_UNIAXIAL, k, axis=(0, 0, 1))])
+    assert abs(s.energy()[0] - E * k) < 1e-7 * k
+    h = s.heff()
+    assert np.allclose(h[0, 0, 0], [0, 0, 2 * k / MU0 / p], rtol=1e-15)
+    assert np.allclose(h[1, 0, 0], np.array(h1) * k / MU0 / p, rtol=1e-15, atol=0)
+
+
+def test_atomistic_dipole_dipole():
+    # python/test/atomistic_dipole_dipole.py:12-152; the reference compares energies exactly,
+    # FFT rounding is allowed here (relative 1e-12).
+    p, dx = 9.3e-24, 2.7e-10
+    c = p / 4.0 / math.pi / dx**3
+    mesh = (2, 1, 1, dx, dx, dx)
+    s = O.Sim(mesh, p, _two((0, 0, 1), (0, 0, 1)), [O.term(O.ATOM_DIPOLE)])
+    Eref = p**2 * MU0 / (4.0 * math.pi) / dx**3
+    assert abs(s.energy()[0] - Eref) < 1e-12 * Eref
+    h = s.heff()
+    assert np.allclose(h[0, 0, 0], [0, 0, -c], rtol=1e-12, atol=1e-12 * c)
+    assert np.allclose(h[1, 0, 0], [0, 0, -c], rtol=1e-12, atol=1e-12 * c)
+    s = O.Sim(mesh, p, _two((0, 0, 1), (1, 0, 0)), [O.term(O.ATOM_DIPOLE)])
+    assert abs(s.energy()[0]) < 1e-12 * Eref
+    h = s.heff()
+    assert np.allclose(h[0, 0, 0], [2 * c, 0, 0], rtol=1e-12, atol=1e-12 * c)
+    assert np.allclose(h[1, 0, 0], [0, 0, -c], rtol=1e-12, atol=1e-12 * c)
+    s = O.Sim(mesh, p, _two((0, 0, 1), (0, 0, -1)), [O.term(O.ATOM_DIPOLE)])
+    assert abs(s.energy()[0] + Eref) < 1e-12 * Eref
+    s = O.Sim((1, 2, 1, dx, dx, dx), p, _two((0, 0, 1), (0, 0, 1), (1, 2, 1)), [O.term(O.ATOM_DIPOLE)])
+    assert abs(s.energy()[0] - Eref) < 1e-12 * Eref
+    s = O.Sim((1, 2, 1, dx, dx, dx), p, _two((1, 0, 0), (0, 0, 1), (1, 2, 1)), [O.term(O.ATOM_DIPOLE)])
+    assert abs(s.energy()[0]) < 1e-12 * Eref
+    s = O.Sim(mesh, p, _two((1, 0, 0), (1, 0, 0)), [O.term(O.ATOM_DIPOLE)])
+    assert abs(s.energy()[0] - (-p**2 * MU0 / (2.0 * math.pi) / dx**3)) < 1e-12 * Eref
+    h = s.heff()
+    assert np.allclose(h[:, 0, 0, 0], [p / 2.0 / math.pi / dx**3] * 2, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- RKKY + demag (magnumaf/test/rkky.cpp:11-55)
+def test_rkky_mumax3_energy_sweep():
+    nx, ny, nz, dx = 10, 10, 2, 1e-9
+    Ms, A, RK = 1e6, 10e-12, -1e-3 * 1e-9
+    N = nx * ny * nz
+    m = np.zeros((nx, ny, nz, 3))
+    m[..., 0] = 1
+    s = O.Sim((nx, ny, nz, dx, dx, dx), Ms, m,
+              [O.term(O.DEMAG), O.term(O.RKKY, array0=np.full(N, RK / 2.0), array1=np.full(N, A))])
+    Es = []
+    for i in range(360):
+        mm = m.copy()
+        mm[:, :, 1, 0] = math.cos(i * math.pi / 180.0)
+        mm[:, :, 1, 1] = math.sin(i * math.pi / 180.0)
+        s.set_m(mm, normalize=False)
+        Es.append(s.energy()[0])
+    assert abs((max(Es) - min(Es)) - 2.13934e-19) < 5e-26
+
+
+# ---------------------------------------------------------------- RK tableaux (magnumaf/test/adaptive_runge_kutta.cpp:9-62)
+@pytest.mark.parametrize("scheme,steps,tol,ctol", [("BS23", 200, 1e-12, 1e-10), ("DP45", 100, 1e-8, 1e-10),
+                                                   ("RKF45", 100, 1e-8, 1e-10)])
+def test_adaptive_rk_analytic(scheme, steps, tol, ctol):
+    m = np.zeros((1, 1, 1, 3))
+    m[0, 0, 0, 0] = 1
+    s = O.Sim((1, 1, 1, 0, 0, 0), 1.0, m, [O.term(O.ZEEMAN)])
+    s.llg_init(0.0, scheme, hmin=1e-15, hmax=1e15, atol=ctol, rtol=ctol, normalize=False, analytic_rhs=True)
+    for _ in range(steps):
+        s.step()
+        t = s.t
+        assert abs(s.m_flat()[0] - (t * t + 4) ** 2 / 16.0) < tol
+
+
+# ---------------------------------------------------------------- SP4 (python/test/sp4.py:21-62)
+def test_sp4_relaxation_and_switch():
+    nx, ny, nz = 100, 25, 1
+    mesh = (nx, ny, nz, 5.0e-7 / 100, 1.25e-7 / 25, 3.0e-9)
+    m = np.zeros((nx, ny, nz, 3))
+    m[1:-1, :, :, 0] = 1
+    m[0, :, :, 1] = 1
+    m[-1, :, :, 1] = 1
+    s = O.Sim(mesh, 8e5, m, [O.term(O.DEMAG), O.term(O.EXCHANGE, 1.3e-11)])
+    s.llg_init(1.0, "RKF45")
+    ix = iy = iz = 0.0
+    while s.t < 1e-9:
+        t1 = s.t
+        s.step()
+        dt = s.t - t1
+        mm = s.mean_m()
+        ix += mm[0] * dt
+        iy += mm[1] * dt
+        iz += mm[2] * dt
+    assert abs(ix - 9.81206172824e-10) < 1e-15
+    assert abs(iy - 9.14350283169e-11) < 1e-15
+    assert abs(iz + 5.74381785359e-13) < 1e-15
+    s.set_alpha(0.02)
+    z = np.zeros((nx, ny, nz, 3))
+    z[..., 0] = -24.6e-3 / MU0
+    z[..., 1] = +4.3e-3 / MU0
+    s.add_term(O.term(O.ZEEMAN, array0=O.to_flat(z)))
+    ix = iy = iz = 0.0
+    while s.t < 2e-9:
+        t1 = s.t
+        s.step()
+        dt = s.t - t1
+        mm = s.mean_m()
+        ix += mm[0] * dt
+        iy += mm[1] * dt
+        iz += mm[2] * dt
+    assert abs(ix + 6.41261165705e-10) < 1e-15
+    assert abs(iy - 1.47353233738e-10) < 1e-15
+    assert abs(iz + 1.78144535231e-11) < 1e-15
+
+
+# ---------------------------------------------------------------- state semantics (magnumaf/test/state.cpp:38-135)
+def test_state_normalize_and_mean():
+    s = O.Sim((1, 1, 1, 1e-9, 1e-9, 1e-9), 1.0, np.ones((1, 1, 1, 3)), [O.term(O.ZEEMAN)])
+    assert np.all(s.m()[0, 0, 0] == 1 / math.sqrt(3))
+    m0 = np.zeros((2, 2, 2, 3))
+    m0[0, :, :, 1] = 1
+    s = O.Sim((2, 2, 2, 1e-9, 1e-9, 1e-9), 1e6, m0, [O.term(O.ZEEMAN)])
+    assert s.has_ms_field
+    assert np.array_equal(s.mean_m(), [0, 1, 0])
+
+
+# ---------------------------------------------------------------- controller clamp (controller.cpp:77-88)
+def test_controller_hmin_clamp_accepts():
+    # a huge field with a tiny hmax/hmin window forces the clamp-accept branch
+    m = np.zeros((2, 1, 1, 3))
+    m[..., 0] = 1
+    m[1, 0, 0] = [0, 1, 0]
+    s = O.Sim((2, 1, 1, 1e-9, 1e-9, 1e-9), 8e5, m, [O.term(O.ZEEMAN, field=(0, 0, 1e12))])
+    s.llg_init(0.5, "DP45", hmin=1e-13, hmax=2e-13, atol=1e-12, rtol=1e-12)
+    r = s.step()
+    assert r["h"] in (1e-13, 2e-13) or r["h"] <= 2e-13
+
+
+# ---------------------------------------------------------------- LBFGS: known minimum
+def test_lbfgs_aligns_with_field():
+    # Zeeman + exchange: the minimum is the uniform state along the field, E = -mu0 Ms H V
+    nx, ny, nz, d = 4, 4, 1, 2e-9
+    rng = np.random.default_rng(5)
+    m = rng.standard_normal((nx, ny, nz, 3))
+    m[..., 2] = np.abs(m[..., 2]) + 0.5
+    H = 1e5
+    s = O.Sim((nx, ny, nz, d, d, d), 8e5, m, [O.term(O.EXCHANGE, 1e-11), O.term(O.ZEEMAN, field=(0, 0, H))])
+    E0, _ = s.energy()
+    r = s.lbfgs(tol=1e-10, maxiter=500)
+    E_min = -MU0 * 8e5 * H * nx * ny * nz * d**3
+    assert r["E"] < E0
+    assert abs(r["E"] - E_min) < 1e-8 * abs(E_min)
+    assert np.min(s.m()[..., 2]) > 1 - 1e-6
