@@ -105,6 +105,12 @@ int mafx_ctx_sync(mafx_ctx* ctx);
 /* Device bytes held by the context (tensor, FFT workspace, parameters). */
 int mafx_ctx_device_bytes(const mafx_ctx* ctx, int64_t* bytes);
 
+/* Real-space Newell demag tensor on the expanded grid, [6][n2][n1][n0] with n0 = 2 nx,
+ * n1 = 2 ny, n2 = (nz == 1 ? 1 : 2 nz), components (xx, xy, xz, yy, yz, zz), evaluated on the host
+ * in the reference's operation order.  Replaces DemagField::calc_N (demag_field.cpp:104-142,207-209).
+ * nthreads <= 0 uses every hardware thread.  Needs no GPU. */
+int mafx_demag_calc_N(const mafx_mesh* mesh, double* N_host, int nthreads);
+
 /* ------------------------------------------------------------------ field and energy */
 /* H_eff = sum of the selected terms, in term order (field_term.hpp:116-123), A/m.
  * term_mask bit i selects terms[i]; ~0u selects all.  h_dev is double* for MAFX_F64 and

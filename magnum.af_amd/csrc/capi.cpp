@@ -1,0 +1,421 @@
+// extern "C" entry points of libmafx (declared in include/mafx.h): context construction, term
+// upload, field / energy evaluation, vector helpers, the LLG integrator, LBFGS and profiling.
+// No exception crosses the boundary: each entry point returns a status and records the message.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "fft_kernels.hpp"
+#include "mafx_internal.hpp"
+#include "mafx_llg.hpp"
+#include "vec_kernels.hpp"
+
+namespace mafx {
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F> int guarded(F&& f) {
+    try {
+        f();
+        return MAFX_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return MAFX_ERR_DEVICE;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return MAFX_ERR_DEVICE;
+    }
+}
+
+// dmi_field.cpp:43-47, util.hpp:62-69, atomistic_dmi_field.cpp:6-9: v / sqrt(v0^2 + v1^2 + v2^2)
+void normalize_axis(const double in[3], double out[3]) {
+    const double n = std::sqrt(std::pow(in[0], 2) + std::pow(in[1], 2) + std::pow(in[2], 2));
+    require(n > 0, "term axis must be non-zero");
+    for (int c = 0; c < 3; ++c) out[c] = in[c] / n;
+}
+
+void upload(DevBuf& b, const void* host, size_t bytes) {
+    b.alloc(bytes);
+    MAFX_HIP(hipMemcpy(b.p, host, bytes, hipMemcpyHostToDevice));
+}
+
+} // namespace
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+hipEvent_t Prof::get_event() {
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    MAFX_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void Prof::flush() {
+    for (size_t i = 0; i < pending.size(); ++i) {
+        auto& p = pending[i];
+        MAFX_HIP(hipEventSynchronize(p.second.second));
+        float ms = 0;
+        MAFX_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        Entry& e = entries[p.first];
+        e.count++;
+        e.ms += ms;
+        e.bytes = pending_bytes[i];
+        pool.push_back(p.second.first);
+        pool.push_back(p.second.second);
+    }
+    pending.clear();
+    pending_bytes.clear();
+}
+
+Prof::~Prof() {
+    for (auto& p : pending) {
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+}
+
+ProfScope::ProfScope(mafx_ctx* c, const char* n, double by) : ctx(c), name(n), bytes(by) {
+    if (ctx->prof.on) {
+        b = ctx->prof.get_event();
+        MAFX_HIP(hipEventRecord(b, ctx->stream));
+    }
+}
+
+ProfScope::~ProfScope() {
+    if (!b) return;
+    hipEvent_t e = nullptr;
+    try {
+        e = ctx->prof.get_event();
+    } catch (...) {
+        return;
+    }
+    if (hipEventRecord(e, ctx->stream) != hipSuccess) return;
+    ctx->prof.pending.push_back({name, {b, e}});
+    ctx->prof.pending_bytes.push_back(bytes);
+    if (ctx->prof.pending.size() > 4096) {
+        try {
+            ctx->prof.flush();
+        } catch (...) {
+        }
+    }
+}
+
+} // namespace mafx
+
+mafx_ctx::~mafx_ctx() {
+    if (host_scal) (void)hipHostFree(host_scal);
+}
+
+using namespace mafx;
+
+extern "C" {
+
+const char* mafx_last_error(void) { return g_last_error.c_str(); }
+
+int mafx_device_count(int* count) {
+    return guarded([&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+    });
+}
+
+int mafx_demag_calc_N(const mafx_mesh* mesh, double* N_host, int nthreads) {
+    return guarded([&] {
+        require(mesh && N_host, "null argument");
+        require(mesh->nx > 0 && mesh->ny > 0 && mesh->nz > 0, "mesh sizes must be positive");
+        newell_tensor_host(*mesh, N_host, nthreads);
+    });
+}
+
+int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_host, const mafx_term_desc* terms,
+                    int nterms, int precision, mafx_ctx** out) {
+    return guarded([&] {
+        require(mesh && out, "null argument");
+        require(mesh->nx > 0 && mesh->ny > 0 && mesh->nz > 0, "mesh sizes must be positive");
+        require(mesh->dx > 0 && mesh->dy > 0 && mesh->dz > 0, "cell sizes must be positive");
+        require(nterms >= 1 && nterms <= 8, "between 1 and 8 terms per context");
+        require(precision == MAFX_F64 || precision == MAFX_F32_FIELD, "unknown precision");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+            throw Error(MAFX_ERR_DEVICE, "no HIP device visible: the MI355X path needs a GPU");
+        auto ctx = std::make_unique<mafx_ctx>();
+        ctx->mesh = *mesh;
+        ctx->N = mesh->nx * mesh->ny * mesh->nz;
+        ctx->Ms = Ms;
+        ctx->precision = precision;
+        const int64_t N = ctx->N;
+        if (Ms_field_host) {
+            upload(ctx->ms_field, Ms_field_host, sizeof(double) * N);
+            ctx->has_ms_field = true;
+        }
+        for (int i = 0; i < nterms; ++i) {
+            const mafx_term_desc& d = terms[i];
+            TermDev t;
+            t.kind = d.kind;
+            t.value = d.value;
+            for (int c = 0; c < 3; ++c) t.field[c] = d.field[c];
+            switch (d.kind) {
+            case MAFX_DEMAG:
+            case MAFX_ATOM_DIPOLE:
+                require(ctx->demag_term < 0, "at most one demag / dipole term per context");
+                ctx->demag_term = i;
+                ctx->atom_dipole = d.kind == MAFX_ATOM_DIPOLE;
+                break;
+            case MAFX_EXCHANGE:
+            case MAFX_ATOM_EXCHANGE: break;
+            case MAFX_DMI:
+            case MAFX_UNIAXIAL:
+            case MAFX_ATOM_DMI:
+            case MAFX_ATOM_UNIAXIAL: normalize_axis(d.axis, t.axis); break;
+            case MAFX_ZEEMAN:
+            case MAFX_ATOM_ZEEMAN:
+                if (d.array0) {
+                    upload(t.zee, d.array0, sizeof(double) * 3 * N);
+                    t.has_zee_array = true;
+                }
+                break;
+            case MAFX_RKKY:
+                require(d.array0 && d.array1, "RKKY needs rkky values (array0) and exchange values (array1)");
+                upload(t.rkky, d.array0, sizeof(double) * N);
+                upload(t.aex, d.array1, sizeof(double) * N);
+                if (d.index0) {
+                    upload(t.ridx, d.index0, sizeof(uint32_t) * N);
+                    t.has_ridx = true;
+                }
+                break;
+            default: throw Error(MAFX_ERR_ARG, "unknown term kind " + std::to_string(d.kind));
+            }
+            ctx->terms.push_back(std::move(t));
+        }
+        int64_t nblk = std::max<int64_t>(mesh->ny * mesh->nz, (N + 255) / 256);
+        nblk = std::max<int64_t>(nblk, reduction_partials_needed(3 * N));
+        ctx->partials.alloc(sizeof(double) * 10 * nblk);
+        ctx->scal.alloc(sizeof(double) * 64);
+        MAFX_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_scal), sizeof(double) * 64, hipHostMallocDefault));
+        ctx->hbuf.alloc(sizeof(double) * 3 * N);
+        if (ctx->demag_term >= 0) {
+            ctx->n0 = 2 * mesh->nx;
+            ctx->n1 = 2 * mesh->ny;
+            ctx->n2 = mesh->nz == 1 ? 1 : 2 * mesh->nz;
+            ctx->hx = ctx->n0 / 2 + 1;
+            ctx->px.build(static_cast<int>(ctx->n0));
+            ctx->py.build(static_cast<int>(ctx->n1));
+            if (ctx->n2 > 1) ctx->pz.build(static_cast<int>(ctx->n2));
+            const size_t csz = precision == MAFX_F64 ? sizeof(double2) : sizeof(float2);
+            ctx->spec.alloc(csz * 3 * ctx->n2 * ctx->n1 * ctx->hx);
+            build_tensor(ctx.get());
+        }
+        ctx->device_bytes = static_cast<int64_t>(ctx->ms_field.bytes + ctx->partials.bytes + ctx->scal.bytes +
+                                                 ctx->hbuf.bytes + ctx->spec.bytes + ctx->tensor.bytes);
+        for (auto& t : ctx->terms)
+            ctx->device_bytes += static_cast<int64_t>(t.zee.bytes + t.rkky.bytes + t.aex.bytes + t.ridx.bytes);
+        *out = ctx.release();
+    });
+}
+
+int mafx_ctx_destroy(mafx_ctx* ctx) {
+    return guarded([&] {
+        if (ctx) {
+            (void)hipStreamSynchronize(ctx->stream);
+            delete ctx;
+        }
+    });
+}
+
+int mafx_ctx_set_stream(mafx_ctx* ctx, void* stream) {
+    return guarded([&] {
+        require(ctx, "null context");
+        ctx->stream = static_cast<hipStream_t>(stream);
+    });
+}
+
+int mafx_ctx_sync(mafx_ctx* ctx) {
+    return guarded([&] { MAFX_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int mafx_ctx_device_bytes(const mafx_ctx* ctx, int64_t* bytes) {
+    return guarded([&] { *bytes = ctx->device_bytes; });
+}
+
+int mafx_heff(mafx_ctx* ctx, const double* m_dev, uint32_t term_mask, void* h_dev, int h_is_f32) {
+    return guarded([&] {
+        require(ctx && m_dev && h_dev, "null argument");
+        EvalArgs a;
+        a.m = m_dev;
+        a.mask = term_mask;
+        a.mode = OutMode::H;
+        a.out = h_dev;
+        a.out_f32 = h_is_f32 != 0;
+        evaluate(ctx, a);
+    });
+}
+
+int mafx_energy(mafx_ctx* ctx, const double* m_dev, uint32_t term_mask, double* E_terms_host, double* E_total_host) {
+    return guarded([&] {
+        require(ctx && m_dev, "null argument");
+        EvalArgs a;
+        a.m = m_dev;
+        a.mask = term_mask;
+        a.mode = OutMode::ENERGY;
+        a.want_energy = true;
+        evaluate(ctx, a);
+        finish_energy(ctx, term_mask, E_terms_host, E_total_host);
+    });
+}
+
+int mafx_llg_rhs(mafx_ctx* ctx, double alpha, double dt, const double* m_dev, const double* h_dev, double* k_dev) {
+    return guarded([&] { llg_rhs(ctx, alpha, dt, m_dev, h_dev, k_dev); });
+}
+
+int mafx_dot(mafx_ctx* ctx, const double* a_dev, const double* b_dev, double* out_host) {
+    return guarded([&] { *out_host = dot(ctx, a_dev, b_dev, 3 * ctx->N); });
+}
+
+int mafx_max_abs(mafx_ctx* ctx, const double* a_dev, double* out_host) {
+    return guarded([&] { *out_host = max_abs(ctx, a_dev, 3 * ctx->N); });
+}
+
+int mafx_normalize(mafx_ctx* ctx, double* m_dev, int zero_safe) {
+    return guarded([&] { normalize(ctx, m_dev, zero_safe != 0); });
+}
+
+int mafx_mean_m(mafx_ctx* ctx, const double* m_dev, double out_host[3]) {
+    return guarded([&] { mean_m(ctx, m_dev, out_host); });
+}
+
+int mafx_llg_create(mafx_ctx* ctx, double alpha, const char* scheme, double hmin, double hmax, double atol,
+                    double rtol, int dissipation_term_only, mafx_llg** out) {
+    return guarded([&] {
+        require(ctx && scheme && out, "null argument");
+        const std::string s(scheme);
+        require(s == "DP45" || s == "RKF45", "scheme must be \"DP45\" or \"RKF45\"");
+        auto L = std::make_unique<mafx_llg>();
+        L->ctx = ctx;
+        L->alpha = alpha;
+        L->scheme = s;
+        L->dissipation_only = dissipation_term_only != 0;
+        L->ctrl.hmin = hmin;
+        L->ctrl.hmax = hmax;
+        L->ctrl.atol = atol;
+        L->ctrl.rtol = rtol;
+        const size_t bytes = sizeof(double) * 3 * ctx->N;
+        for (int j = 1; j <= 7; ++j) L->k[j].alloc(bytes);
+        L->mtmp.alloc(bytes);
+        L->mtmp2.alloc(bytes);
+        L->sumbk.alloc(bytes);
+        *out = L.release();
+    });
+}
+
+int mafx_llg_destroy(mafx_llg* llg) {
+    return guarded([&] { delete llg; });
+}
+
+int mafx_llg_set_alpha(mafx_llg* llg, double alpha) {
+    return guarded([&] { llg->alpha = alpha; });
+}
+
+int mafx_llg_step(mafx_llg* llg, double* m_dev, double* t_inout) {
+    return guarded([&] {
+        require(llg && m_dev && t_inout, "null argument");
+        llg_step(llg, m_dev, t_inout);
+    });
+}
+
+int mafx_llg_run_until(mafx_llg* llg, double* m_dev, double* t_inout, double t_end, int64_t nsteps_max,
+                       int64_t* nsteps_done) {
+    return guarded([&] {
+        int64_t n = 0;
+        while (*t_inout < t_end && n < nsteps_max) {
+            llg_step(llg, m_dev, t_inout);
+            ++n;
+        }
+        if (nsteps_done) *nsteps_done = n;
+    });
+}
+
+int mafx_llg_integrate_dense(mafx_llg* llg, double* m_dev, double* t_inout, double time_in_s, double write_every_dt,
+                             double* out_host, int64_t max_rows, int64_t* rows_written) {
+    return guarded([&] {
+        require(write_every_dt > 0, "write_every_dt must be positive");
+        const int64_t r = llg_integrate_dense(llg, m_dev, t_inout, time_in_s, write_every_dt, out_host, max_rows);
+        if (rows_written) *rows_written = std::min(r, max_rows);
+    });
+}
+
+int mafx_llg_get_stats(const mafx_llg* llg, mafx_llg_stats* out) {
+    return guarded([&] {
+        out->h = llg->h;
+        out->h_last = llg->h_last;
+        out->err_last = llg->err_last;
+        out->accepted = llg->ctrl.counter_accepted;
+        out->rejected = llg->ctrl.counter_reject;
+        out->heff_evals = llg->heff_evals;
+        out->counter_hmin = llg->ctrl.counter_hmin;
+        out->counter_hmax = llg->ctrl.counter_hmax;
+        out->counter_maxscale = llg->ctrl.counter_maxscale;
+        out->counter_minscale = llg->ctrl.counter_minscale;
+    });
+}
+
+int mafx_llg_set_controller_state(mafx_llg* llg, double h, double errold, int reject) {
+    return guarded([&] {
+        llg->h = h;
+        llg->ctrl.errold = errold;
+        llg->ctrl.reject = reject != 0;
+    });
+}
+
+int mafx_llg_get_controller_state(const mafx_llg* llg, double* h, double* errold, int* reject) {
+    return guarded([&] {
+        *h = llg->h;
+        *errold = llg->ctrl.errold;
+        *reject = llg->ctrl.reject ? 1 : 0;
+    });
+}
+
+int mafx_lbfgs_minimize(mafx_ctx* ctx, double* m_dev, double tolerance, int64_t max_iter, int verbose,
+                        double torque_tol, double* E_out, mafx_lbfgs_stats* stats) {
+    int rc = MAFX_OK;
+    const int g = guarded([&] { rc = lbfgs_minimize(ctx, m_dev, tolerance, max_iter, verbose, torque_tol, E_out, stats); });
+    return g != MAFX_OK ? g : rc;
+}
+
+int mafx_prof_enable(mafx_ctx* ctx, int on) {
+    return guarded([&] { ctx->prof.on = on != 0; });
+}
+
+int mafx_prof_reset(mafx_ctx* ctx) {
+    return guarded([&] {
+        ctx->prof.flush();
+        ctx->prof.entries.clear();
+    });
+}
+
+int mafx_prof_report(mafx_ctx* ctx, char* buf, int64_t buflen) {
+    return guarded([&] {
+        ctx->prof.flush();
+        std::string s;
+        char line[256];
+        for (auto& kv : ctx->prof.entries) {
+            std::snprintf(line, sizeof(line), "%s %lld %.6f %.0f\n", kv.first.c_str(),
+                          static_cast<long long>(kv.second.count), kv.second.ms, kv.second.bytes);
+            s += line;
+        }
+        require(buflen > static_cast<int64_t>(s.size()), "report buffer too small");
+        std::memcpy(buf, s.c_str(), s.size() + 1);
+    });
+}
+
+} // extern "C"

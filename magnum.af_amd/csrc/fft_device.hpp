@@ -1,0 +1,149 @@
+// LDS-staged Stockham autosort FFT for gfx950.
+//
+// A work-group transforms W lines of length L held in LDS in [pos][w] order (line index fastest,
+// so consecutive lanes touch consecutive LDS words and, when the lines are adjacent columns of a
+// strided axis, consecutive global addresses).  Each radix-R stage reads all of a thread's
+// butterflies into registers, synchronises, and writes the outputs back to the autosorted
+// positions, so one LDS buffer suffices.  Twiddles come from an FP64-accurate table
+// tw[t] = exp(-2 pi i t / L) built on the host and rounded once to the working precision.
+//
+// Stage recurrence (Stockham autosort, e.g. Govindaraju et al., SC'08): with Ns the product of
+// the radices already applied, butterfly j (< L/R) reads v_r = x[j + r L/R], multiplies v_r by
+// w_L^(r k L/(Ns R)) with k = j mod Ns, applies an R-point DFT and stores y[(j/Ns) Ns R + k + r Ns].
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mafx {
+
+template <class C> struct real_of;
+template <> struct real_of<double2> { using type = double; };
+template <> struct real_of<float2> { using type = float; };
+
+template <class C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <class C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <class C> __device__ __forceinline__ C cmul(C a, C b) {
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by -i (forward) or +i (inverse)
+template <class C> __device__ __forceinline__ C mul_mi(C a, bool inv) { return inv ? C{-a.y, a.x} : C{a.y, -a.x}; }
+template <class C> __device__ __forceinline__ C conj_if(C a, bool inv) { return inv ? C{a.x, -a.y} : a; }
+
+constexpr int kFftBlock = 256;
+
+// R-point DFT in registers; forward uses w = exp(-2 pi i / R), inverse its conjugate.
+// Generic odd radices read w_R^p = tw[p L / R] from the length-L table.
+template <class C, int R>
+__device__ __forceinline__ void dft(C (&v)[R], bool inv, const C* __restrict__ tw, int L) {
+    using T = typename real_of<C>::type;
+    if constexpr (R == 2) {
+        const C a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        const C a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        const C b0 = cadd(v[1], v[3]), b1 = mul_mi(csub(v[1], v[3]), inv);
+        v[0] = cadd(a0, b0);
+        v[2] = csub(a0, b0);
+        v[1] = cadd(a1, b1);
+        v[3] = csub(a1, b1);
+    } else if constexpr (R == 8) {
+        C e[4] = {v[0], v[2], v[4], v[6]};
+        C o[4] = {v[1], v[3], v[5], v[7]};
+        dft<C, 4>(e, inv, tw, L);
+        dft<C, 4>(o, inv, tw, L);
+        const T h = static_cast<T>(0.70710678118654752440084436210485);
+        // o[q] *= w8^q
+        const C w1 = inv ? C{h, h} : C{h, -h};
+        const C w3 = inv ? C{-h, h} : C{-h, -h};
+        o[1] = cmul(o[1], w1);
+        o[2] = mul_mi(o[2], inv);
+        o[3] = cmul(o[3], w3);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v[q] = cadd(e[q], o[q]);
+            v[q + 4] = csub(e[q], o[q]);
+        }
+    } else {
+        C out[R];
+        const int step = L / R;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            C acc = v[0];
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const C w = conj_if(tw[((r * q) % R) * step], inv);
+                acc = cadd(acc, cmul(v[r], w));
+            }
+            out[q] = acc;
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = out[q];
+    }
+}
+
+// One Stockham stage over W lines of length L in buf[pos * W + w].
+// MAXB = max butterflies per thread = ceil(W L / R / blockDim).
+template <class C, int R, int MAXB>
+__device__ __forceinline__ void fft_stage(C* buf, int L, int W, int Ns, const C* __restrict__ tw, bool inv) {
+    const int m = L / R;
+    const int nb = m * W;
+    C v[MAXB][R];
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i) {
+        const int b = threadIdx.x + i * kFftBlock;
+        if (b < nb) {
+            const int w = b % W, j = b / W;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * m) * W + w];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i) {
+        const int b = threadIdx.x + i * kFftBlock;
+        if (b < nb) {
+            const int w = b % W, j = b / W;
+            const int k = j % Ns;
+            if (Ns > 1) {
+                const int step = k * (L / (Ns * R));
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], conj_if(tw[r * step], inv));
+            }
+            dft<C, R>(v[i], inv, tw, L);
+            const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[(base + r * Ns) * W + w] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Radix schedule handed to kernels by value.
+struct FftDesc {
+    int L;
+    int nstages;
+    int radix[16];
+};
+
+// Runs all stages.  LDS_ELEMS bounds W * L (the LDS budget in complex elements).
+template <class C, int LDS_ELEMS>
+__device__ void fft_lds(C* buf, const FftDesc& d, int W, const C* __restrict__ tw, bool inv) {
+    int Ns = 1;
+    for (int s = 0; s < d.nstages; ++s) {
+        const int R = d.radix[s];
+        switch (R) {
+        case 2: fft_stage<C, 2, (LDS_ELEMS / 2 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
+        case 3: fft_stage<C, 3, (LDS_ELEMS / 3 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
+        case 4: fft_stage<C, 4, (LDS_ELEMS / 4 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
+        case 5: fft_stage<C, 5, (LDS_ELEMS / 5 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
+        case 7: fft_stage<C, 7, (LDS_ELEMS / 7 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
+        case 8: fft_stage<C, 8, (LDS_ELEMS / 8 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
+        case 11: fft_stage<C, 11, (LDS_ELEMS / 11 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
+        case 13: fft_stage<C, 13, (LDS_ELEMS / 13 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
+        default: break; // unreachable: FftPlan::build rejects other factors
+        }
+        Ns *= R;
+    }
+}
+
+} // namespace mafx

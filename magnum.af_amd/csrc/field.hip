@@ -1,0 +1,656 @@
+// Effective-field evaluation: demag tensor setup, the zero-padded FFT convolution, and one fused
+// epilogue that finishes the inverse x transform (C2R + crop + 1/P) and adds every local term in
+// term order, then writes H, the LLG right-hand side, or the LBFGS gradient, and accumulates
+// per-term energy partial sums.
+//
+// Replaces (reference paths relative to magnumaf/src):
+//   field_terms/micro/demag_field.cpp:104-264, exchange_field.cpp:14-60,115-157,
+//   dmi_field.cpp:41-110, uniaxial_anisotropy_field.cpp:58-89, external_field.cpp:28-48,
+//   rkky_exchange_field.cpp:19-29,195-337, field_terms/atom/*.cpp,
+//   include/magnumaf/field_terms/field_term.hpp:116-150, micro/micro_term.hpp:18-30,
+//   include/magnumaf/equations.hpp:19-33, solvers/lbfgs_minimizer.cpp:33-42.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "fft_device.hpp"
+#include "fft_kernels.hpp"
+#include "mafx_internal.hpp"
+
+namespace mafx {
+
+constexpr int kMaxTerms = 8;
+constexpr int kBlock = 256;
+
+struct EpiTerm {
+    int kind;
+    double c0;  // coefficient with the scalar Ms (host-computed in the reference's order)
+    double num; // numerator for the per-cell Ms_field variant: coef = num / (mu0 Ms_i)
+    double n[3];
+    double f[3];
+    const double* zee;
+    const double* rk;
+    const double* aex;
+    const uint32_t* ridx;
+};
+
+struct EpiParams {
+    int64_t nx, ny, nz, N;
+    const double* m;
+    const double* ms; // Ms_field or null
+    double Ms;
+    double ex_w[3], ex_self; // exchange filter (exchange_field.cpp:116-125)
+    double dmi_c[3];         // 1 / (2 d) central difference weights (dmi_field.cpp:53-64)
+    double rk_pref[3];       // mu0 * d^2 of the RKKY couplings: dx^2 for x AND y, dz^2 for z
+    int nterms;
+    int fft_slot; // term index fed by the FFT convolution, -1 if none
+    EpiTerm t[kMaxTerms];
+    int mode;
+    void* out;
+    int out_f32;
+    double alpha, dt;
+    int diss_only;
+    int want_energy;
+    double* partials; // [nslots][nblocks]
+    int64_t nblocks;
+};
+
+namespace {
+
+__device__ __forceinline__ double ld(const double* m, int64_t i) { return m[i]; }
+
+template <class T> struct V3 {
+    T x, y, z;
+};
+
+// Sum over the block of `nv` per-thread values with a fixed tree, written to partials[s][blk].
+__device__ void block_reduce_store(double* vals, int nv, const int* is_max, double* partials, int64_t nblocks) {
+    __shared__ double red[16][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int s = 0; s < nv; ++s) {
+        double v = vals[s];
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(v, off, 64);
+            v = is_max[s] ? fmax(v, o) : v + o;
+        }
+        if (lane == 0) red[s][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int s = 0; s < nv; ++s) {
+            double v = red[s][0];
+            for (int w = 1; w < kBlock / 64; ++w) v = is_max[s] ? fmax(v, red[s][w]) : v + red[s][w];
+            partials[s * nblocks + blockIdx.x] = v;
+        }
+    }
+}
+
+// All local terms and the output stage for one cell.  hd holds the FFT-convolution field of the
+// cell (ignored when P.fft_slot < 0).  acc[] collects the energy partials (slots 0..nterms-1),
+// acc[nterms] = max |m x H|, acc[nterms+1] = max |H| (gradient mode).
+template <class T>
+__device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t z, const T hd[3], double* acc) {
+    const int64_t N = P.N;
+    const int64_t i = x + P.nx * (y + P.ny * z);
+    const double* m = P.m;
+    const double mc0 = ld(m, i), mc1 = ld(m, N + i), mc2 = ld(m, 2 * N + i);
+    const T m0 = static_cast<T>(mc0), m1 = static_cast<T>(mc1), m2 = static_cast<T>(mc2);
+
+    // neighbours with zero padding (af::convolve AF_CONV_DEFAULT semantics)
+    V3<T> nb[6]; // x-, x+, y-, y+, z-, z+
+    bool have_nb = false;
+    auto load_nb = [&]() {
+        if (have_nb) return;
+        have_nb = true;
+        const int64_t off[6] = {-1, 1, -P.nx, P.nx, -P.nx * P.ny, P.nx * P.ny};
+        const bool ok[6] = {x > 0, x + 1 < P.nx, y > 0, y + 1 < P.ny, z > 0, z + 1 < P.nz};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if (ok[k]) {
+                const int64_t j = i + off[k];
+                nb[k] = V3<T>{static_cast<T>(ld(m, j)), static_cast<T>(ld(m, N + j)), static_cast<T>(ld(m, 2 * N + j))};
+            } else {
+                nb[k] = V3<T>{0, 0, 0};
+            }
+        }
+    };
+    const double msi = P.ms ? P.ms[i] : P.Ms;
+    T H0 = 0, H1 = 0, H2 = 0;
+    for (int t = 0; t < P.nterms; ++t) {
+        const EpiTerm& tm = P.t[t];
+        T h0 = 0, h1 = 0, h2 = 0;
+        T e_extra = 0; // exchange energy boundary correction (h_withedges - h) . m
+        // micro coefficient with the Ms_field branch (cells with Ms == 0 give 0)
+        auto micro_coef = [&](T& c) -> bool {
+            if (!P.ms) {
+                c = static_cast<T>(tm.c0);
+                return true;
+            }
+            if (msi == 0) return false;
+            c = static_cast<T>(tm.num / (kMu0 * msi));
+            return true;
+        };
+        switch (tm.kind) {
+        case MAFX_DEMAG:
+        case MAFX_ATOM_DIPOLE:
+            h0 = hd[0];
+            h1 = hd[1];
+            h2 = hd[2];
+            break;
+        case MAFX_EXCHANGE: {
+            load_nb();
+            T c;
+            if (micro_coef(c)) {
+                const T wx = static_cast<T>(P.ex_w[0]), wy = static_cast<T>(P.ex_w[1]), wz = static_cast<T>(P.ex_w[2]);
+                const T sf = static_cast<T>(P.ex_self);
+                const T l0 = sf * m0 + wx * (nb[0].x + nb[1].x) + wy * (nb[2].x + nb[3].x) + wz * (nb[4].x + nb[5].x);
+                const T l1 = sf * m1 + wx * (nb[0].y + nb[1].y) + wy * (nb[2].y + nb[3].y) + wz * (nb[4].y + nb[5].y);
+                const T l2 = sf * m2 + wx * (nb[0].z + nb[1].z) + wy * (nb[2].z + nb[3].z) + wz * (nb[4].z + nb[5].z);
+                h0 = c * l0;
+                h1 = c * l1;
+                h2 = c * l2;
+                if (P.want_energy) {
+                    // Neumann correction of exchange_field.cpp:44-51: + m / d^2 on every boundary face
+                    T corr = 0;
+                    const T mm = m0 * m0 + m1 * m1 + m2 * m2;
+                    if (x == 0) corr += wx * mm;
+                    if (x == P.nx - 1) corr += wx * mm;
+                    if (y == 0) corr += wy * mm;
+                    if (y == P.ny - 1) corr += wy * mm;
+                    if (z == 0) corr += wz * mm;
+                    if (z == P.nz - 1) corr += wz * mm;
+                    e_extra = c * corr;
+                }
+            }
+            break;
+        }
+        case MAFX_DMI:
+        case MAFX_ATOM_DMI: {
+            load_nb();
+            T c;
+            bool ok;
+            T cx, cy, cz;
+            if (tm.kind == MAFX_DMI) {
+                ok = micro_coef(c);
+                cx = static_cast<T>(P.dmi_c[0]);
+                cy = static_cast<T>(P.dmi_c[1]);
+                cz = static_cast<T>(P.dmi_c[2]);
+            } else {
+                ok = true;
+                c = static_cast<T>(tm.c0);
+                cx = cy = cz = 1;
+            }
+            if (ok) {
+                const T n0 = static_cast<T>(tm.n[0]), n1 = static_cast<T>(tm.n[1]), n2 = static_cast<T>(tm.n[2]);
+                // div m with central differences (f[i+1] - f[i-1]) / (2 d)
+                const T div = (nb[1].x - nb[0].x) * cx + (nb[3].y - nb[2].y) * cy + (nb[5].z - nb[4].z) * cz;
+                auto ndot = [&](const V3<T>& v) { return n0 * v.x + n1 * v.y + n2 * v.z; };
+                const T g0 = (ndot(nb[1]) - ndot(nb[0])) * cx;
+                const T g1 = (ndot(nb[3]) - ndot(nb[2])) * cy;
+                const T g2 = (ndot(nb[5]) - ndot(nb[4])) * cz;
+                h0 = c * (n0 * div - g0);
+                h1 = c * (n1 * div - g1);
+                h2 = c * (n2 * div - g2);
+            }
+            break;
+        }
+        case MAFX_UNIAXIAL: {
+            T c;
+            if (micro_coef(c)) {
+                const T u0 = static_cast<T>(tm.n[0]), u1 = static_cast<T>(tm.n[1]), u2 = static_cast<T>(tm.n[2]);
+                const T an = u0 * m0 + u1 * m1 + u2 * m2;
+                h0 = c * (u0 * an);
+                h1 = c * (u1 * an);
+                h2 = c * (u2 * an);
+            }
+            break;
+        }
+        case MAFX_ATOM_UNIAXIAL: {
+            const T u0 = static_cast<T>(tm.n[0]), u1 = static_cast<T>(tm.n[1]), u2 = static_cast<T>(tm.n[2]);
+            const T an = u0 * m0 + u1 * m1 + u2 * m2;
+            const T c = static_cast<T>(tm.c0);
+            h0 = c * an * u0;
+            h1 = c * an * u1;
+            h2 = c * an * u2;
+            break;
+        }
+        case MAFX_ZEEMAN:
+        case MAFX_ATOM_ZEEMAN:
+            if (tm.zee) {
+                h0 = static_cast<T>(tm.zee[i]);
+                h1 = static_cast<T>(tm.zee[N + i]);
+                h2 = static_cast<T>(tm.zee[2 * N + i]);
+            } else {
+                h0 = static_cast<T>(tm.f[0]);
+                h1 = static_cast<T>(tm.f[1]);
+                h2 = static_cast<T>(tm.f[2]);
+            }
+            break;
+        case MAFX_ATOM_EXCHANGE: {
+            load_nb();
+            const T c = static_cast<T>(tm.c0);
+            h0 = c * (nb[0].x + nb[1].x + nb[2].x + nb[3].x + nb[4].x + nb[5].x);
+            h1 = c * (nb[0].y + nb[1].y + nb[2].y + nb[3].y + nb[4].y + nb[5].y);
+            h2 = c * (nb[0].z + nb[1].z + nb[2].z + nb[3].z + nb[4].z + nb[5].z);
+            break;
+        }
+        case MAFX_RKKY: {
+            // rkky_exchange_field.cpp:195-337 entries, applied as a stencil, then / Ms (:19-29)
+            load_nb();
+            const int64_t sxy = P.nx * P.ny;
+            const double Ai = tm.aex[i];
+            double a0 = 0, a1 = 0, a2 = 0;
+            if (Ai != 0) {
+                auto inplane = [&](bool ok, int64_t j, int k, double pref) {
+                    if (!ok) return;
+                    const double Aj = tm.aex[j];
+                    const double w = (2. * Ai) / pref * 2. * Aj / (Aj + Ai);
+                    a0 += w * static_cast<double>(nb[k].x);
+                    a1 += w * static_cast<double>(nb[k].y);
+                    a2 += w * static_cast<double>(nb[k].z);
+                };
+                inplane(x + 1 < P.nx, i + 1, 1, P.rk_pref[0]);
+                inplane(x > 0, i - 1, 0, P.rk_pref[0]);
+                inplane(y + 1 < P.ny, i + P.nx, 3, P.rk_pref[1]);
+                inplane(y > 0, i - P.nx, 2, P.rk_pref[1]);
+            }
+            auto zc = [&](bool ok, int64_t j, int k) {
+                if (!ok) return;
+                const double Ri = tm.rk[i], Rj = tm.rk[j];
+                const uint32_t Ii = tm.ridx ? tm.ridx[i] : 0u, Ij = tm.ridx ? tm.ridx[j] : 0u;
+                double w;
+                if ((Ii == Ij) && (Ri != 0) && (Rj != 0)) {
+                    w = (2. * Ri) / P.rk_pref[2];
+                } else {
+                    if (Ai == 0) return;
+                    const double Aj = tm.aex[j];
+                    w = (2. * Ai) / P.rk_pref[2] * 2. * Aj / (Aj + Ai);
+                }
+                a0 += w * static_cast<double>(nb[k].x);
+                a1 += w * static_cast<double>(nb[k].y);
+                a2 += w * static_cast<double>(nb[k].z);
+            };
+            zc(z + 1 < P.nz, i + sxy, 5);
+            zc(z > 0, i - sxy, 4);
+            if (!P.ms) {
+                h0 = static_cast<T>(a0 / P.Ms);
+                h1 = static_cast<T>(a1 / P.Ms);
+                h2 = static_cast<T>(a2 / P.Ms);
+            } else if (msi != 0) {
+                h0 = static_cast<T>(a0 / msi);
+                h1 = static_cast<T>(a1 / msi);
+                h2 = static_cast<T>(a2 / msi);
+            }
+            break;
+        }
+        default: break;
+        }
+        if (t == 0) {
+            H0 = h0;
+            H1 = h1;
+            H2 = h2;
+        } else {
+            H0 = H0 + h0;
+            H1 = H1 + h1;
+            H2 = H2 + h2;
+        }
+        if (P.want_energy) {
+            const double w = (P.ms && tm.kind < MAFX_ATOM_DIPOLE) ? msi : 1.0;
+            acc[t] += w * (static_cast<double>(h0) * mc0 + static_cast<double>(h1) * mc1 +
+                           static_cast<double>(h2) * mc2 + static_cast<double>(e_extra));
+        }
+    }
+    const double Hx = static_cast<double>(H0), Hy = static_cast<double>(H1), Hz = static_cast<double>(H2);
+    switch (P.mode) {
+    case static_cast<int>(OutMode::H):
+        if (P.out_f32) {
+            float* o = static_cast<float*>(P.out);
+            o[i] = static_cast<float>(H0);
+            o[N + i] = static_cast<float>(H1);
+            o[2 * N + i] = static_cast<float>(H2);
+        } else {
+            double* o = static_cast<double*>(P.out);
+            o[i] = Hx;
+            o[N + i] = Hy;
+            o[2 * N + i] = Hz;
+        }
+        break;
+    case static_cast<int>(OutMode::LLG): {
+        // equations.hpp:19-33 with math::cross4 (math.cpp:10-19); k = dt * f (adaptive_runge_kutta.cpp)
+        const double a0 = mc1 * Hz - mc2 * Hy, a1 = mc2 * Hx - mc0 * Hz, a2 = mc0 * Hy - mc1 * Hx;
+        const double b0 = mc1 * a2 - mc2 * a1, b1 = mc2 * a0 - mc0 * a2, b2 = mc0 * a1 - mc1 * a0;
+        const double a2p = P.alpha * P.alpha;
+        const double cp = -kGamma / (1. + a2p);
+        const double cd = -P.alpha * kGamma / (1. + a2p);
+        double* o = static_cast<double*>(P.out);
+        if (P.diss_only) {
+            o[i] = P.dt * (cd * b0);
+            o[N + i] = P.dt * (cd * b1);
+            o[2 * N + i] = P.dt * (cd * b2);
+        } else {
+            o[i] = P.dt * (cp * a0 + cd * b0);
+            o[N + i] = P.dt * (cp * a1 + cd * b1);
+            o[2 * N + i] = P.dt * (cp * a2 + cd * b2);
+        }
+        break;
+    }
+    case static_cast<int>(OutMode::GRAD): {
+        const double a0 = mc1 * Hz - mc2 * Hy, a1 = mc2 * Hx - mc0 * Hz, a2 = mc0 * Hy - mc1 * Hx;
+        double* o = static_cast<double*>(P.out);
+        o[i] = mc1 * a2 - mc2 * a1;
+        o[N + i] = mc2 * a0 - mc0 * a2;
+        o[2 * N + i] = mc0 * a1 - mc1 * a0;
+        acc[P.nterms] = fmax(acc[P.nterms], fmax(fabs(a0), fmax(fabs(a1), fabs(a2))));
+        acc[P.nterms + 1] = fmax(acc[P.nterms + 1], fmax(fabs(Hx), fmax(fabs(Hy), fabs(Hz))));
+        break;
+    }
+    default: break;
+    }
+}
+
+__device__ void finish_block(const EpiParams& P, double* acc) {
+    if (!P.want_energy && P.mode != static_cast<int>(OutMode::GRAD)) return;
+    int is_max[kMaxTerms + 2];
+    for (int s = 0; s < P.nterms; ++s) is_max[s] = 0;
+    is_max[P.nterms] = 1;
+    is_max[P.nterms + 1] = 1;
+    block_reduce_store(acc, P.nterms + 2, is_max, P.partials, P.nblocks);
+}
+
+// One work-group per (y, z) row: inverse x transform of the three components (C2R with the
+// Hermitian extension X[L-k] = conj X[k]), crop to x < nx, scale, then the per-cell epilogue.
+template <class C, class T>
+__global__ void __launch_bounds__(kFftBlock)
+    k_epi_rows(EpiParams P, FftDesc d, const C* __restrict__ tw, const C* __restrict__ spec, int64_t hx, int64_t n1,
+               int64_t n2, double hscale) {
+    extern __shared__ unsigned char smem_raw[];
+    C* line = reinterpret_cast<C*>(smem_raw);
+    T* res = reinterpret_cast<T*>(smem_raw + sizeof(C) * d.L);
+    const int L = d.L;
+    const int64_t y = blockIdx.x % P.ny, z = blockIdx.x / P.ny;
+    for (int c = 0; c < 3; ++c) {
+        const C* row = spec + ((c * n2 + z) * n1 + y) * hx;
+        for (int k = threadIdx.x; k < L; k += kFftBlock) {
+            C v;
+            if (k < hx) {
+                v = row[k];
+            } else {
+                const C w = row[L - k];
+                v = C{w.x, -w.y};
+            }
+            line[k] = v;
+        }
+        __syncthreads();
+        fft_lds<C, 65536 / static_cast<int>(sizeof(C))>(line, d, 1, tw, true);
+        for (int x = threadIdx.x; x < P.nx; x += kFftBlock)
+            res[c * P.nx + x] = static_cast<T>(static_cast<double>(line[x].x) * hscale);
+        __syncthreads();
+    }
+    double acc[kMaxTerms + 2];
+    for (int s = 0; s < kMaxTerms + 2; ++s) acc[s] = 0;
+    for (int64_t x = threadIdx.x; x < P.nx; x += kFftBlock) {
+        const T hd[3] = {res[x], res[P.nx + x], res[2 * P.nx + x]};
+        cell_epilogue<T>(P, x, y, z, hd, acc);
+    }
+    finish_block(P, acc);
+}
+
+// Local terms only (no FFT term): one thread per cell.
+template <class T> __global__ void __launch_bounds__(kBlock) k_epi_cells(EpiParams P) {
+    double acc[kMaxTerms + 2];
+    for (int s = 0; s < kMaxTerms + 2; ++s) acc[s] = 0;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < P.N) {
+        const int64_t x = i % P.nx, yz = i / P.nx, y = yz % P.ny, z = yz / P.ny;
+        const T hd[3] = {0, 0, 0};
+        cell_epilogue<T>(P, x, y, z, hd, acc);
+    }
+    finish_block(P, acc);
+}
+
+// Fixed-order reduction of partials[s][0..nb) into out[s].
+__global__ void k_reduce_slots(const double* __restrict__ partials, int64_t nb, int nslots, int nsum,
+                               double* __restrict__ out) {
+    __shared__ double red[kBlock];
+    const int s = blockIdx.x;
+    if (s >= nslots) return;
+    const bool is_max = s >= nsum;
+    double v = is_max ? 0.0 : 0.0;
+    for (int64_t b = threadIdx.x; b < nb; b += kBlock) {
+        const double p = partials[s * nb + b];
+        v = is_max ? fmax(v, p) : v + p;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + w])
+                                                       : red[threadIdx.x] + red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s] = red[0];
+}
+
+template <class C> C* spec_of(mafx_ctx* ctx) { return ctx->spec.as<C>(); }
+
+EpiParams make_params(mafx_ctx* ctx, const EvalArgs& a, int64_t nblocks) {
+    EpiParams P{};
+    const mafx_mesh& g = ctx->mesh;
+    P.nx = g.nx;
+    P.ny = g.ny;
+    P.nz = g.nz;
+    P.N = ctx->N;
+    P.m = a.m;
+    P.ms = ctx->has_ms_field ? ctx->ms_field.as<double>() : nullptr;
+    P.Ms = ctx->Ms;
+    P.ex_w[0] = 1 / std::pow(g.dx, 2);
+    P.ex_w[1] = 1 / std::pow(g.dy, 2);
+    P.ex_w[2] = 1 / std::pow(g.dz, 2);
+    P.ex_self = -2. / std::pow(g.dx, 2.) - 2. / std::pow(g.dy, 2.) - 2. / std::pow(g.dz, 2.);
+    P.dmi_c[0] = 1 / (2. * g.dx);
+    P.dmi_c[1] = 1 / (2. * g.dy);
+    P.dmi_c[2] = 1 / (2. * g.dz);
+    P.rk_pref[0] = kMu0 * std::pow(g.dx, 2);
+    P.rk_pref[1] = kMu0 * std::pow(g.dx, 2); // the reference couples y-neighbours with dx^2
+    P.rk_pref[2] = kMu0 * std::pow(g.dz, 2);
+    P.nterms = 0;
+    P.fft_slot = -1;
+    for (size_t k = 0; k < ctx->terms.size(); ++k) {
+        if (!(a.mask & (1u << k))) continue;
+        const TermDev& td = ctx->terms[k];
+        EpiTerm& e = P.t[P.nterms];
+        e.kind = td.kind;
+        for (int c = 0; c < 3; ++c) {
+            e.n[c] = td.axis[c];
+            e.f[c] = td.field[c];
+        }
+        e.zee = td.has_zee_array ? td.zee.as<double>() : nullptr;
+        e.rk = td.rkky.as<double>();
+        e.aex = td.aex.as<double>();
+        e.ridx = td.has_ridx ? td.ridx.as<uint32_t>() : nullptr;
+        const double Ms = ctx->Ms;
+        switch (td.kind) {
+        case MAFX_EXCHANGE: e.num = 2. * td.value; e.c0 = (2. * td.value) / (kMu0 * Ms); break;
+        case MAFX_DMI: e.num = 2. * td.value; e.c0 = (2. * td.value) / (kMu0 * Ms); break;
+        case MAFX_UNIAXIAL: e.num = 2. * td.value; e.c0 = 2. * td.value / (kMu0 * Ms); break;
+        case MAFX_ATOM_EXCHANGE: e.c0 = td.value / (kMu0 * Ms); break;
+        case MAFX_ATOM_DMI: e.c0 = td.value / (kMu0 * Ms); break;
+        case MAFX_ATOM_UNIAXIAL: e.c0 = 2 * td.value / (kMu0 * Ms); break;
+        default: break;
+        }
+        if (static_cast<int>(k) == ctx->demag_term) P.fft_slot = P.nterms;
+        P.nterms++;
+    }
+    P.mode = static_cast<int>(a.mode);
+    P.out = a.out;
+    P.out_f32 = a.out_f32 ? 1 : 0;
+    P.alpha = a.alpha;
+    P.dt = a.dt;
+    P.diss_only = a.dissipation_only ? 1 : 0;
+    P.want_energy = (a.want_energy || a.mode == OutMode::ENERGY) ? 1 : 0;
+    P.partials = ctx->partials.as<double>();
+    P.nblocks = nblocks;
+    return P;
+}
+
+template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a) {
+    const mafx_mesh& g = ctx->mesh;
+    const bool fft = ctx->demag_term >= 0 && (a.mask & (1u << ctx->demag_term));
+    const int64_t nrows = g.ny * g.nz;
+    const int64_t nblocks = fft ? nrows : (ctx->N + kBlock - 1) / kBlock;
+    require((ctx->partials.bytes / sizeof(double)) >= static_cast<size_t>((kMaxTerms + 2) * nblocks),
+            "partials buffer too small");
+    EpiParams P = make_params(ctx, a, nblocks);
+    const double field_bytes = (a.mode == OutMode::H) ? (a.out_f32 ? 12.0 : 24.0)
+                               : (a.mode == OutMode::ENERGY) ? 0.0 : 24.0;
+    if (fft) {
+        C* spec = spec_of<C>(ctx);
+        const int64_t hx = ctx->hx, n1 = ctx->n1, n2 = ctx->n2;
+        const int64_t sa = n2 * n1 * hx;
+        // Ms * m (micro; Ms_field per cell) or m (atomistic dipole) -> x half spectrum
+        const double* ms = (!ctx->atom_dipole && ctx->has_ms_field) ? ctx->ms_field.as<double>() : nullptr;
+        const double scale = ctx->atom_dipole ? 1.0 : (ctx->has_ms_field ? 1.0 : ctx->Ms);
+        fft_x_forward<C>(ctx, ctx->px, a.m, ms, scale, g.nx, g.ny, g.nz, 3, g.ny, g.nz, spec);
+        fft_strided<C>(ctx, ctx->py, spec, sa, 3, n1 * hx, static_cast<int>(g.nz), hx, static_cast<int>(g.ny),
+                       static_cast<int>(n1), false, sizeof(C) == 8 ? "fft_y_fwd_f32" : "fft_y_fwd_f64");
+        if (n2 > 1)
+            fft_strided<C>(ctx, ctx->pz, spec, sa, 3, hx, static_cast<int>(n1), n1 * hx, static_cast<int>(g.nz),
+                           static_cast<int>(n2), false, sizeof(C) == 8 ? "fft_z_fwd_f32" : "fft_z_fwd_f64");
+        kspace_mul<C>(ctx, spec, ctx->tensor.as<C>(), sa);
+        if (n2 > 1)
+            fft_strided<C>(ctx, ctx->pz, spec, sa, 3, hx, static_cast<int>(n1), n1 * hx, static_cast<int>(n2),
+                           static_cast<int>(g.nz), true, sizeof(C) == 8 ? "fft_z_inv_f32" : "fft_z_inv_f64");
+        fft_strided<C>(ctx, ctx->py, spec, sa, 3, n1 * hx, static_cast<int>(g.nz), hx, static_cast<int>(n1),
+                       static_cast<int>(g.ny), true, sizeof(C) == 8 ? "fft_y_inv_f32" : "fft_y_inv_f64");
+        const double P_total = static_cast<double>(ctx->n0) * n1 * n2;
+        const double hscale = (ctx->atom_dipole ? ctx->Ms : 1.0) / P_total;
+        const size_t lds = sizeof(C) * ctx->n0 + sizeof(T) * 3 * g.nx;
+        require(lds <= 160 * 1024, "epilogue LDS exceeds 160 KiB (mesh too wide in x)");
+        require(ctx->n0 <= 65536 / static_cast<int64_t>(sizeof(C)), "x transform length exceeds the LDS FFT budget");
+        // algorithmic bytes: the three cropped spectrum rows, m once, the output once
+        const double bytes = static_cast<double>(ctx->N) * (3.0 * sizeof(C) * hx / g.nx + 24.0 + field_bytes);
+        ProfScope ps(ctx, sizeof(C) == 8 ? "epilogue_rows_f32" : "epilogue_rows_f64", bytes);
+        MAFX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_epi_rows<C, T>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        FftDesc d{};
+        d.L = ctx->px.L;
+        d.nstages = ctx->px.nstages;
+        for (int s = 0; s < d.nstages; ++s) d.radix[s] = ctx->px.radix[s];
+        const C* tw = sizeof(C) == 8 ? reinterpret_cast<const C*>(ctx->px.tw32.p)
+                                     : reinterpret_cast<const C*>(ctx->px.tw64.p);
+        hipLaunchKernelGGL((k_epi_rows<C, T>), dim3(static_cast<unsigned>(nrows)), dim3(kFftBlock), lds, ctx->stream,
+                           P, d, tw, spec, hx, n1, n2, hscale);
+        MAFX_HIP(hipGetLastError());
+    } else {
+        ProfScope ps(ctx, sizeof(T) == 4 ? "local_cells_f32" : "local_cells_f64",
+                     static_cast<double>(ctx->N) * (24.0 + field_bytes));
+        hipLaunchKernelGGL(k_epi_cells<T>, dim3(static_cast<unsigned>(nblocks)), dim3(kBlock), 0, ctx->stream, P);
+        MAFX_HIP(hipGetLastError());
+    }
+    ctx->scal_nblocks = nblocks;
+    ctx->scal_nterms = P.nterms;
+}
+
+} // namespace
+
+void evaluate(mafx_ctx* ctx, const EvalArgs& a) {
+    require(ctx->terms.size() <= static_cast<size_t>(kMaxTerms), "at most 8 terms per context");
+    if (ctx->precision == MAFX_F64)
+        evaluate_impl<double2, double>(ctx, a);
+    else
+        evaluate_impl<float2, float>(ctx, a);
+}
+
+// Reduces the partials of the last evaluate() (energy slots and, in gradient mode, the two
+// maxima) to host values.  E_terms is indexed by the context's term order.
+void finish_energy(mafx_ctx* ctx, uint32_t mask, double* E_terms, double* E_total) {
+    const int nt = ctx->scal_nterms;
+    const int nslots = nt + 2;
+    hipLaunchKernelGGL(k_reduce_slots, dim3(nslots), dim3(kBlock), 0, ctx->stream, ctx->partials.as<double>(),
+                       ctx->scal_nblocks, nslots, nt, ctx->scal.as<double>());
+    MAFX_HIP(hipGetLastError());
+    MAFX_HIP(hipMemcpyAsync(ctx->host_scal, ctx->scal.p, sizeof(double) * nslots, hipMemcpyDeviceToHost, ctx->stream));
+    MAFX_HIP(hipStreamSynchronize(ctx->stream));
+    const mafx_mesh& g = ctx->mesh;
+    double total = 0;
+    int slot = 0;
+    for (size_t k = 0; k < ctx->terms.size(); ++k) {
+        double e = 0;
+        if (mask & (1u << k)) {
+            const double S = ctx->host_scal[slot++];
+            const int kind = ctx->terms[k].kind;
+            const bool zee = kind == MAFX_ZEEMAN || kind == MAFX_ATOM_ZEEMAN;
+            if (kind >= MAFX_ATOM_DIPOLE) { // atom_term.hpp:12-15, atomistic_external_field.cpp:5-11
+                e = zee ? -kMu0 * ctx->Ms * S : -kMu0 / 2. * ctx->Ms * S;
+            } else if (!ctx->has_ms_field) { // micro_term.hpp:18-30, external_field.cpp:38-48
+                e = zee ? -kMu0 * ctx->Ms * S * g.dx * g.dy * g.dz : -kMu0 / 2. * ctx->Ms * S * g.dx * g.dy * g.dz;
+            } else {
+                e = zee ? -kMu0 * S * g.dx * g.dy * g.dz : -kMu0 / 2. * S * g.dx * g.dy * g.dz;
+            }
+        }
+        if (E_terms) E_terms[k] = e;
+        total = total + e;
+    }
+    if (E_total) *E_total = total;
+    ctx->last_max_torque = ctx->host_scal[nt];
+    ctx->last_max_h = ctx->host_scal[nt + 1];
+}
+
+// Process-wide cache of the host real-space tensor: the bit-exact Newell setup costs minutes of
+// CPU at 1024 x 1024 x 4, so contexts of the same mesh (e.g. the FP64 and FP32 paths) share it.
+namespace {
+std::mutex g_tensor_mu;
+std::map<std::tuple<int64_t, int64_t, int64_t, double, double, double, bool>, std::shared_ptr<std::vector<double>>>
+    g_tensor_cache;
+
+std::shared_ptr<std::vector<double>> host_tensor(const mafx_mesh& g, bool dipole, int64_t P) {
+    const auto key = std::make_tuple(g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, dipole);
+    {
+        std::lock_guard<std::mutex> lk(g_tensor_mu);
+        auto it = g_tensor_cache.find(key);
+        if (it != g_tensor_cache.end()) return it->second;
+    }
+    auto v = std::make_shared<std::vector<double>>(static_cast<size_t>(6 * P));
+    if (dipole)
+        dipole_tensor_host(g, v->data());
+    else
+        newell_tensor_host(g, v->data(), 0);
+    std::lock_guard<std::mutex> lk(g_tensor_mu);
+    if (g_tensor_cache.size() >= 2) g_tensor_cache.erase(g_tensor_cache.begin());
+    g_tensor_cache[key] = v;
+    return v;
+}
+} // namespace
+
+// Demag / dipole tensor: real-space tensor on the host in the reference's arithmetic, then a
+// full FP64 R2C transform on the GPU with the same kernels, stored as the complex half spectrum.
+void build_tensor(mafx_ctx* ctx) {
+    const int64_t n0 = ctx->n0, n1 = ctx->n1, n2 = ctx->n2, hx = ctx->hx;
+    const int64_t P = n0 * n1 * n2, K = n2 * n1 * hx;
+    auto Nh = host_tensor(ctx->mesh, ctx->atom_dipole, P);
+    DevBuf Nd, T64;
+    Nd.alloc(sizeof(double) * 6 * P);
+    MAFX_HIP(hipMemcpy(Nd.p, Nh->data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
+    Nh.reset();
+    T64.alloc(sizeof(double2) * 6 * K);
+    double2* t = T64.as<double2>();
+    fft_x_forward<double2>(ctx, ctx->px, Nd.as<double>(), nullptr, 1.0, n0, n1, n2, 6, n1, n2, t);
+    fft_strided<double2>(ctx, ctx->py, t, K, 6, n1 * hx, static_cast<int>(n2), hx, static_cast<int>(n1),
+                         static_cast<int>(n1), false, "tensor_fft_y");
+    if (n2 > 1)
+        fft_strided<double2>(ctx, ctx->pz, t, K, 6, hx, static_cast<int>(n1), n1 * hx, static_cast<int>(n2),
+                             static_cast<int>(n2), false, "tensor_fft_z");
+    MAFX_HIP(hipStreamSynchronize(ctx->stream));
+    Nd.release();
+    if (ctx->precision == MAFX_F64) {
+        std::swap(ctx->tensor.p, T64.p);
+        std::swap(ctx->tensor.bytes, T64.bytes);
+    } else {
+        ctx->tensor.alloc(sizeof(float2) * 6 * K);
+        convert_tensor_f32(ctx, t, ctx->tensor.as<float2>(), 6 * K);
+        MAFX_HIP(hipStreamSynchronize(ctx->stream));
+    }
+}
+
+} // namespace mafx

@@ -1,0 +1,339 @@
+// LBFGS minimizer on the GPU: L-BFGS (history 5) with the More-Thuente line search on the sphere.
+// Control flow and scalar arithmetic run on the host exactly as the reference's; every 3N-sized
+// vector lives on the device (gradient m x (m x H) and the energy come from one fused field pass).
+//
+// Replaces magnumaf/src/solvers/lbfgs_minimizer.cpp:33-505.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <limits>
+
+#include "mafx_internal.hpp"
+#include "mafx_llg.hpp"
+#include "vec_kernels.hpp"
+
+namespace mafx {
+
+namespace {
+
+struct Lbfgs {
+    mafx_ctx* ctx;
+    int64_t n3;
+    int64_t evals{0};
+    double last_torque{0}, last_hmax{0};
+
+    // EnergyAndGradient (lbfgs_minimizer.cpp:38-42): one field pass writes g and the energy partials
+    double energy_and_gradient(const double* m, double* g) {
+        EvalArgs ea;
+        ea.m = m;
+        ea.mode = OutMode::GRAD;
+        ea.out = g;
+        ea.want_energy = true;
+        evaluate(ctx, ea);
+        double E = 0;
+        finish_energy(ctx, ~0u, nullptr, &E);
+        last_torque = ctx->last_max_torque;
+        last_hmax = ctx->last_max_h;
+        ++evals;
+        return E;
+    }
+    double mydot(const double* a, const double* b) { return dot(ctx, a, b, n3); }
+    double mynorm(const double* a) { return std::sqrt(mydot(a, a)); }
+    double maxabs(const double* a) { return max_abs(ctx, a, n3); }
+};
+
+// lbfgs_minimizer.cpp:362-505
+int cstep(double& stx, double& fx, double& dx, double& sty, double& fy, double& dy, double& stp, double& fp,
+          double& dp, bool& brackt, double& stpmin, double& stpmax, int& info) {
+    info = 0;
+    bool bound = false;
+    if ((brackt & ((stp <= std::min(stx, sty)) | (stp >= std::max(stx, sty)))) | (dx * (stp - stx) >= 0.0) |
+        (stpmax < stpmin))
+        return -1;
+    const double sgnd = dp * (dx / std::fabs(dx));
+    double stpf = 0, stpc = 0, stpq = 0;
+    if (fp > fx) {
+        info = 1;
+        bound = true;
+        const double theta = 3. * (fx - fp) / (stp - stx) + dx + dp;
+        const double s = std::max(theta, std::max(dx, dp));
+        double gamma = s * std::sqrt((theta / s) * (theta / s) - (dx / s) * (dp / s));
+        if (stp < stx) gamma = -gamma;
+        const double p = (gamma - dx) + theta;
+        const double q = ((gamma - dx) + gamma) + dp;
+        const double r = p / q;
+        stpc = stx + r * (stp - stx);
+        stpq = stx + ((dx / ((fx - fp) / (stp - stx) + dx)) / 2.) * (stp - stx);
+        stpf = std::fabs(stpc - stx) < std::fabs(stpq - stx) ? stpc : stpc + (stpq - stpc) / 2;
+        brackt = true;
+    } else if (sgnd < 0.0) {
+        info = 2;
+        bound = false;
+        const double theta = 3 * (fx - fp) / (stp - stx) + dx + dp;
+        const double s = std::max(theta, std::max(dx, dp));
+        double gamma = s * std::sqrt((theta / s) * (theta / s) - (dx / s) * (dp / s));
+        if (stp > stx) gamma = -gamma;
+        const double p = (gamma - dp) + theta;
+        const double q = ((gamma - dp) + gamma) + dx;
+        const double r = p / q;
+        stpc = stp + r * (stx - stp);
+        stpq = stp + (dp / (dp - dx)) * (stx - stp);
+        stpf = std::fabs(stpc - stp) > std::fabs(stpq - stp) ? stpc : stpq;
+        brackt = true;
+    } else if (std::fabs(dp) < std::fabs(dx)) {
+        info = 3;
+        bound = true;
+        const double theta = 3 * (fx - fp) / (stp - stx) + dx + dp;
+        const double s = std::max(theta, std::max(dx, dp));
+        double gamma = s * std::sqrt(std::max(0., (theta / s) * (theta / s) - (dx / s) * (dp / s)));
+        if (stp > stx) gamma = -gamma;
+        const double p = (gamma - dp) + theta;
+        const double q = (gamma + (dx - dp)) + gamma;
+        const double r = p / q;
+        if ((r < 0.0) & (gamma != 0.0))
+            stpc = stp + r * (stx - stp);
+        else if (stp > stx)
+            stpc = stpmax;
+        else
+            stpc = stpmin;
+        stpq = stp + (dp / (dp - dx)) * (stx - stp);
+        if (brackt)
+            stpf = std::fabs(stp - stpc) < std::fabs(stp - stpq) ? stpc : stpq;
+        else
+            stpf = std::fabs(stp - stpc) > std::fabs(stp - stpq) ? stpc : stpq;
+    } else {
+        info = 4;
+        bound = false;
+        if (brackt) {
+            const double theta = 3 * (fp - fy) / (sty - stp) + dy + dp;
+            const double s = std::max(theta, std::max(dy, dp));
+            double gamma = s * std::sqrt((theta / s) * (theta / s) - (dy / s) * (dp / s));
+            if (stp > sty) gamma = -gamma;
+            const double p = (gamma - dp) + theta;
+            const double q = ((gamma - dp) + gamma) + dy;
+            const double r = p / q;
+            stpc = stp + r * (sty - stp);
+            stpf = stpc;
+        } else if (stp > stx) {
+            stpf = stpmax;
+        } else {
+            stpf = stpmin;
+        }
+    }
+    if (fp > fx) {
+        sty = stp;
+        fy = fp;
+        dy = dp;
+    } else {
+        if (sgnd < 0.0) {
+            sty = stx;
+            fy = fx;
+            dy = dx;
+        }
+        stx = stp;
+        fx = fp;
+        dx = dp;
+    }
+    stpf = std::min(stpmax, stpf);
+    stpf = std::max(stpmin, stpf);
+    stp = stpf;
+    if (brackt & bound) {
+        if (sty > stx)
+            stp = std::min(stx + 0.66 * (sty - stx), stp);
+        else
+            stp = std::max(stx + 0.66 * (sty - stx), stp);
+    }
+    return 0;
+}
+
+// lbfgs_minimizer.cpp:213-360; m is overwritten with normalize(wa + stp s) at every trial.
+int cvsrch(Lbfgs& L, double* m, const double* wa, double& f, double* g, double& stp, const double* s,
+           const double tolf) {
+    int info = 0;
+    int infoc = 1;
+    const double xtol = 1e-15;
+    const double ftol = 1.0e-4;
+    const double gtol = 0.9;
+    const double eps = tolf;
+    const double stpmin = 1e-15;
+    const double stpmax = 1e15;
+    const double xtrapf = 4;
+    const int maxfev = 20;
+    int nfev = 0;
+    const double dginit = L.mydot(g, s);
+    if (dginit >= 0.0) return -1;
+    bool brackt = false;
+    bool stage1 = true;
+    const double finit = f;
+    const double dgtest = ftol * dginit;
+    double width = stpmax - stpmin;
+    double width1 = 2 * width;
+    double stx = 0.0, fx = finit, dgx = dginit;
+    double sty = 0.0, fy = finit, dgy = dginit;
+    double stmin = std::numeric_limits<double>::quiet_NaN();
+    double stmax = std::numeric_limits<double>::quiet_NaN();
+    while (true) {
+        if (brackt) {
+            stmin = std::min(stx, sty);
+            stmax = std::max(stx, sty);
+        } else {
+            stmin = stx;
+            stmax = stp + xtrapf * (stp - stx);
+        }
+        stp = std::max(stp, stpmin);
+        stp = std::min(stp, stpmax);
+        if ((brackt && ((stp <= stmin) | (stp >= stmax))) | (nfev >= maxfev - 1) | (infoc == 0) |
+            (brackt & (stmax - stmin <= xtol * stmax)))
+            stp = stx;
+        line_point(L.ctx, m, wa, stp, s);
+        f = L.energy_and_gradient(m, g);
+        nfev++;
+        const double dg = L.mydot(g, s);
+        const double ftest1 = finit + stp * dgtest;
+        const double ftest2 = finit + eps * std::fabs(finit);
+        const double ft = 2 * ftol - 1;
+        if ((brackt & ((stp <= stmin) | (stp >= stmax))) | (infoc == 0)) info = 6;
+        if ((stp == stpmax) & (f <= ftest2) & (dg <= dgtest)) info = 5;
+        if ((stp == stpmin) & ((f > ftest2) | (dg >= dgtest))) info = 4;
+        if (nfev >= maxfev) info = 3;
+        if (brackt & (stmax - stmin <= xtol * stmax)) info = 2;
+        if ((f <= ftest1) & (std::fabs(dg) <= gtol * (-dginit))) info = 1;
+        if (((f <= ftest2) && (ft * dginit >= dg)) && (std::fabs(dg) <= gtol * (-dginit))) info = 1;
+        if (info != 0) return -1;
+        if (stage1 & ((f <= ftest2) && (ft * dginit >= dg)) & (dg >= std::min(ftol, gtol) * dginit)) stage1 = false;
+        if (stage1 & (f <= fx) & (!((f <= ftest2) && (ft * dginit >= dg)))) {
+            double fm = f - stp * dgtest;
+            double fxm = fx - stx * dgtest;
+            double fym = fy - sty * dgtest;
+            double dgm = dg - dgtest;
+            double dgxm = dgx - dgtest;
+            double dgym = dgy - dgtest;
+            cstep(stx, fxm, dgxm, sty, fym, dgym, stp, fm, dgm, brackt, stmin, stmax, infoc);
+            fx = fxm + stx * dgtest;
+            fy = fym + sty * dgtest;
+            dgx = dgxm + dgtest;
+            dgy = dgym + dgtest;
+        } else {
+            double fv = f, dgv = dg;
+            cstep(stx, fx, dgx, sty, fy, dgy, stp, fv, dgv, brackt, stmin, stmax, infoc);
+        }
+        if (brackt) {
+            if (std::fabs(sty - stx) >= 0.66 * width1) stp = stx + 0.5 * (sty - stx);
+            width1 = width;
+            width = std::fabs(sty - stx);
+        }
+    }
+}
+
+} // namespace
+
+// lbfgs_minimizer.cpp:55-198
+int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter, int verbose, double torque_tol,
+                   double* E_out, mafx_lbfgs_stats* stats) {
+    (void)verbose;
+    Lbfgs L{ctx, 3 * ctx->N};
+    const int64_t n3 = L.n3;
+    const size_t bytes = sizeof(double) * n3;
+    constexpr size_t hist = 5;
+    std::array<DevBuf, hist> sV, yV;
+    DevBuf q, s, y, grad, grad_old, x_old, dir;
+    for (auto* b : {&q, &s, &y, &grad, &grad_old, &x_old, &dir}) b->alloc(bytes);
+    for (size_t i = 0; i < hist; ++i) {
+        sV[i].alloc(bytes); // read only after being written (k <= iter)
+        yV[i].alloc(bytes);
+    }
+    const double eps = 2.22e-16;
+    const double eps2 = std::sqrt(eps);
+    const double epsr = std::pow(eps, 0.9);
+    const double tolf2 = std::sqrt(tolerance);
+    const double tolf3 = std::pow(tolerance, 0.3333333333333333333333333);
+    int reason = 0;
+    double f = L.energy_and_gradient(m, grad.as<double>());
+    double gradNorm = L.maxabs(grad.as<double>());
+    size_t globIter = 0;
+    auto finish = [&](int r) {
+        if (E_out) *E_out = f;
+        if (stats) {
+            stats->iterations = static_cast<int64_t>(globIter);
+            stats->energy_evals = L.evals;
+            stats->final_energy = f;
+            stats->final_torque_rel = L.last_hmax > 0 ? L.last_torque / L.last_hmax : 0.0;
+            stats->stop_reason = r;
+        }
+        return MAFX_OK;
+    };
+    if (gradNorm < (epsr * (1 + std::fabs(f)))) return finish(1);
+    std::array<double, hist> alpha{};
+    double f_old = 0.0;
+    size_t iter = 0;
+    double H0k = 1;
+    do {
+        f_old = f;
+        copy(ctx, x_old.as<double>(), m, n3);
+        copy(ctx, grad_old.as<double>(), grad.as<double>(), n3);
+        copy(ctx, q.as<double>(), grad.as<double>(), n3);
+        const int k = static_cast<int>(std::min(hist, iter));
+        for (int i = k - 1; i >= 0; i--) {
+            const double rho = 1.0 / L.mydot(sV[i].as<double>(), yV[i].as<double>());
+            alpha[i] = rho * L.mydot(sV[i].as<double>(), q.as<double>());
+            axpy(ctx, q.as<double>(), -alpha[i], yV[i].as<double>(), n3);
+        }
+        scale(ctx, q.as<double>(), H0k, n3);
+        for (int i = 0; i < k; i++) {
+            const double rho = 1.0 / L.mydot(sV[i].as<double>(), yV[i].as<double>());
+            const double beta = rho * L.mydot(yV[i].as<double>(), q.as<double>());
+            axpy(ctx, q.as<double>(), alpha[i] - beta, sV[i].as<double>(), n3);
+        }
+        const double phiPrime0 = -L.mydot(grad.as<double>(), q.as<double>());
+        if (phiPrime0 > 0) {
+            copy(ctx, q.as<double>(), grad.as<double>(), n3);
+            iter = 0;
+        }
+        // searchDir = -q (multiplying by -1 is exact)
+        copy(ctx, dir.as<double>(), q.as<double>(), n3);
+        scale(ctx, dir.as<double>(), -1.0, n3);
+        double rate = 1.0;
+        cvsrch(L, m, x_old.as<double>(), f, grad.as<double>(), rate, dir.as<double>(), tolerance);
+        const double f1 = 1 + std::fabs(f);
+        gradNorm = L.maxabs(grad.as<double>());
+        if (gradNorm < (epsr * f1)) {
+            globIter++;
+            reason = 1;
+            break;
+        }
+        sub(ctx, s.as<double>(), m, x_old.as<double>(), n3);
+        if (((f_old - f) < (tolerance * f1)) && (L.maxabs(s.as<double>()) < (tolf2 * (1 + L.maxabs(m)))) &&
+            (gradNorm <= (tolf3 * f1))) {
+            globIter++;
+            reason = 2;
+            break;
+        }
+        if (torque_tol > 0 && L.last_hmax > 0 && L.last_torque / L.last_hmax <= torque_tol) {
+            globIter++;
+            reason = 3;
+            break;
+        }
+        sub(ctx, y.as<double>(), grad.as<double>(), grad_old.as<double>(), n3);
+        const double ys = L.mydot(y.as<double>(), s.as<double>());
+        if (ys <= eps2 * L.mynorm(y.as<double>()) * L.mynorm(s.as<double>())) {
+            // Dennis-Schnabel 9.4.1: skip the update
+        } else {
+            if (iter < hist) {
+                copy(ctx, sV[iter].as<double>(), s.as<double>(), n3);
+                copy(ctx, yV[iter].as<double>(), y.as<double>(), n3);
+            } else {
+                std::rotate(sV.begin(), sV.begin() + 1, sV.end());
+                std::rotate(yV.begin(), yV.begin() + 1, yV.end());
+                copy(ctx, sV[hist - 1].as<double>(), s.as<double>(), n3);
+                copy(ctx, yV[hist - 1].as<double>(), y.as<double>(), n3);
+            }
+            H0k = ys / L.mydot(y.as<double>(), y.as<double>());
+            iter++;
+        }
+        globIter++;
+    } while (globIter < static_cast<size_t>(max_iter));
+    MAFX_HIP(hipStreamSynchronize(ctx->stream));
+    return finish(reason);
+}
+
+} // namespace mafx

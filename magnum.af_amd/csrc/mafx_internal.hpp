@@ -1,0 +1,203 @@
+// Internal declarations of libmafx: context layout, FFT plans, error handling, launch profiling.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mafx.h"
+
+namespace mafx {
+
+// magnumaf/include/magnumaf/constants.hpp:9-11
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kMu0 = 4e-7 * kPi;
+constexpr double kGamma = 1.760859644e11 * kMu0;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define MAFX_HIP(call)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::mafx::Error(MAFX_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+inline void require(bool ok, const std::string& msg) {
+    if (!ok) throw Error(MAFX_ERR_ARG, msg);
+}
+
+// ------------------------------------------------------------------ device buffers
+struct DevBuf {
+    void* p{nullptr};
+    size_t bytes{0};
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t b) {
+        release();
+        if (b) MAFX_HIP(hipMalloc(&p, b));
+        bytes = b;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ------------------------------------------------------------------ launch profiling
+struct Prof {
+    bool on{false};
+    struct Entry {
+        int64_t count{0};
+        double ms{0};
+        double bytes{0}; // algorithmic bytes per launch
+    };
+    std::map<std::string, Entry> entries;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<double> pending_bytes;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get_event();
+    void flush(); // resolves pending events (synchronises)
+    ~Prof();
+};
+
+// ------------------------------------------------------------------ FFT plan (one axis length)
+struct FftPlan {
+    int L{0};                 // transform length
+    int nstages{0};
+    int radix[16]{};          // radices in application order
+    DevBuf tw64, tw32;        // exp(-2 pi i t / L), t in [0, L): double2 / float2
+    void build(int len);      // throws if a prime factor is unsupported
+};
+
+// ------------------------------------------------------------------ terms
+struct TermDev {
+    int kind{0};
+    double value{0};
+    double axis[3]{0, 0, 1};
+    double field[3]{0, 0, 0};
+    DevBuf zee;       // 3N f64 when the external field is an array
+    DevBuf rkky, aex; // N f64
+    DevBuf ridx;      // N u32 (optional)
+    bool has_zee_array{false}, has_ridx{false};
+};
+
+struct Workspace; // precision-specific demag buffers
+
+} // namespace mafx
+
+struct mafx_ctx {
+    mafx_mesh mesh{};
+    int64_t N{0};
+    double Ms{0};
+    bool has_ms_field{false};
+    mafx::DevBuf ms_field; // N f64
+    int precision{MAFX_F64};
+    std::vector<mafx::TermDev> terms;
+    int demag_term{-1};       // index of the (micro or atomistic) FFT term, -1 if none
+    bool atom_dipole{false};
+    // FFT geometry: expanded lengths (demag_field.cpp:14-16) and half-spectrum width
+    int64_t n0{0}, n1{0}, n2{0}, hx{0};
+    mafx::FftPlan px, py, pz;
+    mafx::DevBuf tensor;      // [6][n2][n1][hx] complex, precision of the field path
+    mafx::DevBuf spec;        // [3][n2][n1][hx] complex workspace
+    mafx::DevBuf hbuf;        // 3N f64 scratch (field output for helpers)
+    mafx::DevBuf partials;    // reduction partials (f64)
+    mafx::DevBuf scal;        // small device scalars
+    double* host_scal{nullptr}; // pinned host scalars (hipHostMalloc)
+    int64_t scal_nblocks{0};  // partial-sum blocks of the last evaluation
+    int scal_nterms{0};       // terms selected in the last evaluation
+    double last_max_torque{0}, last_max_h{0}; // max|m x H|, max|H| of the last gradient evaluation
+    mafx::DevBuf vtmp;        // 3N f64 scratch for vector helpers
+    hipStream_t stream{nullptr};
+    mafx::Prof prof;
+    int64_t device_bytes{0};
+    ~mafx_ctx();
+};
+
+namespace mafx {
+
+// ------------------------------------------------------------------ host tensor setup (newell_host.cpp)
+// Real-space Newell tensor [6][n2][n1][n0], reference operation order (demag_field.cpp:20-142).
+void newell_tensor_host(const mafx_mesh& mesh, double* N, int nthreads);
+// Point-dipole tensor of AtomisticDipoleDipoleField (atomistic_dipole_dipole_field.cpp:55-113), x 1/(4 pi).
+void dipole_tensor_host(const mafx_mesh& mesh, double* N);
+
+// ------------------------------------------------------------------ field evaluation (field.hip)
+enum class OutMode : int {
+    H = 0,      // write H (f64 or f32)
+    LLG = 1,    // write k = dt * LLG(alpha, m, H) (f64)
+    GRAD = 2,   // write g = m x (m x H) (f64) and per-term energy partials
+    ENERGY = 3  // per-term energy partials only
+};
+struct EvalArgs {
+    const double* m{nullptr};
+    uint32_t mask{~0u};
+    OutMode mode{OutMode::H};
+    void* out{nullptr};
+    bool out_f32{false};
+    double alpha{0}, dt{0};
+    bool dissipation_only{false};
+    bool want_energy{false};
+};
+// Runs the selected terms on m; the energy partial sums (one per term) land in ctx->partials and
+// are reduced by finish_energy.
+void evaluate(mafx_ctx* ctx, const EvalArgs& a);
+void finish_energy(mafx_ctx* ctx, uint32_t mask, double* E_terms, double* E_total);
+void build_tensor(mafx_ctx* ctx);
+
+// ------------------------------------------------------------------ vector kernels (vec.hip)
+double dot(mafx_ctx* ctx, const double* a, const double* b, int64_t n);
+double max_abs(mafx_ctx* ctx, const double* a, int64_t n);
+void normalize(mafx_ctx* ctx, double* m, bool zero_safe);
+void mean_m(mafx_ctx* ctx, const double* m, double out[3]);
+void llg_rhs(mafx_ctx* ctx, double alpha, double dt, const double* m, const double* h, double* k);
+// out = base + sum_j c_j v_j (j < n, n <= 8), evaluated left to right
+void lincomb(mafx_ctx* ctx, double* out, const double* base, int n, const double* const* v, const double* c,
+             int64_t len);
+// x = normalize(a + s * b) with the zero-safe normalisation (lbfgs_minimizer.cpp:277-278)
+void line_point(mafx_ctx* ctx, double* x, const double* a, double s, const double* b);
+
+// ------------------------------------------------------------------ profiling helper
+struct ProfScope {
+    mafx_ctx* ctx;
+    const char* name;
+    double bytes;
+    hipEvent_t b{nullptr};
+    ProfScope(mafx_ctx* c, const char* n, double by);
+    ~ProfScope();
+};
+
+inline unsigned grid_for(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    return static_cast<unsigned>(g < 1 ? 1 : g);
+}
+
+} // namespace mafx

@@ -1,0 +1,353 @@
+// FP64 vector kernels of the integrator and minimizer: deterministic two-pass reductions
+// (fixed block count and fixed tree, so results are reproducible run to run), normalisation,
+// <m>, the LLG right-hand side, RK stage combinations, the embedded error norm and the accept
+// update.  Reference citations are relative to magnumaf/src.
+#include <cmath>
+
+#include "mafx_internal.hpp"
+#include "vec_kernels.hpp"
+
+namespace mafx {
+
+namespace {
+
+constexpr int kB = 256;
+constexpr int kMaxRedBlocks = 1024;
+
+unsigned red_blocks(int64_t n) {
+    int64_t b = (n + kB - 1) / kB;
+    if (b > kMaxRedBlocks) b = kMaxRedBlocks;
+    return static_cast<unsigned>(b < 1 ? 1 : b);
+}
+
+__device__ double block_sum(double v) {
+    __shared__ double red[kB / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = red[0];
+    for (int w = 1; w < kB / 64; ++w) s += red[w];
+    return s;
+}
+__device__ double block_max(double v) {
+    __shared__ double red[kB / 64];
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = red[0];
+    for (int w = 1; w < kB / 64; ++w) s = fmax(s, red[w]);
+    return s;
+}
+
+__global__ void __launch_bounds__(kB) k_dot(const double* __restrict__ a, const double* __restrict__ b, int64_t n,
+                                            double* __restrict__ part) {
+    double s = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * kB)
+        s += a[i] * b[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kB) k_maxabs(const double* __restrict__ a, int64_t n, double* __restrict__ part) {
+    double s = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * kB)
+        s = fmax(s, fabs(a[i]));
+    s = block_max(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// final pass over `nb` partials of `nslots` slots laid out [slot][nb]; slots >= nsum take the max
+__global__ void __launch_bounds__(kB) k_final(const double* __restrict__ part, int nb, int nslots, int nsum,
+                                             double* __restrict__ out) {
+    const int s = blockIdx.x;
+    if (s >= nslots) return;
+    const bool mx = s >= nsum;
+    double v = 0;
+    for (int i = threadIdx.x; i < nb; i += kB) v = mx ? fmax(v, part[s * nb + i]) : v + part[s * nb + i];
+    v = mx ? block_max(v) : block_sum(v);
+    if (threadIdx.x == 0) out[s] = v;
+}
+
+// util.cpp:34 (a / |a|) and util.cpp:37-42 (zero vectors stay zero)
+__global__ void __launch_bounds__(kB) k_normalize(double* __restrict__ m, int64_t N, int zero_safe) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    if (i >= N) return;
+    const double a = m[i], b = m[N + i], c = m[2 * N + i];
+    const double n = sqrt(a * a + b * b + c * c);
+    if (zero_safe && n == 0) {
+        m[i] = 0;
+        m[N + i] = 0;
+        m[2 * N + i] = 0;
+    } else {
+        m[i] = a / n;
+        m[N + i] = b / n;
+        m[2 * N + i] = c / n;
+    }
+}
+
+// partials: [0..2] component sums, [3] count of |m| != 0  (state.cpp:32-45)
+__global__ void __launch_bounds__(kB) k_mean(const double* __restrict__ m, int64_t N, double* __restrict__ part) {
+    double s0 = 0, s1 = 0, s2 = 0, cnt = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < N; i += static_cast<int64_t>(gridDim.x) * kB) {
+        const double a = m[i], b = m[N + i], c = m[2 * N + i];
+        s0 += a;
+        s1 += b;
+        s2 += c;
+        if (sqrt(a * a + b * b + c * c) != 0) cnt += 1;
+    }
+    s0 = block_sum(s0);
+    s1 = block_sum(s1);
+    s2 = block_sum(s2);
+    cnt = block_sum(cnt);
+    if (threadIdx.x == 0) {
+        part[0 * gridDim.x + blockIdx.x] = s0;
+        part[1 * gridDim.x + blockIdx.x] = s1;
+        part[2 * gridDim.x + blockIdx.x] = s2;
+        part[3 * gridDim.x + blockIdx.x] = cnt;
+    }
+}
+
+// k = dt * (-gamma/(1+a^2) m x h - a gamma/(1+a^2) m x (m x h))  (equations.hpp:19-33)
+__global__ void __launch_bounds__(kB) k_llg_rhs(const double* __restrict__ m, const double* __restrict__ h,
+                                               double* __restrict__ k, int64_t N, double alpha, double dt) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    if (i >= N) return;
+    const double m0 = m[i], m1 = m[N + i], m2 = m[2 * N + i];
+    const double H0 = h[i], H1 = h[N + i], H2 = h[2 * N + i];
+    const double a0 = m1 * H2 - m2 * H1, a1 = m2 * H0 - m0 * H2, a2 = m0 * H1 - m1 * H0;
+    const double b0 = m1 * a2 - m2 * a1, b1 = m2 * a0 - m0 * a2, b2 = m0 * a1 - m1 * a0;
+    const double cp = -kGamma / (1. + alpha * alpha), cd = -alpha * kGamma / (1. + alpha * alpha);
+    k[i] = dt * (cp * a0 + cd * b0);
+    k[N + i] = dt * (cp * a1 + cd * b1);
+    k[2 * N + i] = dt * (cp * a2 + cd * b2);
+}
+
+struct LinArgs {
+    const double* v[8];
+    double c[8];
+    int n;
+};
+
+// out = ((base + c0 v0) + c1 v1) + ...  (the element-wise evaluation order of the reference's
+// stage inputs, adaptive_runge_kutta.cpp:139-165)
+__global__ void __launch_bounds__(kB) k_lincomb(double* __restrict__ out, const double* __restrict__ base, LinArgs a,
+                                               int64_t len) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    if (i >= len) return;
+    double s = base[i];
+    for (int j = 0; j < a.n; ++j) s = s + a.c[j] * a.v[j][i];
+    out[i] = s;
+}
+
+// x = normalize_handle_zero_vectors(a + s b)  (lbfgs_minimizer.cpp:277-278)
+__global__ void __launch_bounds__(kB) k_line_point(double* __restrict__ x, const double* __restrict__ a, double s,
+                                                  const double* __restrict__ b, int64_t N) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    if (i >= N) return;
+    const double p = a[i] + s * b[i], q = a[N + i] + s * b[N + i], r = a[2 * N + i] + s * b[2 * N + i];
+    const double n = sqrt(p * p + q * q + r * r);
+    x[i] = n != 0 ? p / n : 0.0;
+    x[N + i] = n != 0 ? q / n : 0.0;
+    x[2 * N + i] = n != 0 ? r / n : 0.0;
+}
+
+struct ErrArgs {
+    const double* k[8]; // k[1..ns]
+    double b[8], e[8];
+    int ns;
+    double atol, rtol;
+};
+
+// sumbk = sum b_j k_j, rk_error = sum e_j k_j (left to right, adaptive_runge_kutta.cpp:105-106,168-169),
+// err = max |rk_error / (atol + rtol |max(m, m + sumbk)|)| (:108,171) as per-block maxima.
+// For RKF45 the reference forms rk_error = sumbk - (sum c_j k_j); `e` then holds c_j and
+// rkf_form = 1 selects that shape.
+__global__ void __launch_bounds__(kB) k_rk_error(const double* __restrict__ m, ErrArgs a, int rkf_form,
+                                                double* __restrict__ sumbk, int64_t len, double* __restrict__ part) {
+    double emax = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < len; i += static_cast<int64_t>(gridDim.x) * kB) {
+        double sb = a.b[1] * a.k[1][i];
+        for (int j = 2; j <= a.ns; ++j) sb = sb + a.b[j] * a.k[j][i];
+        double re;
+        if (rkf_form) {
+            double t = a.e[1] * a.k[1][i];
+            for (int j = 2; j <= a.ns; ++j) t = t + a.e[j] * a.k[j][i];
+            re = sb - t;
+        } else {
+            re = a.e[1] * a.k[1][i];
+            for (int j = 2; j <= a.ns; ++j) re = re + a.e[j] * a.k[j][i];
+        }
+        sumbk[i] = sb;
+        const double mi = m[i];
+        const double mm = fmax(mi, mi + sb);
+        emax = fmax(emax, fabs(re / (a.atol + a.rtol * fabs(mm))));
+    }
+    emax = block_max(emax);
+    if (threadIdx.x == 0) part[blockIdx.x] = emax;
+}
+
+// m += dm, then normalise (adaptive_runge_kutta.cpp:59-66)
+__global__ void __launch_bounds__(kB) k_accept(double* __restrict__ m, const double* __restrict__ dm, int64_t N,
+                                              int normalize, int zero_safe) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    if (i >= N) return;
+    double a = m[i] + dm[i], b = m[N + i] + dm[N + i], c = m[2 * N + i] + dm[2 * N + i];
+    if (normalize) {
+        const double n = sqrt(a * a + b * b + c * c);
+        if (zero_safe && n == 0) {
+            a = b = c = 0;
+        } else {
+            a = a / n;
+            b = b / n;
+            c = c / n;
+        }
+    }
+    m[i] = a;
+    m[N + i] = b;
+    m[2 * N + i] = c;
+}
+
+__global__ void __launch_bounds__(kB) k_axpy(double* __restrict__ y, double a, const double* __restrict__ x, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    if (i < n) y[i] = y[i] + a * x[i];
+}
+
+__global__ void __launch_bounds__(kB) k_scale(double* __restrict__ y, double a, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    if (i < n) y[i] = a * y[i];
+}
+
+__global__ void __launch_bounds__(kB) k_sub(double* __restrict__ out, const double* __restrict__ a,
+                                           const double* __restrict__ b, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    if (i < n) out[i] = a[i] - b[i];
+}
+
+double read_scalar(mafx_ctx* ctx, int nslots) {
+    MAFX_HIP(hipMemcpyAsync(ctx->host_scal, ctx->scal.p, sizeof(double) * nslots, hipMemcpyDeviceToHost, ctx->stream));
+    MAFX_HIP(hipStreamSynchronize(ctx->stream));
+    return ctx->host_scal[0];
+}
+
+} // namespace
+
+double dot(mafx_ctx* ctx, const double* a, const double* b, int64_t n) {
+    const unsigned nb = red_blocks(n);
+    {
+        ProfScope ps(ctx, "dot_f64", 16.0 * n);
+        hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kB), 0, ctx->stream, a, b, n, ctx->partials.as<double>());
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(),
+                           static_cast<int>(nb), 1, 1, ctx->scal.as<double>());
+    }
+    MAFX_HIP(hipGetLastError());
+    return read_scalar(ctx, 1);
+}
+
+double max_abs(mafx_ctx* ctx, const double* a, int64_t n) {
+    const unsigned nb = red_blocks(n);
+    {
+        ProfScope ps(ctx, "maxabs_f64", 8.0 * n);
+        hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(kB), 0, ctx->stream, a, n, ctx->partials.as<double>());
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(),
+                           static_cast<int>(nb), 1, 0, ctx->scal.as<double>());
+    }
+    MAFX_HIP(hipGetLastError());
+    return read_scalar(ctx, 1);
+}
+
+void normalize(mafx_ctx* ctx, double* m, bool zero_safe) {
+    ProfScope ps(ctx, "normalize_f64", 48.0 * ctx->N);
+    hipLaunchKernelGGL(k_normalize, dim3(grid_for(ctx->N, kB)), dim3(kB), 0, ctx->stream, m, ctx->N,
+                       zero_safe ? 1 : 0);
+    MAFX_HIP(hipGetLastError());
+}
+
+void mean_m(mafx_ctx* ctx, const double* m, double out[3]) {
+    const unsigned nb = red_blocks(ctx->N);
+    hipLaunchKernelGGL(k_mean, dim3(nb), dim3(kB), 0, ctx->stream, m, ctx->N, ctx->partials.as<double>());
+    hipLaunchKernelGGL(k_final, dim3(4), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(), static_cast<int>(nb), 4,
+                       4, ctx->scal.as<double>());
+    MAFX_HIP(hipGetLastError());
+    read_scalar(ctx, 4);
+    for (int c = 0; c < 3; ++c) out[c] = ctx->host_scal[c] / ctx->host_scal[3];
+}
+
+void llg_rhs(mafx_ctx* ctx, double alpha, double dt, const double* m, const double* h, double* k) {
+    hipLaunchKernelGGL(k_llg_rhs, dim3(grid_for(ctx->N, kB)), dim3(kB), 0, ctx->stream, m, h, k, ctx->N, alpha, dt);
+    MAFX_HIP(hipGetLastError());
+}
+
+void lincomb(mafx_ctx* ctx, double* out, const double* base, int n, const double* const* v, const double* c,
+             int64_t len) {
+    require(n >= 0 && n <= 8, "lincomb: at most 8 vectors");
+    LinArgs a{};
+    a.n = n;
+    for (int j = 0; j < n; ++j) {
+        a.v[j] = v[j];
+        a.c[j] = c[j];
+    }
+    ProfScope ps(ctx, "rk_stage_f64", 8.0 * len * (n + 2));
+    hipLaunchKernelGGL(k_lincomb, dim3(grid_for(len, kB)), dim3(kB), 0, ctx->stream, out, base, a, len);
+    MAFX_HIP(hipGetLastError());
+}
+
+void line_point(mafx_ctx* ctx, double* x, const double* a, double s, const double* b) {
+    ProfScope ps(ctx, "line_point_f64", 72.0 * ctx->N);
+    hipLaunchKernelGGL(k_line_point, dim3(grid_for(ctx->N, kB)), dim3(kB), 0, ctx->stream, x, a, s, b, ctx->N);
+    MAFX_HIP(hipGetLastError());
+}
+
+double rk_error(mafx_ctx* ctx, const double* m, const double* const* k, const double* b, const double* e, int ns,
+                bool rkf_form, double atol, double rtol, double* sumbk) {
+    ErrArgs a{};
+    a.ns = ns;
+    for (int j = 1; j <= ns; ++j) {
+        a.k[j] = k[j];
+        a.b[j] = b[j];
+        a.e[j] = e[j];
+    }
+    a.atol = atol;
+    a.rtol = rtol;
+    const int64_t len = 3 * ctx->N;
+    const unsigned nb = red_blocks(len);
+    {
+        ProfScope ps(ctx, "rk_error_f64", 8.0 * len * (ns + 2));
+        hipLaunchKernelGGL(k_rk_error, dim3(nb), dim3(kB), 0, ctx->stream, m, a, rkf_form ? 1 : 0, sumbk, len,
+                           ctx->partials.as<double>());
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(),
+                           static_cast<int>(nb), 1, 0, ctx->scal.as<double>());
+    }
+    MAFX_HIP(hipGetLastError());
+    return read_scalar(ctx, 1);
+}
+
+void accept(mafx_ctx* ctx, double* m, const double* dm, bool normalize_m, bool zero_safe) {
+    ProfScope ps(ctx, "rk_accept_f64", 72.0 * ctx->N);
+    hipLaunchKernelGGL(k_accept, dim3(grid_for(ctx->N, kB)), dim3(kB), 0, ctx->stream, m, dm, ctx->N,
+                       normalize_m ? 1 : 0, zero_safe ? 1 : 0);
+    MAFX_HIP(hipGetLastError());
+}
+
+void axpy(mafx_ctx* ctx, double* y, double a, const double* x, int64_t n) {
+    hipLaunchKernelGGL(k_axpy, dim3(grid_for(n, kB)), dim3(kB), 0, ctx->stream, y, a, x, n);
+    MAFX_HIP(hipGetLastError());
+}
+void scale(mafx_ctx* ctx, double* y, double a, int64_t n) {
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(n, kB)), dim3(kB), 0, ctx->stream, y, a, n);
+    MAFX_HIP(hipGetLastError());
+}
+void sub(mafx_ctx* ctx, double* out, const double* a, const double* b, int64_t n) {
+    hipLaunchKernelGGL(k_sub, dim3(grid_for(n, kB)), dim3(kB), 0, ctx->stream, out, a, b, n);
+    MAFX_HIP(hipGetLastError());
+}
+void copy(mafx_ctx* ctx, double* dst, const double* src, int64_t n) {
+    MAFX_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+int64_t reduction_partials_needed(int64_t n) { return static_cast<int64_t>(red_blocks(n)) * 4; }
+
+} // namespace mafx

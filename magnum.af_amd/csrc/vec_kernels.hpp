@@ -1,0 +1,19 @@
+// Host launchers of the FP64 vector kernels (vec.hip) used by the integrator and the minimizer.
+#pragma once
+#include "mafx_internal.hpp"
+
+namespace mafx {
+
+// sumbk = sum_j b_j k_j and the max-norm error of the embedded pair; k, b, e are indexed 1..ns.
+// rkf_form selects rk_error = sumbk - sum_j e_j k_j (adaptive_runge_kutta.cpp:105-106) instead of
+// rk_error = sum_j e_j k_j (:169).
+double rk_error(mafx_ctx* ctx, const double* m, const double* const* k, const double* b, const double* e, int ns,
+                bool rkf_form, double atol, double rtol, double* sumbk);
+void accept(mafx_ctx* ctx, double* m, const double* dm, bool normalize_m, bool zero_safe);
+void axpy(mafx_ctx* ctx, double* y, double a, const double* x, int64_t n);
+void scale(mafx_ctx* ctx, double* y, double a, int64_t n);
+void sub(mafx_ctx* ctx, double* out, const double* a, const double* b, int64_t n);
+void copy(mafx_ctx* ctx, double* dst, const double* src, int64_t n);
+int64_t reduction_partials_needed(int64_t n);
+
+} // namespace mafx

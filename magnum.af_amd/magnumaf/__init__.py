@@ -1,0 +1,542 @@
+"""Python host layer over libmafx mirroring the reference's `magnumaf` module for the hot path
+(python/magnumaf.pyx:522-1836): Mesh, State, the field terms, LLGIntegrator and LBFGS_Minimizer,
+with the reference's class names, argument names, defaults and units.
+
+Arrays in and out use the reference's indexing m[x, y, z, c] (numpy, shape (nx, ny, nz, 3)).
+The magnetization lives on the GPU (a torch float64 tensor, torch serving only as the device
+allocator); every field evaluation, integrator step and minimizer iteration runs in libmafx.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, lib
+
+__all__ = ["Mesh", "State", "Constants", "DemagField", "ExchangeField", "DmiField", "UniaxialAnisotropyField",
+           "ExternalField", "RKKYExchangeField", "AtomisticDipoleDipoleField", "AtomisticExchangeField",
+           "AtomisticDmiField", "AtomisticUniaxialAnisotropyField", "AtomisticExternalField", "LLGIntegrator",
+           "LBFGS_Minimizer", "Context", "to_flat", "from_flat"]
+
+
+class Constants:
+    """magnumaf/include/magnumaf/constants.hpp:9-22"""
+    mu0 = 4e-7 * math.pi
+    gamma = 1.760859644e11 * mu0
+    mu_b = 9.274009994e-24
+    e = -1.6021766208e-19
+    kb = 1.38064852e-23
+    hbar = 1.0545718e-34
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("magnumaf (MI355X build) needs a GPU: torch.cuda.is_available() is False")
+    return torch
+
+
+def to_flat(v):
+    """(nx, ny, nz, 3) -> flat [3][nz][ny][nx] float64 (the af::array column-major layout)."""
+    v = np.asarray(v, dtype=np.float64)
+    return np.ascontiguousarray(v.transpose(3, 2, 1, 0)).ravel()
+
+
+def from_flat(f, nx, ny, nz):
+    return np.ascontiguousarray(np.asarray(f).reshape(3, nz, ny, nx).transpose(3, 2, 1, 0))
+
+
+def _scalar_flat(s, mesh):
+    s = np.asarray(s, dtype=np.float64)
+    if s.ndim == 4:
+        s = s[..., 0]
+    if s.ndim == 3:
+        return np.ascontiguousarray(s.transpose(2, 1, 0)).ravel()
+    return np.ascontiguousarray(s.ravel())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Mesh:
+    """Mesh(nx, ny, nz, dx, dy, dz) (mesh.hpp:11-20, magnumaf.pyx:522-616)."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+
+    def _c(self):
+        return _abi.Mesh(self.nx, self.ny, self.nz, self.dx, self.dy, self.dz)
+
+    @property
+    def n(self):
+        return self.nx * self.ny * self.nz
+
+    def key(self):
+        return (self.nx, self.ny, self.nz, self.dx, self.dy, self.dz)
+
+
+class Context:
+    """One libmafx evaluation context: mesh, material, an ordered term list, precision."""
+
+    def __init__(self, mesh, Ms, Ms_field, terms, precision="f64"):
+        self.mesh = mesh
+        self.terms = list(terms)
+        descs, self._keep = [], []
+        for t in self.terms:
+            d, keep = t._desc(mesh)
+            descs.append(d)
+            self._keep.extend(keep)
+        arr = (_abi.TermDesc * len(descs))(*descs)
+        self._keep.append(arr)
+        msf = None
+        if Ms_field is not None:
+            msf = np.ascontiguousarray(Ms_field, dtype=np.float64)
+            self._keep.append(msf)
+        prec = _abi.MAFX_F64 if precision in ("f64", _abi.MAFX_F64) else _abi.MAFX_F32_FIELD
+        self.precision = prec
+        h = C.c_void_p()
+        torch = _torch()
+        torch.cuda.init()
+        check(lib().mafx_ctx_create(C.byref(mesh._c()), float(Ms) if Ms is not None else 0.0,
+                                    _dp(msf) if msf is not None else None, arr, len(descs), prec, C.byref(h)))
+        self.h = h
+        check(lib().mafx_ctx_set_stream(self.h, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mafx_ctx_destroy(self.h)
+            self.h = None
+
+    def heff(self, m_dev, mask=0xFFFFFFFF, out=None, f32=False):
+        torch = _torch()
+        n3 = 3 * self.mesh.n
+        if out is None:
+            out = torch.empty(n3, dtype=torch.float32 if f32 else torch.float64, device="cuda")
+        check(lib().mafx_heff(self.h, C.c_void_p(m_dev.data_ptr()), mask, C.c_void_p(out.data_ptr()), int(f32)))
+        return out
+
+    def energy(self, m_dev, mask=0xFFFFFFFF):
+        e = np.zeros(len(self.terms))
+        tot = C.c_double()
+        check(lib().mafx_energy(self.h, C.c_void_p(m_dev.data_ptr()), mask, _dp(e), C.byref(tot)))
+        return tot.value, e
+
+    def device_bytes(self):
+        b = C.c_int64()
+        check(lib().mafx_ctx_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def prof(self, on=True):
+        check(lib().mafx_prof_enable(self.h, int(on)))
+
+    def prof_reset(self):
+        check(lib().mafx_prof_reset(self.h))
+
+    def prof_report(self):
+        buf = C.create_string_buffer(1 << 16)
+        check(lib().mafx_prof_report(self.h, buf, len(buf)))
+        rows = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms, by = line.split()
+            rows[name] = dict(count=int(cnt), ms=float(ms), bytes=float(by))
+        return rows
+
+
+class State:
+    """State(mesh, Ms, m) (state.cpp:91-111, magnumaf.pyx:694-878).
+
+    m is normalised with zero vectors kept at zero; when Ms is a float and some |m| == 0, an
+    Ms_field = Ms * (|m| != 0) is built as the reference does (state.cpp:71-89)."""
+
+    def __init__(self, mesh, Ms, m, verbose=False, mute_warning=False, precision="f64"):
+        torch = _torch()
+        self.mesh = mesh
+        self.t = 0.0
+        self.steps = 0
+        mf = to_flat(m) if np.ndim(m) == 4 else np.array(m, dtype=np.float64).ravel()
+        N = mesh.n
+        a, b, c = mf[:N], mf[N:2 * N], mf[2 * N:]
+        nrm = np.sqrt(a * a + b * b + c * c)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            mf = np.where(np.concatenate([nrm, nrm, nrm]) != 0, mf / np.concatenate([nrm, nrm, nrm]), 0.0)
+        if np.ndim(Ms) == 0:
+            self.Ms = float(Ms)
+            self.Ms_field = None
+            if np.min(nrm) == 0:
+                self.Ms_field = self.Ms * (nrm != 0).astype(np.float64)
+        else:
+            self.Ms = float("nan")
+            self.Ms_field = _scalar_flat(Ms, mesh)
+        self._m = torch.from_numpy(mf).to("cuda")
+        self._ctx = {}
+
+    # -- material key for context reuse
+    def _context(self, terms, precision="f64"):
+        key = (tuple(id(t) for t in terms), precision)
+        ctx = self._ctx.get(key)
+        if ctx is None:
+            ctx = Context(self.mesh, self.Ms, self.Ms_field, terms, precision)
+            self._ctx[key] = ctx
+        return ctx
+
+    @property
+    def m(self):
+        return from_flat(self._m.cpu().numpy(), self.mesh.nx, self.mesh.ny, self.mesh.nz)
+
+    @m.setter
+    def m(self, value):
+        mf = to_flat(value)
+        N = self.mesh.n
+        a, b, c = mf[:N], mf[N:2 * N], mf[2 * N:]
+        nrm = np.sqrt(a * a + b * b + c * c)
+        nrm3 = np.concatenate([nrm, nrm, nrm])
+        with np.errstate(invalid="ignore", divide="ignore"):
+            mf = np.where(nrm3 != 0, mf / nrm3, 0.0)
+        self._m.copy_(_torch().from_numpy(mf))
+
+    def set_m_raw(self, value):
+        """Sets m without renormalisation (for energy sweeps over prescribed states)."""
+        self._m.copy_(_torch().from_numpy(to_flat(value)))
+
+    @property
+    def m_dev(self):
+        return self._m
+
+    def _util_ctx(self):
+        # a context with one zero external field serves the vector helpers
+        if "_util" not in self._ctx:
+            self._ctx["_util"] = Context(self.mesh, self.Ms if self.Ms_field is None else 0.0, self.Ms_field,
+                                         [ExternalField(homogeneous=(0.0, 0.0, 0.0))])
+        return self._ctx["_util"]
+
+    def normalize(self):
+        check(lib().mafx_normalize(self._util_ctx().h, C.c_void_p(self._m.data_ptr()), 1))
+
+    def mean_m(self, i=None):
+        """<m> over cells with |m| != 0 (state.cpp:32-45)."""
+        out = np.zeros(3)
+        check(lib().mafx_mean_m(self._util_ctx().h, C.c_void_p(self._m.data_ptr()), _dp(out)))
+        mean = (float(out[0]), float(out[1]), float(out[2]))
+        return mean if i is None else mean[i]
+
+    def mean_mx(self):
+        return self.mean_m(0)
+
+    def mean_my(self):
+        return self.mean_m(1)
+
+    def mean_mz(self):
+        return self.mean_m(2)
+
+    def __str__(self):
+        mx, my, mz = self.mean_m()
+        return f"{self.t} {mx} {my} {mz}"
+
+
+# ---------------------------------------------------------------- field terms
+class FieldTerm:
+    """Base of the field terms (field_term.hpp:12-51): H_in_Apm(state), Energy_in_J(state)."""
+
+    def _desc(self, mesh):
+        raise NotImplementedError
+
+    def H_in_Apm(self, state, precision="f64"):
+        ctx = state._context([self], precision)
+        h = ctx.heff(state.m_dev)
+        return from_flat(h.cpu().numpy(), state.mesh.nx, state.mesh.ny, state.mesh.nz)
+
+    def H_in_T(self, state):
+        return self.H_in_Apm(state) * Constants.mu0
+
+    def Energy_in_J(self, state, precision="f64"):
+        ctx = state._context([self], precision)
+        return ctx.energy(state.m_dev)[0]
+
+    def Energy_in_eV(self, state):
+        return self.Energy_in_J(state) / 1.6021766208e-19
+
+
+def _td(kind, value=0.0, axis=(0.0, 0.0, 1.0), field=(0.0, 0.0, 0.0)):
+    d = _abi.TermDesc()
+    d.kind = kind
+    d.value = float(value)
+    d.axis[:] = [float(v) for v in axis]
+    d.field[:] = [float(v) for v in field]
+    return d
+
+
+class DemagField(FieldTerm):
+    """DemagField(mesh, verbose=False, caching=False, nthreads=4) (demag_field.hpp:14-30)."""
+
+    def __init__(self, mesh, verbose=False, caching=False, nthreads=4):
+        self.mesh = mesh
+
+    def _desc(self, mesh):
+        return _td(_abi.DEMAG), []
+
+
+class ExchangeField(FieldTerm):
+    """ExchangeField(A) in J/m (exchange_field.hpp:8-16)."""
+
+    def __init__(self, A, mesh=None, verbose=False):
+        self.A = float(A)
+
+    def _desc(self, mesh):
+        return _td(_abi.EXCHANGE, self.A), []
+
+
+class DmiField(FieldTerm):
+    """DmiField(D, D_axis=[0, 0, 1]) interface DMI in J/m^2 (dmi_field.hpp:8-23)."""
+
+    def __init__(self, D, D_axis=(0.0, 0.0, 1.0)):
+        self.D = float(D)
+        self.D_axis = list(D_axis)
+
+    def _desc(self, mesh):
+        return _td(_abi.DMI, self.D, self.D_axis), []
+
+
+class UniaxialAnisotropyField(FieldTerm):
+    """UniaxialAnisotropyField(Ku1, Ku1_axis=[0, 0, 1]) in J/m^3 (uniaxial_anisotropy_field.hpp)."""
+
+    def __init__(self, Ku1, Ku1_axis=(0.0, 0.0, 1.0)):
+        self.Ku1 = float(Ku1)
+        n = math.sqrt(sum(float(v) ** 2 for v in Ku1_axis))
+        self.Ku1_axis = [float(v) / n for v in Ku1_axis]
+
+    def _desc(self, mesh):
+        return _td(_abi.UNIAXIAL, self.Ku1, self.Ku1_axis), []
+
+
+class ExternalField(FieldTerm):
+    """ExternalField(array (nx, ny, nz, 3) in A/m) or ExternalField(homogeneous=(hx, hy, hz))."""
+
+    kind = _abi.ZEEMAN
+
+    def __init__(self, array_in=None, homogeneous=None):
+        self.array = None if array_in is None else np.asarray(array_in, dtype=np.float64)
+        self.homogeneous = None if homogeneous is None else [float(v) for v in homogeneous]
+
+    def set_homogeneous_field(self, x, y, z):
+        self.homogeneous = [float(x), float(y), float(z)]
+        self.array = None
+
+    def _desc(self, mesh):
+        if self.array is not None:
+            a = self.array
+            if a.shape[:3] != (mesh.nx, mesh.ny, mesh.nz):
+                a = np.broadcast_to(a, (mesh.nx, mesh.ny, mesh.nz, 3))
+            flat = to_flat(a)
+            d = _td(self.kind)
+            d.array0 = _dp(flat)
+            return d, [flat]
+        return _td(self.kind, field=self.homogeneous or (0, 0, 0)), []
+
+
+class RKKYExchangeField(FieldTerm):
+    """RKKYExchangeField(rkky_values, exchange_values, mesh, rkky_indices=None)
+    (rkky_exchange_field.hpp:37-55): scalar fields (nx, ny, nz) or (nx, ny, nz, 3)."""
+
+    def __init__(self, rkky_values, exchange_values, mesh, rkky_indices=None, verbose=True):
+        self.rk = _scalar_flat(rkky_values, mesh)
+        self.A = _scalar_flat(exchange_values, mesh)
+        self.idx = None
+        if rkky_indices is not None:
+            r = np.asarray(rkky_indices)
+            if r.ndim == 4:
+                r = r[..., 0]
+            self.idx = np.ascontiguousarray(r.transpose(2, 1, 0)).ravel().astype(np.uint32)
+
+    def _desc(self, mesh):
+        d = _td(_abi.RKKY)
+        d.array0 = _dp(self.rk)
+        d.array1 = _dp(self.A)
+        if self.idx is not None:
+            d.index0 = self.idx.ctypes.data_as(C.POINTER(C.c_uint32))
+        return d, [self.rk, self.A, self.idx]
+
+
+class AtomisticDipoleDipoleField(FieldTerm):
+    def __init__(self, mesh):
+        self.mesh = mesh
+
+    def _desc(self, mesh):
+        return _td(_abi.ATOM_DIPOLE), []
+
+
+class AtomisticExchangeField(FieldTerm):
+    def __init__(self, J_atom):
+        self.J_atom = float(J_atom)
+
+    def _desc(self, mesh):
+        return _td(_abi.ATOM_EXCHANGE, self.J_atom), []
+
+
+class AtomisticDmiField(FieldTerm):
+    """Python default axis [0, 0, -1] (magnumaf.pyx:1637)."""
+
+    def __init__(self, D_atom, D_atom_axis=(0.0, 0.0, -1.0)):
+        self.D_atom = float(D_atom)
+        self.axis = list(D_atom_axis)
+
+    def _desc(self, mesh):
+        return _td(_abi.ATOM_DMI, self.D_atom, self.axis), []
+
+
+class AtomisticUniaxialAnisotropyField(FieldTerm):
+    def __init__(self, K_atom, K_atom_axis=(0.0, 0.0, 1.0)):
+        self.K_atom = float(K_atom)
+        self.axis = list(K_atom_axis)
+
+    def _desc(self, mesh):
+        return _td(_abi.ATOM_UNIAXIAL, self.K_atom, self.axis), []
+
+
+class AtomisticExternalField(ExternalField):
+    kind = _abi.ATOM_ZEEMAN
+
+
+# ---------------------------------------------------------------- integrator
+class LLGIntegrator:
+    """LLGIntegrator(alpha, terms, mode="RKF45", hmin=1e-15, hmax=3.5e-10, atol=1e-6, rtol=1e-6,
+    dissipation_term_only=False) (magnumaf.pyx:912-1033, llg_integrator.hpp:26-82).
+
+    precision="f32" selects the FP32-field path (demag FFT and stencils in FP32, state and RK in FP64)."""
+
+    def __init__(self, alpha, terms, mode="RKF45", hmin=1e-15, hmax=3.5e-10, atol=1e-6, rtol=1e-6,
+                 dissipation_term_only=False, precision="f64"):
+        self._alpha = float(alpha)
+        self.terms = list(terms)
+        self.mode = mode
+        self.hmin, self.hmax, self.atol, self.rtol = hmin, hmax, atol, rtol
+        self.dissipation_term_only = dissipation_term_only
+        self.precision = precision
+        self._llg = None
+        self._ctx = None
+        self._ctrl = None  # (h, errold, reject) carried across rebuilds
+        self.accumulated_steps = 0
+
+    def _bind(self, state):
+        ctx = state._context(self.terms, self.precision)
+        if self._llg is not None and self._ctx is ctx:
+            return
+        if self._llg is not None:
+            self._ctrl = self.controller_state()
+            lib().mafx_llg_destroy(self._llg)
+            self._llg = None
+        h = C.c_void_p()
+        check(lib().mafx_llg_create(ctx.h, self._alpha, self.mode.encode(), self.hmin, self.hmax, self.atol,
+                                    self.rtol, int(self.dissipation_term_only), C.byref(h)))
+        self._llg, self._ctx = h, ctx
+        if self._ctrl is not None:
+            check(lib().mafx_llg_set_controller_state(self._llg, *self._ctrl))
+
+    def __del__(self):
+        if getattr(self, "_llg", None):
+            lib().mafx_llg_destroy(self._llg)
+            self._llg = None
+
+    @property
+    def alpha(self):
+        return self._alpha
+
+    @alpha.setter
+    def alpha(self, v):
+        self._alpha = float(v)
+        if self._llg is not None:
+            check(lib().mafx_llg_set_alpha(self._llg, self._alpha))
+
+    def controller_state(self):
+        h, e, r = C.c_double(), C.c_double(), C.c_int()
+        check(lib().mafx_llg_get_controller_state(self._llg, C.byref(h), C.byref(e), C.byref(r)))
+        return h.value, e.value, r.value
+
+    def add_terms(self, *args):
+        if self._llg is not None:
+            self._ctrl = self.controller_state()
+            lib().mafx_llg_destroy(self._llg)
+            self._llg, self._ctx = None, None
+        self.terms.extend(args)
+
+    def step(self, state):
+        self._bind(state)
+        t = C.c_double(state.t)
+        check(lib().mafx_llg_step(self._llg, C.c_void_p(state.m_dev.data_ptr()), C.byref(t)))
+        state.t = t.value
+        state.steps += 1
+        self.accumulated_steps += 1
+
+    def run_until(self, state, t_end, max_steps=1 << 62):
+        self._bind(state)
+        t = C.c_double(state.t)
+        n = C.c_int64()
+        check(lib().mafx_llg_run_until(self._llg, C.c_void_p(state.m_dev.data_ptr()), C.byref(t), t_end, max_steps,
+                                       C.byref(n)))
+        state.t = t.value
+        state.steps += n.value
+        self.accumulated_steps += n.value
+        return n.value
+
+    def stats(self):
+        s = _abi.LlgStats()
+        check(lib().mafx_llg_get_stats(self._llg, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _abi.LlgStats._fields_}
+
+    def integrate_dense(self, state, time_in_s, write_every_dt_in_s, filename=None, verbose=False, append=False):
+        """Returns rows (t, <mx>, <my>, <mz>); also written to `filename` in the reference's
+        't<TAB>mx<TAB>my<TAB>mz' line format when given."""
+        self._bind(state)
+        rows_max = int(time_in_s / write_every_dt_in_s) + 8
+        out = np.zeros(4 * rows_max)
+        t = C.c_double(state.t)
+        nr = C.c_int64()
+        check(lib().mafx_llg_integrate_dense(self._llg, C.c_void_p(state.m_dev.data_ptr()), C.byref(t), time_in_s,
+                                             write_every_dt_in_s, _dp(out), rows_max, C.byref(nr)))
+        state.t = t.value
+        rows = out[: 4 * nr.value].reshape(nr.value, 4)
+        if filename:
+            with open(filename, "a" if append else "w") as f:
+                for r in rows:
+                    f.write(f"{r[0]!r}\t{r[1]!r}\t{r[2]!r}\t{r[3]!r}\n")
+        return rows
+
+    def H_in_Apm(self, state):
+        ctx = state._context(self.terms, self.precision)
+        h = ctx.heff(state.m_dev)
+        return from_flat(h.cpu().numpy(), state.mesh.nx, state.mesh.ny, state.mesh.nz)
+
+    def Eeff_in_J(self, state):
+        return state._context(self.terms, self.precision).energy(state.m_dev)[0]
+
+    E = Eeff_in_J
+
+
+class LBFGS_Minimizer:
+    """LBFGS_Minimizer(terms=[], tol=1e-6, maxiter=230, verbose=0) (magnumaf.pyx:1786-1836,
+    lbfgs_minimizer.hpp:9-24).  torque_tol > 0 adds the stop max|m x H| / max|H| <= torque_tol."""
+
+    def __init__(self, terms=(), tol=1e-6, maxiter=230, verbose=0, torque_tol=0.0, precision="f64"):
+        self.terms = list(terms)
+        self.tol, self.maxiter, self.verbose = tol, maxiter, verbose
+        self.torque_tol = torque_tol
+        self.precision = precision
+        self.last_stats = None
+
+    def add_terms(self, *args):
+        self.terms.extend(args)
+
+    def delete_last_term(self):
+        self.terms.pop()
+
+    def minimize(self, state):
+        ctx = state._context(self.terms, self.precision)
+        E = C.c_double()
+        st = _abi.LbfgsStats()
+        check(lib().mafx_lbfgs_minimize(ctx.h, C.c_void_p(state.m_dev.data_ptr()), self.tol, self.maxiter,
+                                        self.verbose, self.torque_tol, C.byref(E), C.byref(st)))
+        self.last_stats = {k: getattr(st, k) for k, _ in _abi.LbfgsStats._fields_}
+        return E.value

@@ -1,0 +1,375 @@
+"""GPU parity: the HIP path (through the C-ABI, via the magnumaf host layer) against the FP64 CPU
+oracle on the same seeded inputs.
+
+Tolerances: the FP64 path must match H_eff normwise, max|dH| / max|H_ref| <= 1e-12; energies to
+1e-12 relative to the largest term energy; trajectories step for step.  The FP32-field path's
+error is measured and bounded at 1e-5 normwise."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+MU0 = O.MU0
+FP64_TOL = 1e-12
+
+
+def mf():
+    import magnumaf
+    return magnumaf
+
+
+def rand_m(shape, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(tuple(shape) + (3,))
+
+
+def normwise(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def energy_scale(mesh, Ms, E_terms):
+    """max(|E_term|, mu0 Ms^2 V): a vanishing term energy is rounding noise of its own size."""
+    V = mesh[0] * mesh[1] * mesh[2] * mesh[3] * mesh[4] * mesh[5]
+    return max(float(np.max(np.abs(E_terms))), MU0 * Ms * Ms * V)
+
+
+def build(mesh, Ms, m, specs, Ms_field=None):
+    """specs: list of (kind, kwargs) -> (gpu State, gpu terms, oracle Sim)"""
+    M = mf()
+    g = M.Mesh(*mesh)
+    gterms, oterms = [], []
+    for kind, kw in specs:
+        if kind == "demag":
+            gterms.append(M.DemagField(g))
+            oterms.append(O.term(O.DEMAG))
+        elif kind == "exchange":
+            gterms.append(M.ExchangeField(kw["A"]))
+            oterms.append(O.term(O.EXCHANGE, kw["A"]))
+        elif kind == "dmi":
+            gterms.append(M.DmiField(kw["D"], kw.get("axis", (0, 0, 1))))
+            oterms.append(O.term(O.DMI, kw["D"], axis=kw.get("axis", (0, 0, 1))))
+        elif kind == "uniaxial":
+            gterms.append(M.UniaxialAnisotropyField(kw["K"], kw.get("axis", (0, 0, 1))))
+            oterms.append(O.term(O.UNIAXIAL, kw["K"], axis=kw.get("axis", (0, 0, 1))))
+        elif kind == "zeeman":
+            gterms.append(M.ExternalField(kw["H"]))
+            oterms.append(O.term(O.ZEEMAN, array0=O.to_flat(kw["H"])))
+        elif kind == "zeeman_h":
+            gterms.append(M.ExternalField(homogeneous=kw["h"]))
+            oterms.append(O.term(O.ZEEMAN, field=kw["h"]))
+        elif kind == "rkky":
+            gterms.append(M.RKKYExchangeField(kw["rk"], kw["A"], g, kw.get("idx")))
+            idx = None if kw.get("idx") is None else O.scalar_flat(kw["idx"]).astype(np.uint32)
+            oterms.append(O.term(O.RKKY, array0=O.scalar_flat(kw["rk"]), array1=O.scalar_flat(kw["A"]), index0=idx))
+        elif kind == "atom_dipole":
+            gterms.append(M.AtomisticDipoleDipoleField(g))
+            oterms.append(O.term(O.ATOM_DIPOLE))
+        elif kind == "atom_exchange":
+            gterms.append(M.AtomisticExchangeField(kw["J"]))
+            oterms.append(O.term(O.ATOM_EXCHANGE, kw["J"]))
+        elif kind == "atom_dmi":
+            gterms.append(M.AtomisticDmiField(kw["D"], kw.get("axis", (0, 0, -1))))
+            oterms.append(O.term(O.ATOM_DMI, kw["D"], axis=kw.get("axis", (0, 0, -1))))
+        elif kind == "atom_uniaxial":
+            gterms.append(M.AtomisticUniaxialAnisotropyField(kw["K"], kw.get("axis", (0, 0, 1))))
+            oterms.append(O.term(O.ATOM_UNIAXIAL, kw["K"], axis=kw.get("axis", (0, 0, 1))))
+        elif kind == "atom_zeeman_h":
+            gterms.append(M.AtomisticExternalField(homogeneous=kw["h"]))
+            oterms.append(O.term(O.ATOM_ZEEMAN, field=kw["h"]))
+        else:
+            raise ValueError(kind)
+    state = M.State(g, Ms if Ms_field is None else Ms_field, m)
+    sim = O.Sim(mesh, Ms, m, oterms, Ms_field=Ms_field)
+    return state, gterms, sim
+
+
+MICRO_ALL = [("demag", {}), ("exchange", {"A": 1.5e-11}), ("dmi", {"D": 3e-3}),
+             ("uniaxial", {"K": 8e5, "axis": (0.1, 0.2, 1.0)})]
+
+
+@pytest.mark.parametrize("mesh", [(5, 4, 3, 1e-9, 1.2e-9, 0.8e-9), (7, 3, 1, 2e-9, 1e-9, 3e-9),
+                                  (16, 8, 4, 2e-9, 2e-9, 2e-9), (100, 25, 1, 5e-9, 5e-9, 3e-9),
+                                  (33, 7, 5, 1e-9, 2e-9, 1.5e-9), (1, 1, 1, 2e-9, 2e-9, 2e-9)])
+@pytest.mark.parametrize("which", ["demag", "exchange", "dmi", "uniaxial", "all"])
+def test_heff_fp64_matches_oracle(mesh, which):
+    print(f"\n[heff fp64] mesh={mesh} term={which}", flush=True)
+    specs = MICRO_ALL if which == "all" else [s for s in MICRO_ALL if s[0] == which]
+    m = rand_m(mesh[:3], 7)
+    state, terms, sim = build(mesh, 1.1e6, m, specs)
+    M = mf()
+    llg = M.LLGIntegrator(0.1, terms)
+    h = llg.H_in_Apm(state)
+    ref = sim.heff()
+    assert normwise(h, ref) <= FP64_TOL, normwise(h, ref)
+    E_gpu = llg.Eeff_in_J(state)
+    E_ref, Et_ref = sim.energy()
+    assert abs(E_gpu - E_ref) <= FP64_TOL * energy_scale(mesh, 1.1e6, Et_ref)
+
+
+def test_heff_each_term_energy_and_zeeman():
+    mesh = (12, 10, 4, 2e-9, 2e-9, 2e-9)
+    nx, ny, nz = mesh[:3]
+    rng = np.random.default_rng(3)
+    H = rng.standard_normal((nx, ny, nz, 3)) * 1e5
+    specs = MICRO_ALL + [("zeeman", {"H": H})]
+    m = rand_m(mesh[:3], 9)
+    state, terms, sim = build(mesh, 8e5, m, specs)
+    _, Et_ref = sim.energy()
+    for i, t in enumerate(terms):
+        h = t.H_in_Apm(state)
+        ref = sim.heff(mask=1 << i)
+        assert normwise(h, ref) <= FP64_TOL, (i, normwise(h, ref))
+        e = t.Energy_in_J(state)
+        assert abs(e - Et_ref[i]) <= FP64_TOL * energy_scale(mesh, 8e5, Et_ref[i]), (i, e, Et_ref[i])
+
+
+def test_ms_field_airbox():
+    # zero-|m| cells build Ms_field (state.cpp:71-89): demag cube energy in an air box
+    x = y = z = 1e-9
+    nx, ny, nz = 12, 10, 8
+    m = np.zeros((2 * nx, 2 * ny, 2 * nz, 3))
+    m[nx // 2:2 * nx - nx // 2, ny // 2:2 * ny - ny // 2, nz // 2:2 * nz - nz // 2, 0] = 1
+    mesh = (2 * nx, 2 * ny, 2 * nz, x / nx, y / ny, z / nz)
+    M = mf()
+    g = M.Mesh(*mesh)
+    state = M.State(g, 1e5, m)
+    assert state.Ms_field is not None
+    demag = M.DemagField(g)
+    E = demag.Energy_in_J(state)
+    analytic = 1.0 / 6.0 * x * y * z * 1e5**2 * MU0
+    assert abs(E - analytic) <= (E + analytic) / 2 * 5.3e-8  # magnumaf/test/micro_demag.cpp:31-55
+    sim = O.Sim(mesh, 1e5, m, [O.term(O.DEMAG), O.term(O.EXCHANGE, 1e-11)])
+    ex = M.ExchangeField(1e-11)
+    llg = M.LLGIntegrator(0.1, [demag, ex])
+    assert normwise(llg.H_in_Apm(state), sim.heff()) <= FP64_TOL
+    assert abs(llg.Eeff_in_J(state) - sim.energy()[0]) <= FP64_TOL * abs(sim.energy()[1]).max()
+
+
+def test_demag_cube_energy_gpu():
+    # magnumaf/test/micro_demag.cpp:6-29
+    x = y = z = 1e-9
+    nx, ny, nz = 8, 10, 12
+    M = mf()
+    g = M.Mesh(nx, ny, nz, x / nx, y / ny, z / nz)
+    m = np.zeros((nx, ny, nz, 3))
+    m[..., 0] = 1
+    E = M.DemagField(g).Energy_in_J(M.State(g, 1e5, m))
+    analytic = 1.0 / 6.0 * x * y * z * 1e5**2 * MU0
+    assert abs(E - analytic) <= (E + analytic) / 2 * 5.3e-8
+
+
+@pytest.mark.parametrize("mesh", [(16, 8, 4, 2e-9, 2e-9, 2e-9), (64, 32, 1, 3e-9, 3e-9, 3e-9)])
+def test_heff_fp32_field_error(mesh):
+    m = rand_m(mesh[:3], 5)
+    state, terms, sim = build(mesh, 1.1e6, m, MICRO_ALL)
+    M = mf()
+    h32 = M.LLGIntegrator(0.1, terms, precision="f32").H_in_Apm(state)
+    err = normwise(h32, sim.heff())
+    print(f"\n[fp32 field] mesh={mesh} normwise error={err:.3e}", flush=True)
+    assert err <= 1e-5
+
+
+def test_rkky_energy_sweep_gpu():
+    # magnumaf/test/rkky.cpp:11-55 (mumax3 RKKY example) on the GPU path
+    nx, ny, nz, dx = 10, 10, 2, 1e-9
+    mesh = (nx, ny, nz, dx, dx, dx)
+    rk = np.full((nx, ny, nz), -1e-3 * dx / 2.0)
+    A = np.full((nx, ny, nz), 10e-12)
+    m = np.zeros((nx, ny, nz, 3))
+    m[..., 0] = 1
+    state, terms, sim = build(mesh, 1e6, m, [("demag", {}), ("rkky", {"rk": rk, "A": A})])
+    M = mf()
+    llg = M.LLGIntegrator(1.0, terms)
+    Es, Eo = [], []
+    for i in range(0, 360, 3):
+        mm = m.copy()
+        mm[:, :, 1, 0] = math.cos(i * math.pi / 180.0)
+        mm[:, :, 1, 1] = math.sin(i * math.pi / 180.0)
+        state.set_m_raw(mm)
+        sim.set_m(mm, normalize=False)
+        Es.append(llg.Eeff_in_J(state))
+        Eo.append(sim.energy()[0])
+    Es, Eo = np.array(Es), np.array(Eo)
+    assert np.max(np.abs(Es - Eo)) <= FP64_TOL * np.max(np.abs(Eo))
+    h = llg.H_in_Apm(state)
+    assert normwise(h, sim.heff()) <= FP64_TOL
+
+
+def test_rkky_layers_with_indices():
+    nx, ny, nz = 6, 5, 4
+    mesh = (nx, ny, nz, 2e-9, 2e-9, 2e-9)
+    rng = np.random.default_rng(1)
+    rk = np.zeros((nx, ny, nz))
+    rk[:, :, 1:3] = -1e-3 * 2e-9 / 2
+    A = 1e-11 * (1 + 0.5 * rng.random((nx, ny, nz)))
+    idx = np.zeros((nx, ny, nz), dtype=np.uint32)
+    idx[:, :, 2:] = 1
+    m = rand_m((nx, ny, nz), 4)
+    state, terms, sim = build(mesh, 1e6, m, [("rkky", {"rk": rk, "A": A, "idx": idx})])
+    assert normwise(terms[0].H_in_Apm(state), sim.heff()) <= FP64_TOL
+
+
+@pytest.mark.parametrize("mesh", [(2, 1, 1, 2.7e-10, 2.7e-10, 2.7e-10), (16, 12, 1, 2.7e-10, 2.7e-10, 2.7e-10),
+                                  (8, 6, 3, 2.7e-10, 2.7e-10, 2.7e-10)])
+def test_atomistic_terms(mesh):
+    J = 1.6e-21
+    specs = [("atom_dipole", {}), ("atom_exchange", {"J": J}), ("atom_dmi", {"D": 0.2 * J}),
+             ("atom_uniaxial", {"K": 0.02 * J}), ("atom_zeeman_h", {"h": (0, 0, 1 / MU0)})]
+    mus = 2.6 * 9.274009994e-24
+    m = rand_m(mesh[:3], 13)
+    state, terms, sim = build(mesh, mus, m, specs)
+    M = mf()
+    llg = M.LLGIntegrator(0.1, terms)
+    assert normwise(llg.H_in_Apm(state), sim.heff()) <= FP64_TOL
+    E_ref, Et = sim.energy()
+    assert abs(llg.Eeff_in_J(state) - E_ref) <= FP64_TOL * np.max(np.abs(Et))
+
+
+def test_vector_helpers():
+    import ctypes as C
+    import torch
+    M = mf()
+    from magnumaf._abi import check, lib
+    g = M.Mesh(37, 11, 3, 1e-9, 1e-9, 1e-9)
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal(3 * g.n)
+    b = rng.standard_normal(3 * g.n)
+    state = M.State(g, 8e5, rand_m((37, 11, 3), 2))
+    ctx = state._util_ctx()
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    out = C.c_double()
+    check(lib().mafx_dot(ctx.h, C.c_void_p(da.data_ptr()), C.c_void_p(db.data_ptr()), C.byref(out)))
+    assert abs(out.value - float(np.dot(a, b))) <= 1e-13 * np.sum(np.abs(a * b))
+    check(lib().mafx_max_abs(ctx.h, C.c_void_p(da.data_ptr()), C.byref(out)))
+    assert out.value == np.max(np.abs(a))
+    # LLG right-hand side against the oracle's restatement of equations.hpp
+    h = torch.from_numpy(b * 1e5).cuda()
+    k = torch.empty_like(h)
+    check(lib().mafx_llg_rhs(ctx.h, 0.3, 1e-13, C.c_void_p(state.m_dev.data_ptr()), C.c_void_p(h.data_ptr()),
+                             C.c_void_p(k.data_ptr())))
+    mm = state.m_dev.cpu().numpy().reshape(3, -1)
+    hh = b.reshape(3, -1) * 1e5
+    mxh = np.cross(mm.T, hh.T).T
+    mxmxh = np.cross(mm.T, mxh.T).T
+    gam = 1.760859644e11 * MU0
+    kref = 1e-13 * (-gam / (1 + 0.09) * mxh - 0.3 * gam / (1 + 0.09) * mxmxh)
+    assert np.max(np.abs(k.cpu().numpy().reshape(3, -1) - kref)) <= 1e-13 * np.max(np.abs(kref))
+    mean = state.mean_m()
+    assert np.allclose(mean, mm.mean(axis=1), rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("scheme", ["DP45", "RKF45"])
+def test_llg_step_replay(scheme):
+    # identical controller decisions and states, step by step, on a C1-like film
+    mesh = (32, 8, 1, 4e-9, 4e-9, 3e-9)
+    nx, ny, nz = mesh[:3]
+    rng = np.random.default_rng(42)
+    m = np.zeros((nx, ny, nz, 3))
+    m[..., 0] = 1
+    m += 0.1 * rng.standard_normal(m.shape)
+    h = 25e-3 / MU0
+    H = (h * math.cos(math.radians(170)), h * math.sin(math.radians(170)), 0.0)
+    specs = [("demag", {}), ("exchange", {"A": 1.3e-11}), ("zeeman_h", {"h": H})]
+    state, terms, sim = build(mesh, 8e5, m, specs)
+    M = mf()
+    llg = M.LLGIntegrator(0.02, terms, scheme)
+    sim.llg_init(0.02, scheme)
+    # From h = 1.01e-15 the embedded error estimate of the first steps is rounding noise of the
+    # stage sums, and the PI controller turns its relative differences into err^-0.14 changes of
+    # h (~1e-8 between two FP64 summation orders).  The replay therefore requires the same
+    # accept/reject decisions, h within 1e-6 and the state within 1e-8.
+    for i in range(40):
+        r = sim.step()
+        llg.step(state)
+        st = llg.stats()
+        assert abs(st["h_last"] - r["h"]) <= 1e-6 * r["h"], (i, st["h_last"], r["h"])
+        assert abs(state.t - sim.t) <= 1e-6 * sim.t
+        assert st["rejected"] == r["rejected"] and st["accepted"] == r["accepted"]
+    assert np.max(np.abs(state.m - sim.m())) <= 1e-8
+
+
+def test_sp4_relaxation_and_switch_gpu():
+    # python/test/sp4.py:21-62 on the GPU FP64 path (RKF45, the LLGIntegrator default)
+    M = mf()
+    nx, ny, nz = 100, 25, 1
+    g = M.Mesh(nx, ny, nz, 5.0e-7 / 100, 1.25e-7 / 25, 3.0e-9)
+    m = np.zeros((nx, ny, nz, 3))
+    m[1:-1, :, :, 0] = 1
+    m[0, :, :, 1] = 1
+    m[-1, :, :, 1] = 1
+    state = M.State(g, 8e5, m)
+    llg = M.LLGIntegrator(alpha=1, terms=[M.DemagField(g), M.ExchangeField(1.3e-11)])
+    ix = iy = iz = 0.0
+    while state.t < 1e-9:
+        t1 = state.t
+        llg.step(state)
+        dt = state.t - t1
+        mx, my, mz = state.mean_m()
+        ix += mx * dt
+        iy += my * dt
+        iz += mz * dt
+    print(f"\n[sp4 relax] dints = {ix - 9.81206172824e-10:.3e} {iy - 9.14350283169e-11:.3e} "
+          f"{iz + 5.74381785359e-13:.3e}", flush=True)
+    assert abs(ix - 9.81206172824e-10) < 1e-15
+    assert abs(iy - 9.14350283169e-11) < 1e-15
+    assert abs(iz + 5.74381785359e-13) < 1e-15
+    llg.alpha = 0.02
+    z = np.zeros((nx, ny, nz, 3))
+    z[..., 0] = -24.6e-3 / MU0
+    z[..., 1] = +4.3e-3 / MU0
+    llg.add_terms(M.ExternalField(z))
+    ix = iy = iz = 0.0
+    while state.t < 2e-9:
+        t1 = state.t
+        llg.step(state)
+        dt = state.t - t1
+        mx, my, mz = state.mean_m()
+        ix += mx * dt
+        iy += my * dt
+        iz += mz * dt
+    print(f"[sp4 switch] dints = {ix + 6.41261165705e-10:.3e} {iy - 1.47353233738e-10:.3e} "
+          f"{iz + 1.78144535231e-11:.3e}", flush=True)
+    assert abs(ix + 6.41261165705e-10) < 1e-15
+    assert abs(iy - 1.47353233738e-10) < 1e-15
+    assert abs(iz + 1.78144535231e-11) < 1e-15
+
+
+def test_integrate_dense_matches_oracle():
+    mesh = (32, 8, 1, 4e-9, 4e-9, 3e-9)
+    nx, ny, nz = mesh[:3]
+    rng = np.random.default_rng(8)
+    m = np.zeros((nx, ny, nz, 3))
+    m[..., 0] = 1
+    m += 0.1 * rng.standard_normal(m.shape)
+    h = 25e-3 / MU0
+    H = (h * math.cos(math.radians(170)), h * math.sin(math.radians(170)), 0.0)
+    specs = [("demag", {}), ("exchange", {"A": 1.3e-11}), ("zeeman_h", {"h": H})]
+    state, terms, sim = build(mesh, 8e5, m, specs)
+    M = mf()
+    llg = M.LLGIntegrator(0.02, terms, "DP45")
+    sim.llg_init(0.02, "DP45")
+    rows = llg.integrate_dense(state, 5e-11, 1e-11)
+    ref = sim.integrate_dense(5e-11, 1e-11)
+    assert rows.shape == ref.shape
+    assert np.max(np.abs(rows - ref)) <= 1e-10
+
+
+def test_lbfgs_matches_oracle():
+    mesh = (16, 16, 1, 3e-9, 3e-9, 3e-9)
+    m = rand_m(mesh[:3], 21)
+    specs = [("demag", {}), ("exchange", {"A": 1.5e-11}), ("dmi", {"D": 3e-3}), ("uniaxial", {"K": 8e5}),
+             ("zeeman_h", {"h": (0, 0, 0.2 / MU0)})]
+    state, terms, sim = build(mesh, 1.1e6, m, specs)
+    M = mf()
+    E0 = M.LLGIntegrator(0.1, terms).Eeff_in_J(state)
+    mini = M.LBFGS_Minimizer(terms, tol=1e-6, maxiter=60)
+    E = mini.minimize(state)
+    r = sim.lbfgs(tol=1e-6, maxiter=60)
+    print(f"\n[lbfgs] E0={E0:.6e} gpu E={E:.12e} it={mini.last_stats['iterations']} "
+          f"oracle E={r['E']:.12e} it={r['iterations']}", flush=True)
+    assert E < E0
+    assert abs(E - r["E"]) <= 1e-9 * abs(r["E"])
