@@ -212,8 +212,10 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
             ctx->px.build(static_cast<int>(ctx->n0));
             ctx->py.build(static_cast<int>(ctx->n1));
             if (ctx->n2 > 1) ctx->pz.build(static_cast<int>(ctx->n2));
+            // the z transform is fused into the tensor product for 2 <= nz <= 8 (nz planes suffice)
+            ctx->nzs = (mesh->nz >= 2 && mesh->nz <= 8) ? mesh->nz : ctx->n2;
             const size_t csz = precision == MAFX_F64 ? sizeof(double2) : sizeof(float2);
-            ctx->spec.alloc(csz * 3 * ctx->n2 * ctx->n1 * ctx->hx);
+            ctx->spec.alloc(csz * 3 * ctx->nzs * ctx->n1 * ctx->hx);
             build_tensor(ctx.get());
         }
         ctx->device_bytes = static_cast<int64_t>(ctx->ms_field.bytes + ctx->partials.bytes + ctx->scal.bytes +

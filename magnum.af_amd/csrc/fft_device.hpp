@@ -82,15 +82,15 @@ __device__ __forceinline__ void dft(C (&v)[R], bool inv, const C* __restrict__ t
 }
 
 // One Stockham stage over W lines of length L in buf[pos * W + w].
-// MAXB = max butterflies per thread = ceil(W L / R / blockDim).
-template <class C, int R, int MAXB>
+// MAXB = max butterflies per thread = ceil(W L / R / BLOCK).
+template <class C, int R, int MAXB, int BLOCK>
 __device__ __forceinline__ void fft_stage(C* buf, int L, int W, int Ns, const C* __restrict__ tw, bool inv) {
     const int m = L / R;
     const int nb = m * W;
     C v[MAXB][R];
 #pragma unroll
     for (int i = 0; i < MAXB; ++i) {
-        const int b = threadIdx.x + i * kFftBlock;
+        const int b = threadIdx.x + i * BLOCK;
         if (b < nb) {
             const int w = b % W, j = b / W;
 #pragma unroll
@@ -100,7 +100,7 @@ __device__ __forceinline__ void fft_stage(C* buf, int L, int W, int Ns, const C*
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < MAXB; ++i) {
-        const int b = threadIdx.x + i * kFftBlock;
+        const int b = threadIdx.x + i * BLOCK;
         if (b < nb) {
             const int w = b % W, j = b / W;
             const int k = j % Ns;
@@ -125,25 +125,39 @@ struct FftDesc {
     int radix[16];
 };
 
-// Runs all stages.  LDS_ELEMS bounds W * L (the LDS budget in complex elements).
-template <class C, int LDS_ELEMS>
+// Runs all stages.  LDS_ELEMS bounds W * L (the LDS budget in complex elements).  POW2 plans
+// (radices 2, 4, 8 only) get a stage switch without the odd radices, whose register arrays
+// would otherwise set the kernel's VGPR count.
+template <class C, int LDS_ELEMS, bool POW2, int B = kFftBlock>
 __device__ void fft_lds(C* buf, const FftDesc& d, int W, const C* __restrict__ tw, bool inv) {
     int Ns = 1;
     for (int s = 0; s < d.nstages; ++s) {
         const int R = d.radix[s];
         switch (R) {
-        case 2: fft_stage<C, 2, (LDS_ELEMS / 2 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
-        case 3: fft_stage<C, 3, (LDS_ELEMS / 3 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
-        case 4: fft_stage<C, 4, (LDS_ELEMS / 4 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
-        case 5: fft_stage<C, 5, (LDS_ELEMS / 5 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
-        case 7: fft_stage<C, 7, (LDS_ELEMS / 7 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
-        case 8: fft_stage<C, 8, (LDS_ELEMS / 8 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
-        case 11: fft_stage<C, 11, (LDS_ELEMS / 11 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
-        case 13: fft_stage<C, 13, (LDS_ELEMS / 13 + kFftBlock - 1) / kFftBlock>(buf, d.L, W, Ns, tw, inv); break;
-        default: break; // unreachable: FftPlan::build rejects other factors
+        case 2: fft_stage<C, 2, (LDS_ELEMS / 2 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
+        case 4: fft_stage<C, 4, (LDS_ELEMS / 4 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
+        case 8: fft_stage<C, 8, (LDS_ELEMS / 8 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
+        default:
+            if constexpr (!POW2) {
+                switch (R) {
+                case 3: fft_stage<C, 3, (LDS_ELEMS / 3 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
+                case 5: fft_stage<C, 5, (LDS_ELEMS / 5 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
+                case 7: fft_stage<C, 7, (LDS_ELEMS / 7 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
+                case 11: fft_stage<C, 11, (LDS_ELEMS / 11 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
+                case 13: fft_stage<C, 13, (LDS_ELEMS / 13 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
+                default: break; // unreachable: FftPlan::build rejects other factors
+                }
+            }
+            break;
         }
         Ns *= R;
     }
+}
+
+inline bool is_pow2_plan(const FftDesc& d) {
+    for (int s = 0; s < d.nstages; ++s)
+        if (d.radix[s] != 2 && d.radix[s] != 4 && d.radix[s] != 8) return false;
+    return true;
 }
 
 } // namespace mafx

@@ -361,7 +361,7 @@ __device__ void finish_block(const EpiParams& P, double* acc) {
 
 // One work-group per (y, z) row: inverse x transform of the three components (C2R with the
 // Hermitian extension X[L-k] = conj X[k]), crop to x < nx, scale, then the per-cell epilogue.
-template <class C, class T>
+template <class C, class T, bool POW2>
 __global__ void __launch_bounds__(kFftBlock)
     k_epi_rows(EpiParams P, FftDesc d, const C* __restrict__ tw, const C* __restrict__ spec, int64_t hx, int64_t n1,
                int64_t n2, double hscale) {
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(kFftBlock)
             line[k] = v;
         }
         __syncthreads();
-        fft_lds<C, 65536 / static_cast<int>(sizeof(C))>(line, d, 1, tw, true);
+        fft_lds<C, 65536 / static_cast<int>(sizeof(C)), POW2>(line, d, 1, tw, true);
         for (int x = threadIdx.x; x < P.nx; x += kFftBlock)
             res[c * P.nx + x] = static_cast<T>(static_cast<double>(line[x].x) * hscale);
         __syncthreads();
@@ -506,21 +506,25 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
                                : (a.mode == OutMode::ENERGY) ? 0.0 : 24.0;
     if (fft) {
         C* spec = spec_of<C>(ctx);
-        const int64_t hx = ctx->hx, n1 = ctx->n1, n2 = ctx->n2;
-        const int64_t sa = n2 * n1 * hx;
+        const int64_t hx = ctx->hx, n1 = ctx->n1, n2 = ctx->n2, nzs = ctx->nzs;
+        const int64_t sa = nzs * n1 * hx;
         // Ms * m (micro; Ms_field per cell) or m (atomistic dipole) -> x half spectrum
         const double* ms = (!ctx->atom_dipole && ctx->has_ms_field) ? ctx->ms_field.as<double>() : nullptr;
         const double scale = ctx->atom_dipole ? 1.0 : (ctx->has_ms_field ? 1.0 : ctx->Ms);
-        fft_x_forward<C>(ctx, ctx->px, a.m, ms, scale, g.nx, g.ny, g.nz, 3, g.ny, g.nz, spec);
+        fft_x_forward<C>(ctx, ctx->px, a.m, ms, scale, g.nx, g.ny, g.nz, 3, g.ny, g.nz, spec, nzs);
         fft_strided<C>(ctx, ctx->py, spec, sa, 3, n1 * hx, static_cast<int>(g.nz), hx, static_cast<int>(g.ny),
                        static_cast<int>(n1), false, sizeof(C) == 8 ? "fft_y_fwd_f32" : "fft_y_fwd_f64");
-        if (n2 > 1)
+        if (n2 == 1) {
+            kspace_mul<C>(ctx, spec, ctx->tensor.as<C>(), sa);
+        } else if (nzs == g.nz) {
+            zfused<C>(ctx, spec, ctx->tensor.as<C>(), static_cast<int>(g.nz));
+        } else {
             fft_strided<C>(ctx, ctx->pz, spec, sa, 3, hx, static_cast<int>(n1), n1 * hx, static_cast<int>(g.nz),
                            static_cast<int>(n2), false, sizeof(C) == 8 ? "fft_z_fwd_f32" : "fft_z_fwd_f64");
-        kspace_mul<C>(ctx, spec, ctx->tensor.as<C>(), sa);
-        if (n2 > 1)
+            kspace_mul<C>(ctx, spec, ctx->tensor.as<C>(), sa);
             fft_strided<C>(ctx, ctx->pz, spec, sa, 3, hx, static_cast<int>(n1), n1 * hx, static_cast<int>(n2),
                            static_cast<int>(g.nz), true, sizeof(C) == 8 ? "fft_z_inv_f32" : "fft_z_inv_f64");
+        }
         fft_strided<C>(ctx, ctx->py, spec, sa, 3, n1 * hx, static_cast<int>(g.nz), hx, static_cast<int>(n1),
                        static_cast<int>(g.ny), true, sizeof(C) == 8 ? "fft_y_inv_f32" : "fft_y_inv_f64");
         const double P_total = static_cast<double>(ctx->n0) * n1 * n2;
@@ -531,7 +535,9 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
         // algorithmic bytes: the three cropped spectrum rows, m once, the output once
         const double bytes = static_cast<double>(ctx->N) * (3.0 * sizeof(C) * hx / g.nx + 24.0 + field_bytes);
         ProfScope ps(ctx, sizeof(C) == 8 ? "epilogue_rows_f32" : "epilogue_rows_f64", bytes);
-        MAFX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_epi_rows<C, T>),
+        const bool pow2 = is_pow2_plan(desc_of(ctx->px));
+        MAFX_HIP(hipFuncSetAttribute(pow2 ? reinterpret_cast<const void*>(&k_epi_rows<C, T, true>)
+                                          : reinterpret_cast<const void*>(&k_epi_rows<C, T, false>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
         FftDesc d{};
         d.L = ctx->px.L;
@@ -539,8 +545,12 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
         for (int s = 0; s < d.nstages; ++s) d.radix[s] = ctx->px.radix[s];
         const C* tw = sizeof(C) == 8 ? reinterpret_cast<const C*>(ctx->px.tw32.p)
                                      : reinterpret_cast<const C*>(ctx->px.tw64.p);
-        hipLaunchKernelGGL((k_epi_rows<C, T>), dim3(static_cast<unsigned>(nrows)), dim3(kFftBlock), lds, ctx->stream,
-                           P, d, tw, spec, hx, n1, n2, hscale);
+        if (pow2)
+            hipLaunchKernelGGL((k_epi_rows<C, T, true>), dim3(static_cast<unsigned>(nrows)), dim3(kFftBlock), lds,
+                               ctx->stream, P, d, tw, spec, hx, n1, nzs, hscale);
+        else
+            hipLaunchKernelGGL((k_epi_rows<C, T, false>), dim3(static_cast<unsigned>(nrows)), dim3(kFftBlock), lds,
+                               ctx->stream, P, d, tw, spec, hx, n1, nzs, hscale);
         MAFX_HIP(hipGetLastError());
     } else {
         ProfScope ps(ctx, sizeof(T) == 4 ? "local_cells_f32" : "local_cells_f64",
@@ -635,7 +645,7 @@ void build_tensor(mafx_ctx* ctx) {
     Nh.reset();
     T64.alloc(sizeof(double2) * 6 * K);
     double2* t = T64.as<double2>();
-    fft_x_forward<double2>(ctx, ctx->px, Nd.as<double>(), nullptr, 1.0, n0, n1, n2, 6, n1, n2, t);
+    fft_x_forward<double2>(ctx, ctx->px, Nd.as<double>(), nullptr, 1.0, n0, n1, n2, 6, n1, n2, t, n2);
     fft_strided<double2>(ctx, ctx->py, t, K, 6, n1 * hx, static_cast<int>(n2), hx, static_cast<int>(n1),
                          static_cast<int>(n1), false, "tensor_fft_y");
     if (n2 > 1)

@@ -125,6 +125,7 @@ struct mafx_ctx {
     bool atom_dipole{false};
     // FFT geometry: expanded lengths (demag_field.cpp:14-16) and half-spectrum width
     int64_t n0{0}, n1{0}, n2{0}, hx{0};
+    int64_t nzs{0};           // z planes of the spectrum workspace: nz when the z stage is fused, else n2
     mafx::FftPlan px, py, pz;
     mafx::DevBuf tensor;      // [6][n2][n1][hx] complex, precision of the field path
     mafx::DevBuf spec;        // [3][n2][n1][hx] complex workspace
