@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--dp45-steps", type=int, default=12)
     ap.add_argument("--size", default="1024x1024x4")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--heff-only", choices=["f32", "f64"], default=None,
+                    help="run --steps H_eff evaluations of one path and exit (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the sharding design is "
@@ -66,6 +68,14 @@ def main():
 
     t0 = time.time()
     state = M.State(mesh, Ms, m0)
+    if args.heff_only:
+        ctx = state._context(terms, args.heff_only)
+        h = torch.empty(3 * mesh.n, dtype=torch.float64, device="cuda")
+        for _ in range(args.warmup + args.steps):
+            ctx.heff(state.m_dev, out=h)
+        torch.cuda.synchronize()
+        log(f"[bench] {args.steps} {args.heff_only} H_eff evaluations done ({time.time() - t0:.1f} s)")
+        return
     ctx32 = state._context(terms, "f32")
     log(f"[bench] FP32 context ready in {time.time() - t0:.1f} s (includes host Newell tensor)")
     t0 = time.time()

@@ -209,13 +209,17 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
             ctx->n1 = 2 * mesh->ny;
             ctx->n2 = mesh->nz == 1 ? 1 : 2 * mesh->nz;
             ctx->hx = ctx->n0 / 2 + 1;
-            ctx->px.build(static_cast<int>(ctx->n0));
+            ctx->hxs = (ctx->hx + 15) / 16 * 16; // rows start on 128 B (f32) / 256 B (f64) boundaries
+            ctx->px.build(static_cast<int>(ctx->n0));  // twiddles of the length-2 nx packing
+            ctx->pxh.build(static_cast<int>(mesh->nx)); // the half-length x transform
             ctx->py.build(static_cast<int>(ctx->n1));
+            ctx->pyh.build(static_cast<int>(mesh->ny)); // the split y transform
             if (ctx->n2 > 1) ctx->pz.build(static_cast<int>(ctx->n2));
             // the z transform is fused into the tensor product for 2 <= nz <= 8 (nz planes suffice)
             ctx->nzs = (mesh->nz >= 2 && mesh->nz <= 8) ? mesh->nz : ctx->n2;
             const size_t csz = precision == MAFX_F64 ? sizeof(double2) : sizeof(float2);
-            ctx->spec.alloc(csz * 3 * ctx->nzs * ctx->n1 * ctx->hx);
+            ctx->spec.alloc(csz * 3 * ctx->nzs * ctx->n1 * ctx->hxs);
+            MAFX_HIP(hipMemset(ctx->spec.p, 0, ctx->spec.bytes));
             build_tensor(ctx.get());
         }
         ctx->device_bytes = static_cast<int64_t>(ctx->ms_field.bytes + ctx->partials.bytes + ctx->scal.bytes +

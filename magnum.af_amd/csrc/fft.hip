@@ -3,12 +3,18 @@
 // atomistic_dipole_dipole_field.cpp:20-51).
 //
 // Spectrum layout: [comp][kz][ky][kx] complex with kx in [0, hx), hx = n0/2 + 1 (R2C half
-// spectrum along x), ky in [0, n1), kz in [0, nzs) where nzs = nz when the z transform is fused
-// into the tensor product (2 nz <= 16) and nzs = n2 otherwise.  Only rows that can hold non-zero
-// data are touched: the forward x pass writes rows y < ny, z < nz; the forward y pass reads
-// y < ny and writes every ky for z < nz; the fused z kernel reads and writes z < nz; the inverse
-// passes stop at the cropped extent.
+// spectrum along x) and a row stride hxs >= hx padded to 16 elements; ky in [0, n1); kz in
+// [0, nzs) where nzs = nz when the z transform is fused into the tensor product (2 <= nz <= 8)
+// and nzs = n2 otherwise.  Only rows that can hold non-zero data are touched: the forward x pass
+// writes rows y < ny, z < nz; the forward y pass reads y < ny and writes every ky for z < nz; the
+// fused z kernel reads and writes z < nz; the inverse passes stop at the cropped extent.
+//
+// Both x passes run a complex FFT of half the padded length on the packed sequence
+// z[n] = x[2n] + i x[2n+1] (the standard real-FFT packing); the zero-padded y passes split the
+// length-2M transform into its even and odd outputs, two length-M transforms.
 #include <cmath>
+#include <map>
+#include <mutex>
 
 #include "fft_device.hpp"
 #include "fft_kernels.hpp"
@@ -63,57 +69,86 @@ FftDesc desc_of(const FftPlan& p) {
     FftDesc d{};
     d.L = p.L;
     d.nstages = p.nstages;
-    for (int i = 0; i < p.nstages; ++i) d.radix[i] = p.radix[i];
+    int ns = 1;
+    for (int i = 0; i < p.nstages; ++i) {
+        d.radix[i] = p.radix[i];
+        d.ns[i] = make_fastdiv(static_cast<uint32_t>(ns));
+        d.twstep[i] = p.L / (ns * p.radix[i]);
+        ns *= p.radix[i];
+    }
     return d;
 }
 
 namespace {
 
-template <class C> const C* tw_of(const FftPlan& p);
-template <> const double2* tw_of<double2>(const FftPlan& p) { return p.tw64.as<double2>(); }
-template <> const float2* tw_of<float2>(const FftPlan& p) { return p.tw32.as<float2>(); }
-
-template <class C> constexpr int lds_elems() { return 65536 / static_cast<int>(sizeof(C)); }
-
 // 512-thread work-groups for the axis transforms: W L <= 64 KiB of complex data spread over twice
 // the lanes halves the butterflies each lane keeps in registers across a stage.
 constexpr int kAxisBlock = 512;
 
-// Forward real-to-complex transform along x of rows (c, z, y), zero-padded from Xi to L.
-// in[((c Zi + z) Yi + y) Xi + x] * scale * (ms ? ms[(z Yi + y) Xi + x] : 1); rows land at
-// out[((c n2o + z) n1 + y) hx + k].
+// Forward real-to-complex transform along x of rows (c, z, y): padded length L = 2 M, input
+// x[t] = in[((c Zi + z) Yi + y) Xi + t] * scale * (ms ? ms[cell] : 1) for t < Xi, 0 beyond.
+// Z = FFT_M(x[2n] + i x[2n+1]);  X[k] = E + w_L^k O,  E = (Z[k] + conj Z[M-k]) / 2,
+// O = -i (Z[k] - conj Z[M-k]) / 2, k = 0..M, written to out[((c n2o + z) n1 + y) hxs + k].
+// Lanes walk along rows (consecutive lanes, consecutive addresses) for the global accesses.  The
+// (c, z, y) decomposition of each of the block's W <= 64 rows is done once, into LDS.
 template <class C, bool POW2>
 __global__ void __launch_bounds__(kAxisBlock)
-    k_fft_x_fwd(const double* __restrict__ in, const double* __restrict__ ms, double scale, int64_t Xi, int64_t Yi,
-                int64_t Zi, int64_t Yr, int64_t Zr, int64_t rows, int W, FftDesc d, const C* __restrict__ tw,
-                C* __restrict__ out, int64_t hx, int64_t n1, int64_t n2o) {
+    k_fft_x_fwd(const double* __restrict__ in, const double* __restrict__ ms, double scale, int Xi, int64_t Yi,
+                int64_t Zi, FastDiv Yr, FastDiv Zr, int rows, FastDiv Wd, FftDesc dh, FastDiv Md, FastDiv M1d,
+                const C* __restrict__ twh, const C* __restrict__ twL, C* __restrict__ out, int64_t hxs, int64_t n1,
+                int64_t n2o) {
     using T = typename real_of<C>::type;
     extern __shared__ unsigned char smem_raw[];
+    __shared__ int64_t s_src[64], s_cell[64], s_dst[64];
     C* buf = reinterpret_cast<C*>(smem_raw);
-    const int L = d.L;
-    const int64_t row0 = static_cast<int64_t>(blockIdx.x) * W;
-    for (int e = threadIdx.x; e < L * W; e += kAxisBlock) {
-        const int w = e % W, pos = e / W;
-        const int64_t r = row0 + w;
-        T v = 0;
-        if (r < rows && pos < Xi) {
-            const int64_t y = r % Yr, zc = r / Yr, z = zc % Zr, c = zc / Zr;
-            const int64_t cell = (z * Yi + y) * Xi + pos;
-            double val = in[(c * Zi * Yi) * Xi + cell] * scale;
-            if (ms) val *= ms[cell];
-            v = static_cast<T>(val);
+    const int M = dh.L;
+    const int W = static_cast<int>(Wd.d);
+    const int row0 = static_cast<int>(blockIdx.x) * W;
+    if (static_cast<int>(threadIdx.x) < W) {
+        const int r = row0 + static_cast<int>(threadIdx.x);
+        int64_t src = -1, cell0 = 0, dst = -1;
+        if (r < rows) {
+            const uint32_t zc = fdiv(r, Yr), y = r - zc * Yr.d;
+            const uint32_t c = fdiv(zc, Zr), z = zc - c * Zr.d;
+            cell0 = (z * Yi + y) * Xi;
+            src = c * Zi * Yi * Xi + cell0;
+            dst = ((c * n2o + z) * n1 + y) * hxs;
         }
-        buf[pos * W + w] = C{v, 0};
+        s_src[threadIdx.x] = src;
+        s_cell[threadIdx.x] = cell0;
+        s_dst[threadIdx.x] = dst;
     }
     __syncthreads();
-    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, d, W, tw, false);
-    for (int e = threadIdx.x; e < hx * W; e += kAxisBlock) {
-        const int w = e % W, k = e / W;
-        const int64_t r = row0 + w;
-        if (r < rows) {
-            const int64_t y = r % Yr, zc = r / Yr, z = zc % Zr, c = zc / Zr;
-            out[((c * n2o + z) * n1 + y) * hx + k] = buf[k * W + w];
+    for (int e = threadIdx.x; e < M * W; e += kAxisBlock) {
+        const int w = static_cast<int>(fdiv(e, Md)), n = e - w * M;
+        const int64_t so = s_src[w];
+        T re = 0, im = 0;
+        if (so >= 0) {
+            const double* src = in + so;
+            const int t0 = 2 * n, t1 = t0 + 1;
+            if (ms) {
+                const double* mrow = ms + s_cell[w];
+                if (t0 < Xi) re = static_cast<T>(src[t0] * scale * mrow[t0]);
+                if (t1 < Xi) im = static_cast<T>(src[t1] * scale * mrow[t1]);
+            } else {
+                if (t0 < Xi) re = static_cast<T>(src[t0] * scale);
+                if (t1 < Xi) im = static_cast<T>(src[t1] * scale);
+            }
         }
+        buf[lds_pad(n * W + w)] = C{re, im};
+    }
+    __syncthreads();
+    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, dh, Wd, twh, false);
+    const T half = static_cast<T>(0.5);
+    for (int e = threadIdx.x; e < (M + 1) * W; e += kAxisBlock) {
+        const int w = static_cast<int>(fdiv(e, M1d)), k = e - w * (M + 1);
+        const int64_t dst = s_dst[w];
+        if (dst < 0) continue;
+        const C Zk = buf[lds_pad((k == M ? 0 : k) * W + w)];
+        const C Zm = buf[lds_pad((k == 0 ? 0 : M - k) * W + w)];
+        const C E{half * (Zk.x + Zm.x), half * (Zk.y - Zm.y)};
+        const C O{half * (Zk.y + Zm.y), -half * (Zk.x - Zm.x)};
+        out[dst + k] = cadd(E, cmul(twL[k], O));
     }
 }
 
@@ -123,24 +158,116 @@ __global__ void __launch_bounds__(kAxisBlock)
 template <class C, bool POW2>
 __global__ void __launch_bounds__(kAxisBlock)
     k_fft_strided(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, int n_in, int n_out,
-                  int W, FftDesc d, const C* __restrict__ tw, int inv) {
+                  FastDiv Wd, FftDesc d, const C* __restrict__ tw, int inv) {
     extern __shared__ unsigned char smem_raw[];
     C* buf = reinterpret_cast<C*>(smem_raw);
     const int L = d.L;
-    const int64_t kx0 = static_cast<int64_t>(blockIdx.x) * W;
-    C* base = data + static_cast<int64_t>(blockIdx.z) * sa + static_cast<int64_t>(blockIdx.y) * sb + kx0;
+    const int W = static_cast<int>(Wd.d);
+    const uint3 blk = swizzled_block();
+    const int64_t kx0 = static_cast<int64_t>(blk.x) * W;
+    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + kx0;
     const int wv = static_cast<int>(hx - kx0 < W ? hx - kx0 : W);
     for (int e = threadIdx.x; e < L * W; e += kAxisBlock) {
-        const int w = e % W, pos = e / W;
+        const int pos = static_cast<int>(fdiv(e, Wd)), w = e - pos * W;
         C v{0, 0};
         if (w < wv && pos < n_in) v = base[pos * stride + w];
-        buf[pos * W + w] = v;
+        buf[lds_pad(pos * W + w)] = v;
     }
     __syncthreads();
-    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, d, W, tw, inv != 0);
+    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, d, Wd, tw, inv != 0);
     for (int e = threadIdx.x; e < n_out * W; e += kAxisBlock) {
-        const int w = e % W, pos = e / W;
-        if (w < wv) base[pos * stride + w] = buf[pos * W + w];
+        const int pos = static_cast<int>(fdiv(e, Wd)), w = e - pos * W;
+        if (w < wv) base[pos * stride + w] = buf[lds_pad(pos * W + w)];
+    }
+}
+
+// Forward transform of zero-padded strided lines (inputs pos < M, L = 2 M outputs) as two
+// length-M transforms: X[2k] = FFT_M(x)[k], X[2k+1] = FFT_M(x[n] w_L^n)[k].  The inputs are kept
+// in registers between the halves, so rows may be overwritten in place.
+template <class C, bool POW2>
+__global__ void __launch_bounds__(kAxisBlock)
+    k_fft_y_fwd_split(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, FastDiv Wd, FftDesc dh,
+                      const C* __restrict__ twh, const C* __restrict__ twL) {
+    constexpr int MAXE = (lds_elems<C>() + kAxisBlock - 1) / kAxisBlock;
+    extern __shared__ unsigned char smem_raw[];
+    C* buf = reinterpret_cast<C*>(smem_raw);
+    const int M = dh.L;
+    const int W = static_cast<int>(Wd.d);
+    const uint3 blk = swizzled_block();
+    const int64_t kx0 = static_cast<int64_t>(blk.x) * W;
+    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + kx0;
+    const int wv = static_cast<int>(hx - kx0 < W ? hx - kx0 : W);
+    C v[MAXE];
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+        const int e = threadIdx.x + i * kAxisBlock;
+        if (e < M * W) {
+            const int n = static_cast<int>(fdiv(e, Wd)), w = e - n * W;
+            v[i] = w < wv ? base[n * stride + w] : C{0, 0};
+            buf[lds_pad(e)] = v[i];
+        }
+    }
+    __syncthreads();
+    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, dh, Wd, twh, false);
+    for (int e = threadIdx.x; e < M * W; e += kAxisBlock) {
+        const int k = static_cast<int>(fdiv(e, Wd)), w = e - k * W;
+        if (w < wv) base[(2 * k) * stride + w] = buf[lds_pad(e)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+        const int e = threadIdx.x + i * kAxisBlock;
+        if (e < M * W) buf[lds_pad(e)] = cmul(v[i], twL[fdiv(e, Wd)]);
+    }
+    __syncthreads();
+    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, dh, Wd, twh, false);
+    for (int e = threadIdx.x; e < M * W; e += kAxisBlock) {
+        const int k = static_cast<int>(fdiv(e, Wd)), w = e - k * W;
+        if (w < wv) base[(2 * k + 1) * stride + w] = buf[lds_pad(e)];
+    }
+}
+
+// Inverse transform of strided lines of length L = 2 M keeping outputs n < M:
+// x[n] = IFFT_M(X_even)[n] + w_L^-n IFFT_M(X_odd)[n].
+template <class C, bool POW2>
+__global__ void __launch_bounds__(kAxisBlock)
+    k_fft_y_inv_split(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, FastDiv Wd, FftDesc dh,
+                      const C* __restrict__ twh, const C* __restrict__ twL) {
+    constexpr int MAXE = (lds_elems<C>() + kAxisBlock - 1) / kAxisBlock;
+    extern __shared__ unsigned char smem_raw[];
+    C* buf = reinterpret_cast<C*>(smem_raw);
+    const int M = dh.L;
+    const int W = static_cast<int>(Wd.d);
+    const uint3 blk = swizzled_block();
+    const int64_t kx0 = static_cast<int64_t>(blk.x) * W;
+    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + kx0;
+    const int wv = static_cast<int>(hx - kx0 < W ? hx - kx0 : W);
+    for (int e = threadIdx.x; e < M * W; e += kAxisBlock) {
+        const int k = static_cast<int>(fdiv(e, Wd)), w = e - k * W;
+        buf[lds_pad(e)] = w < wv ? base[(2 * k) * stride + w] : C{0, 0};
+    }
+    __syncthreads();
+    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, dh, Wd, twh, true);
+    C v[MAXE];
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+        const int e = threadIdx.x + i * kAxisBlock;
+        if (e < M * W) v[i] = buf[lds_pad(e)];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < M * W; e += kAxisBlock) {
+        const int k = static_cast<int>(fdiv(e, Wd)), w = e - k * W;
+        buf[lds_pad(e)] = w < wv ? base[(2 * k + 1) * stride + w] : C{0, 0};
+    }
+    __syncthreads();
+    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, dh, Wd, twh, true);
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+        const int e = threadIdx.x + i * kAxisBlock;
+        if (e < M * W) {
+            const int n = static_cast<int>(fdiv(e, Wd)), w = e - n * W;
+            if (w < wv) base[n * stride + w] = cadd(v[i], cmul(conj_if(twL[n], true), buf[lds_pad(e)]));
+        }
     }
 }
 
@@ -211,9 +338,118 @@ __global__ void __launch_bounds__(256)
         for (int z = 0; z < NZ; ++z) s[(c * NZ + z) * plane + col] = h[c][z];
 }
 
-__global__ void k_c64_to_c32(const double2* __restrict__ a, float2* __restrict__ b, int64_t n) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = float2{static_cast<float>(a[i].x), static_cast<float>(a[i].y)};
+// ---- FP32 path: the real, mirror-symmetric tensor spectrum ------------------------------------
+// The padded tensor [6][n2][n1][n0] holds the pair distance j for j < n/2 and j - n above, so the
+// planes x = n0/2, y = n1/2, z = n2/2 (distance -n/2) have no mirror image.  No cell pair of the
+// cropped convolution is that far apart, so zeroing them leaves the cropped field unchanged, and
+// then each component is even or odd along every axis: xx, yy, zz even; xy odd in x and y; xz odd
+// in x and z; yz odd in y and z.  The spectrum is then real, with N^(-ky) = sy N^(ky) and
+// N^(-kz) = sz N^(kz), sy = -1 for xy and yz, sz = -1 for xz and yz.  Stored as
+// Ts[comp][kz <= n2/2][ky <= n1/2][kx] (row stride hxs): a quarter of the k-points, real.
+__global__ void k_zero_unpaired(double* __restrict__ N, int64_t n0, int64_t n1, int64_t n2, int64_t rows) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int64_t y = r % n1, z = (r / n1) % n2;
+    double* row = N + r * n0;
+    if (y == n1 / 2 || (n2 > 1 && z == n2 / 2)) {
+        for (int64_t x = 0; x < n0; ++x) row[x] = 0.0;
+    } else {
+        row[n0 / 2] = 0.0;
+    }
+}
+
+__global__ void k_tensor_sym_f32(const double2* __restrict__ T, float* __restrict__ Ts, int64_t n1, int64_t n2,
+                                 int64_t hx, int64_t hxs, int64_t n1h, int64_t n2h, int64_t total) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const int64_t kx = e % hxs, r = e / hxs, ky = r % n1h, r2 = r / n1h, kz = r2 % n2h, c = r2 / n2h;
+    Ts[e] = kx < hx ? static_cast<float>(T[((c * n2 + kz) * n1 + ky) * hxs + kx].x) : 0.0f;
+}
+
+// H^ = N^ M^ with the symmetric tensor, in place on the spectrum [3][n2][n1][hxs]; one thread per
+// k-point (kz, ky, kx < hx).
+__global__ void __launch_bounds__(256)
+    k_kspace_mul_sym(float2* __restrict__ s, const float* __restrict__ t, int n1, int n2, FastDiv hxd, FastDiv n1d,
+                     int hxs, int n1h, int n2h, int total) {
+    const int e = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (e >= total) return;
+    const int r = static_cast<int>(fdiv(e, hxd)), kx = e - r * static_cast<int>(hxd.d);
+    const int kz = static_cast<int>(fdiv(r, n1d)), ky = r - kz * n1;
+    const bool my = ky >= n1h, mz = kz >= n2h;
+    const int kys = my ? n1 - ky : ky, kzs = mz ? n2 - kz : kz;
+    const float sy = my ? -1.0f : 1.0f, sz = mz ? -1.0f : 1.0f;
+    const int64_t tc = static_cast<int64_t>(n2h) * n1h * hxs;
+    const int64_t to = (static_cast<int64_t>(kzs) * n1h + kys) * hxs + kx;
+    const float nxx = t[to], nxy = sy * t[tc + to], nxz = sz * t[2 * tc + to], nyy = t[3 * tc + to],
+                nyz = sy * sz * t[4 * tc + to], nzz = t[5 * tc + to];
+    const int64_t K = static_cast<int64_t>(n2) * n1 * hxs;
+    const int64_t so = (static_cast<int64_t>(kz) * n1 + ky) * hxs + kx;
+    const float2 m0 = s[so], m1 = s[K + so], m2 = s[2 * K + so];
+    s[so] = float2{nxx * m0.x + nxy * m1.x + nxz * m2.x, nxx * m0.y + nxy * m1.y + nxz * m2.y};
+    s[K + so] = float2{nxy * m0.x + nyy * m1.x + nyz * m2.x, nxy * m0.y + nyy * m1.y + nyz * m2.y};
+    s[2 * K + so] = float2{nxz * m0.x + nyz * m1.x + nzz * m2.x, nxz * m0.y + nyz * m1.y + nzz * m2.y};
+}
+
+// Fused z stage with the symmetric tensor.  One thread per (kys <= n1/2, kx < hx): it loads the
+// tensor column once (6 (NZ + 1) reals) and serves both spectrum columns ky = kys and ky = n1 - kys.
+template <int NZ>
+__global__ void __launch_bounds__(256)
+    k_zfused_sym(float2* __restrict__ s, const float* __restrict__ t, int n1, FastDiv hxd, int hxs, int n1h,
+                 ZTw<float2> tw) {
+    constexpr int NZ2 = 2 * NZ, NH = NZ + 1;
+    const int e = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    const int kys = static_cast<int>(fdiv(e, hxd)), kx = e - kys * static_cast<int>(hxd.d);
+    if (kys >= n1h) return;
+    const int64_t plane = static_cast<int64_t>(n1) * hxs;
+    const int64_t tc = static_cast<int64_t>(NH) * n1h * hxs;
+    float tv[6][NH];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int k = 0; k < NH; ++k) tv[c][k] = t[c * tc + (static_cast<int64_t>(k) * n1h + kys) * hxs + kx];
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1 && (kys == 0 || kys == n1h - 1)) break;
+        const int ky = pass == 0 ? kys : n1 - kys;
+        const float sy = pass == 0 ? 1.0f : -1.0f;
+        const int64_t col = static_cast<int64_t>(ky) * hxs + kx;
+        float2 m[3][NZ], h[3][NZ];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int z = 0; z < NZ; ++z) {
+                m[c][z] = s[(c * NZ + z) * plane + col];
+                h[c][z] = float2{0.0f, 0.0f};
+            }
+#pragma unroll
+        for (int k = 0; k < NZ2; ++k) {
+            float2 X[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float2 acc = m[c][0];
+#pragma unroll
+                for (int z = 1; z < NZ; ++z) acc = cadd(acc, cmul(m[c][z], tw.w[(k * z) % NZ2]));
+                X[c] = acc;
+            }
+            const int kzs = k <= NZ ? k : NZ2 - k;
+            const float sz = k <= NZ ? 1.0f : -1.0f;
+            const float nxx = tv[0][kzs], nxy = sy * tv[1][kzs], nxz = sz * tv[2][kzs], nyy = tv[3][kzs],
+                        nyz = sy * sz * tv[4][kzs], nzz = tv[5][kzs];
+            const float2 H0{nxx * X[0].x + nxy * X[1].x + nxz * X[2].x, nxx * X[0].y + nxy * X[1].y + nxz * X[2].y};
+            const float2 H1{nxy * X[0].x + nyy * X[1].x + nyz * X[2].x, nxy * X[0].y + nyy * X[1].y + nyz * X[2].y};
+            const float2 H2{nxz * X[0].x + nyz * X[1].x + nzz * X[2].x, nxz * X[0].y + nyz * X[1].y + nzz * X[2].y};
+#pragma unroll
+            for (int z = 0; z < NZ; ++z) {
+                const float2 w = conj_if(tw.w[(k * z) % NZ2], true);
+                h[0][z] = cadd(h[0][z], z == 0 ? H0 : cmul(H0, w));
+                h[1][z] = cadd(h[1][z], z == 0 ? H1 : cmul(H1, w));
+                h[2][z] = cadd(h[2][z], z == 0 ? H2 : cmul(H2, w));
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int z = 0; z < NZ; ++z) s[(c * NZ + z) * plane + col] = h[c][z];
+    }
 }
 
 int lines_per_block(int L, int elems) {
@@ -223,25 +459,44 @@ int lines_per_block(int L, int elems) {
     return W;
 }
 
+const char* nm(const char* f32, const char* f64, size_t csz) { return csz == 8 ? f32 : f64; }
+
+// Raises a kernel's dynamic LDS limit above the 64 KiB default once per kernel (gfx950: 160 KiB).
+void allow_lds(const void* fn, size_t bytes) {
+    static std::mutex mu;
+    static std::map<const void*, size_t> granted;
+    if (bytes <= 65536) return;
+    require(bytes <= 160 * 1024, "kernel LDS request exceeds 160 KiB");
+    std::lock_guard<std::mutex> lk(mu);
+    size_t& g = granted[fn];
+    if (g >= bytes) return;
+    MAFX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+    g = bytes;
+}
+
 } // namespace
 
 template <class C>
-void fft_x_forward(mafx_ctx* ctx, const FftPlan& p, const double* in, const double* ms, double scale, int64_t Xi,
-                   int64_t Yi, int64_t Zi, int ncomp, int64_t Yr, int64_t Zr, C* out, int64_t n2o) {
-    const int L = p.L;
-    require(L <= lds_elems<C>(), "x transform length exceeds the LDS budget");
-    const int W = lines_per_block(L, lds_elems<C>());
+void fft_x_forward(mafx_ctx* ctx, const double* in, const double* ms, double scale, int64_t Xi, int64_t Yi,
+                   int64_t Zi, int ncomp, int64_t Yr, int64_t Zr, C* out, int64_t n2o) {
+    const FftPlan& ph = ctx->pxh;
+    const int M = ph.L;
+    require(M <= lds_elems<C>(), "x transform length exceeds the LDS budget");
+    const int W = lines_per_block(M, lds_elems<C>());
     const int64_t rows = static_cast<int64_t>(ncomp) * Zr * Yr;
+    require(rows * (M + 1) < (int64_t{1} << 31) && Xi < (int64_t{1} << 30), "x transform: too many rows");
     const unsigned grid = static_cast<unsigned>((rows + W - 1) / W);
     const double bytes = static_cast<double>(rows) * (Xi * 8.0 + ctx->hx * sizeof(C));
-    const FftDesc d = desc_of(p);
-    ProfScope ps(ctx, sizeof(C) == 8 ? "fft_x_fwd_f32" : "fft_x_fwd_f64", bytes);
-    if (is_pow2_plan(d))
-        hipLaunchKernelGGL((k_fft_x_fwd<C, true>), dim3(grid), dim3(kAxisBlock), W * L * sizeof(C), ctx->stream, in,
-                           ms, scale, Xi, Yi, Zi, Yr, Zr, rows, W, d, tw_of<C>(p), out, ctx->hx, ctx->n1, n2o);
-    else
-        hipLaunchKernelGGL((k_fft_x_fwd<C, false>), dim3(grid), dim3(kAxisBlock), W * L * sizeof(C), ctx->stream, in,
-                           ms, scale, Xi, Yi, Zi, Yr, Zr, rows, W, d, tw_of<C>(p), out, ctx->hx, ctx->n1, n2o);
+    const FftDesc d = desc_of(ph);
+    ProfScope ps(ctx, nm("fft_x_fwd_f32", "fft_x_fwd_f64", sizeof(C)), bytes);
+    const C* twL = tw_of<C>(ctx->px);
+    auto fn = is_pow2_plan(d) ? k_fft_x_fwd<C, true> : k_fft_x_fwd<C, false>;
+    const size_t lds = lds_padded(W * M) * sizeof(C);
+    allow_lds(reinterpret_cast<const void*>(fn), lds);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kAxisBlock), lds, ctx->stream, in, ms, scale, static_cast<int>(Xi), Yi, Zi,
+                       make_fastdiv(static_cast<uint32_t>(Yr)), make_fastdiv(static_cast<uint32_t>(Zr)),
+                       static_cast<int>(rows), make_fastdiv(W), d, make_fastdiv(M), make_fastdiv(M + 1), tw_of<C>(ph),
+                       twL, out, ctx->hxs, ctx->n1, n2o);
     MAFX_HIP(hipGetLastError());
 }
 
@@ -256,23 +511,46 @@ void fft_strided(mafx_ctx* ctx, const FftPlan& p, C* data, int64_t sa, int na, i
     const double bytes = static_cast<double>(na) * nb * ctx->hx * (n_in + n_out) * sizeof(C);
     const FftDesc d = desc_of(p);
     ProfScope ps(ctx, name, bytes);
-    if (is_pow2_plan(d))
-        hipLaunchKernelGGL((k_fft_strided<C, true>), dim3(gx, nb, na), dim3(kAxisBlock), W * L * sizeof(C),
-                           ctx->stream, data, sa, sb, stride, ctx->hx, n_in, n_out, W, d, tw_of<C>(p), inv ? 1 : 0);
-    else
-        hipLaunchKernelGGL((k_fft_strided<C, false>), dim3(gx, nb, na), dim3(kAxisBlock), W * L * sizeof(C),
-                           ctx->stream, data, sa, sb, stride, ctx->hx, n_in, n_out, W, d, tw_of<C>(p), inv ? 1 : 0);
+    auto fn = is_pow2_plan(d) ? k_fft_strided<C, true> : k_fft_strided<C, false>;
+    const size_t lds = lds_padded(W * L) * sizeof(C);
+    allow_lds(reinterpret_cast<const void*>(fn), lds);
+    hipLaunchKernelGGL(fn, dim3(gx, nb, na), dim3(kAxisBlock), lds, ctx->stream, data, sa, sb, stride, ctx->hx, n_in,
+                       n_out, make_fastdiv(W), d, tw_of<C>(p), inv ? 1 : 0);
+    MAFX_HIP(hipGetLastError());
+}
+
+template <class C>
+void fft_y_split(mafx_ctx* ctx, C* data, int64_t sa, int na, int64_t sb, int nb, int64_t stride, bool inv) {
+    const FftPlan& ph = ctx->pyh;
+    const int M = ph.L;
+    require(M <= lds_elems<C>(), "y transform length exceeds the LDS budget");
+    require(nb <= 65535 && na <= 65535, "y transform grid too large");
+    const int W = lines_per_block(M, lds_elems<C>());
+    const unsigned gx = static_cast<unsigned>((ctx->hx + W - 1) / W);
+    const double bytes = static_cast<double>(na) * nb * ctx->hx * (3.0 * M) * sizeof(C);
+    const FftDesc d = desc_of(ph);
+    const C* twL = tw_of<C>(ctx->py);
+    ProfScope ps(ctx, inv ? nm("fft_y_inv_f32", "fft_y_inv_f64", sizeof(C)) : nm("fft_y_fwd_f32", "fft_y_fwd_f64", sizeof(C)),
+                 bytes);
+    const dim3 g(gx, nb, na), b(kAxisBlock);
+    const size_t lds = lds_padded(W * M) * sizeof(C);
+    const bool p2 = is_pow2_plan(d);
+    auto fn = inv ? (p2 ? k_fft_y_inv_split<C, true> : k_fft_y_inv_split<C, false>)
+                  : (p2 ? k_fft_y_fwd_split<C, true> : k_fft_y_fwd_split<C, false>);
+    allow_lds(reinterpret_cast<const void*>(fn), lds);
+    hipLaunchKernelGGL(fn, g, b, lds, ctx->stream, data, sa, sb, stride, ctx->hx, make_fastdiv(W), d, tw_of<C>(ph),
+                       twL);
     MAFX_HIP(hipGetLastError());
 }
 
 template <class C> void kspace_mul(mafx_ctx* ctx, C* spec, const C* tensor, int64_t K) {
-    ProfScope ps(ctx, sizeof(C) == 8 ? "kspace_mul_f32" : "kspace_mul_f64", static_cast<double>(K) * 12 * sizeof(C));
+    ProfScope ps(ctx, nm("kspace_mul_f32", "kspace_mul_f64", sizeof(C)), static_cast<double>(K) * 12 * sizeof(C));
     hipLaunchKernelGGL(k_kspace_mul<C>, dim3(grid_for(K, 256)), dim3(256), 0, ctx->stream, spec, tensor, K);
     MAFX_HIP(hipGetLastError());
 }
 
 template <class C> void zfused(mafx_ctx* ctx, C* spec, const C* tensor, int nz) {
-    const int64_t plane = ctx->n1 * ctx->hx;
+    const int64_t plane = ctx->n1 * ctx->hxs;
     ZTw<C> tw{};
     const int n2 = 2 * nz;
     const long double two_pi = 6.283185307179586476925286766559L;
@@ -281,8 +559,8 @@ template <class C> void zfused(mafx_ctx* ctx, C* spec, const C* tensor, int nz) 
         using T = typename real_of<C>::type;
         tw.w[t] = C{static_cast<T>(std::cos(a)), static_cast<T>(-std::sin(a))};
     }
-    const double bytes = static_cast<double>(plane) * (2.0 * 3 * nz + 6.0 * n2) * sizeof(C);
-    ProfScope ps(ctx, sizeof(C) == 8 ? "zfused_mul_f32" : "zfused_mul_f64", bytes);
+    const double bytes = static_cast<double>(ctx->n1 * ctx->hx) * (2.0 * 3 * nz + 6.0 * n2) * sizeof(C);
+    ProfScope ps(ctx, nm("zfused_mul_f32", "zfused_mul_f64", sizeof(C)), bytes);
     const dim3 g(grid_for(plane, 256)), b(256);
     switch (nz) {
     case 2: hipLaunchKernelGGL((k_zfused<C, 2>), g, b, 0, ctx->stream, spec, tensor, plane, tw); break;
@@ -297,22 +575,73 @@ template <class C> void zfused(mafx_ctx* ctx, C* spec, const C* tensor, int nz) 
     MAFX_HIP(hipGetLastError());
 }
 
-void convert_tensor_f32(mafx_ctx* ctx, const double2* a, float2* b, int64_t n) {
-    hipLaunchKernelGGL(k_c64_to_c32, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, a, b, n);
+void zero_unpaired_planes(mafx_ctx* ctx, double* N, int ncomp) {
+    const int64_t rows = static_cast<int64_t>(ncomp) * ctx->n2 * ctx->n1;
+    hipLaunchKernelGGL(k_zero_unpaired, dim3(grid_for(rows, 256)), dim3(256), 0, ctx->stream, N, ctx->n0, ctx->n1,
+                       ctx->n2, rows);
     MAFX_HIP(hipGetLastError());
 }
 
-template void fft_x_forward<double2>(mafx_ctx*, const FftPlan&, const double*, const double*, double, int64_t,
-                                     int64_t, int64_t, int, int64_t, int64_t, double2*, int64_t);
-template void fft_x_forward<float2>(mafx_ctx*, const FftPlan&, const double*, const double*, double, int64_t,
-                                    int64_t, int64_t, int, int64_t, int64_t, float2*, int64_t);
+void tensor_sym_f32(mafx_ctx* ctx, const double2* T, float* Ts) {
+    const int64_t total = 6 * ctx->n2h * ctx->n1h * ctx->hxs;
+    hipLaunchKernelGGL(k_tensor_sym_f32, dim3(grid_for(total, 256)), dim3(256), 0, ctx->stream, T, Ts, ctx->n1,
+                       ctx->n2, ctx->hx, ctx->hxs, ctx->n1h, ctx->n2h, total);
+    MAFX_HIP(hipGetLastError());
+}
+
+void kspace_mul_sym(mafx_ctx* ctx, float2* spec, const float* ts, int64_t n2) {
+    const int64_t total = n2 * ctx->n1 * ctx->hx;
+    require(total < (int64_t{1} << 31), "spectrum too large for the symmetric tensor product");
+    const double bytes = static_cast<double>(total) * (6.0 * sizeof(float2)) +
+                         6.0 * sizeof(float) * static_cast<double>(ctx->n2h * ctx->n1h * ctx->hx);
+    ProfScope ps(ctx, "kspace_mul_sym_f32", bytes);
+    hipLaunchKernelGGL(k_kspace_mul_sym, dim3(grid_for(total, 256)), dim3(256), 0, ctx->stream, spec, ts,
+                       static_cast<int>(ctx->n1), static_cast<int>(n2), make_fastdiv(static_cast<uint32_t>(ctx->hx)),
+                       make_fastdiv(static_cast<uint32_t>(ctx->n1)), static_cast<int>(ctx->hxs),
+                       static_cast<int>(ctx->n1h), static_cast<int>(ctx->n2h), static_cast<int>(total));
+    MAFX_HIP(hipGetLastError());
+}
+
+void zfused_sym(mafx_ctx* ctx, float2* spec, const float* ts, int nz) {
+    ZTw<float2> tw{};
+    const int n2 = 2 * nz;
+    const long double two_pi = 6.283185307179586476925286766559L;
+    for (int t = 0; t < n2; ++t) {
+        const long double a = two_pi * t / n2;
+        tw.w[t] = float2{static_cast<float>(std::cos(a)), static_cast<float>(-std::sin(a))};
+    }
+    const int64_t threads = ctx->n1h * ctx->hx;
+    // spectrum columns read and written once, the tensor quarter read once
+    const double bytes = static_cast<double>(ctx->n1 * ctx->hx) * (2.0 * 3 * nz) * sizeof(float2) +
+                         static_cast<double>(threads) * 6.0 * (nz + 1) * sizeof(float);
+    ProfScope ps(ctx, "zfused_mul_f32", bytes);
+    const dim3 g(grid_for(threads, 256)), b(256);
+    const int n1 = static_cast<int>(ctx->n1), hxs = static_cast<int>(ctx->hxs), n1h = static_cast<int>(ctx->n1h);
+    const FastDiv hxd = make_fastdiv(static_cast<uint32_t>(ctx->hx));
+    switch (nz) {
+    case 2: hipLaunchKernelGGL((k_zfused_sym<2>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
+    case 3: hipLaunchKernelGGL((k_zfused_sym<3>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
+    case 4: hipLaunchKernelGGL((k_zfused_sym<4>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
+    case 5: hipLaunchKernelGGL((k_zfused_sym<5>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
+    case 6: hipLaunchKernelGGL((k_zfused_sym<6>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
+    case 7: hipLaunchKernelGGL((k_zfused_sym<7>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
+    case 8: hipLaunchKernelGGL((k_zfused_sym<8>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
+    default: throw Error(MAFX_ERR_ARG, "fused z stage supports 2 <= nz <= 8");
+    }
+    MAFX_HIP(hipGetLastError());
+}
+
+template void fft_x_forward<double2>(mafx_ctx*, const double*, const double*, double, int64_t, int64_t, int64_t, int,
+                                     int64_t, int64_t, double2*, int64_t);
+template void fft_x_forward<float2>(mafx_ctx*, const double*, const double*, double, int64_t, int64_t, int64_t, int,
+                                    int64_t, int64_t, float2*, int64_t);
 template void fft_strided<double2>(mafx_ctx*, const FftPlan&, double2*, int64_t, int, int64_t, int, int64_t, int,
                                    int, bool, const char*);
 template void fft_strided<float2>(mafx_ctx*, const FftPlan&, float2*, int64_t, int, int64_t, int, int64_t, int, int,
                                   bool, const char*);
+template void fft_y_split<double2>(mafx_ctx*, double2*, int64_t, int, int64_t, int, int64_t, bool);
+template void fft_y_split<float2>(mafx_ctx*, float2*, int64_t, int, int64_t, int, int64_t, bool);
 template void kspace_mul<double2>(mafx_ctx*, double2*, const double2*, int64_t);
-template void kspace_mul<float2>(mafx_ctx*, float2*, const float2*, int64_t);
 template void zfused<double2>(mafx_ctx*, double2*, const double2*, int);
-template void zfused<float2>(mafx_ctx*, float2*, const float2*, int);
 
 } // namespace mafx

@@ -13,6 +13,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "fastdiv.hpp"
+
 namespace mafx {
 
 template <class C> struct real_of;
@@ -29,6 +31,21 @@ template <class C> __device__ __forceinline__ C mul_mi(C a, bool inv) { return i
 template <class C> __device__ __forceinline__ C conj_if(C a, bool inv) { return inv ? C{a.x, -a.y} : a; }
 
 constexpr int kFftBlock = 256;
+
+// Block coordinates of a 3-D grid after xcd_swizzle of the x-fastest linear block id.
+__device__ __forceinline__ uint3 swizzled_block() {
+    const uint32_t gx = gridDim.x, gy = gridDim.y;
+    const uint32_t id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const uint32_t l = xcd_swizzle(id, gx * gy * gridDim.z);
+    return uint3{l % gx, (l / gx) % gy, l / (gx * gy)};
+}
+
+// LDS index padding: one spare element after every 32.  The autosort writes of a stage move
+// groups of W lanes at a stride of R W elements, a multiple of the bank count for the power-of-two
+// shapes here; the padding spreads consecutive groups over different banks.
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
+// LDS elements to allocate for n logical elements
+constexpr int lds_padded(int n) { return n + (n >> 5) + 1; }
 
 // R-point DFT in registers; forward uses w = exp(-2 pi i / R), inverse its conjugate.
 // Generic odd radices read w_R^p = tw[p L / R] from the length-L table.
@@ -81,20 +98,33 @@ __device__ __forceinline__ void dft(C (&v)[R], bool inv, const C* __restrict__ t
     }
 }
 
+// Radix schedule handed to kernels by value, with the per-stage constants precomputed on the
+// host: ns[s] = Ns (product of the radices before stage s) as a divisor, twstep[s] = L / (Ns R).
+struct FftDesc {
+    int L;
+    int nstages;
+    int radix[16];
+    int twstep[16];
+    FastDiv ns[16];
+};
+
 // One Stockham stage over W lines of length L in buf[pos * W + w].
-// MAXB = max butterflies per thread = ceil(W L / R / BLOCK).
+// MAXB = max butterflies per thread = ceil(W L / R / BLOCK).  Butterfly b covers line w = b mod W,
+// index j = b / W; its inputs sit at pos j + r L/R, i.e. at b + r (L/R) W, so the reads need no
+// division.
 template <class C, int R, int MAXB, int BLOCK>
-__device__ __forceinline__ void fft_stage(C* buf, int L, int W, int Ns, const C* __restrict__ tw, bool inv) {
+__device__ __forceinline__ void fft_stage(C* buf, int L, const FastDiv& W, const FastDiv& Ns, int twstep,
+                                          const C* __restrict__ tw, bool inv) {
     const int m = L / R;
-    const int nb = m * W;
+    const int nb = m * static_cast<int>(W.d);
+    const int rs = m * static_cast<int>(W.d);
     C v[MAXB][R];
 #pragma unroll
     for (int i = 0; i < MAXB; ++i) {
         const int b = threadIdx.x + i * BLOCK;
         if (b < nb) {
-            const int w = b % W, j = b / W;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * m) * W + w];
+            for (int r = 0; r < R; ++r) v[i][r] = buf[lds_pad(b + r * rs)];
         }
     }
     __syncthreads();
@@ -102,55 +132,51 @@ __device__ __forceinline__ void fft_stage(C* buf, int L, int W, int Ns, const C*
     for (int i = 0; i < MAXB; ++i) {
         const int b = threadIdx.x + i * BLOCK;
         if (b < nb) {
-            const int w = b % W, j = b / W;
-            const int k = j % Ns;
-            if (Ns > 1) {
-                const int step = k * (L / (Ns * R));
+            const int j = static_cast<int>(fdiv(b, W)), w = b - j * static_cast<int>(W.d);
+            const int q = static_cast<int>(fdiv(j, Ns));
+            const int ns = static_cast<int>(Ns.d);
+            const int k = j - q * ns;
+            if (ns > 1) {
+                const int step = k * twstep;
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], conj_if(tw[r * step], inv));
             }
             dft<C, R>(v[i], inv, tw, L);
-            const int base = (j / Ns) * Ns * R + k;
+            const int base = (q * ns * R + k) * static_cast<int>(W.d) + w;
+            const int ostride = ns * static_cast<int>(W.d);
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[(base + r * Ns) * W + w] = v[i][r];
+            for (int r = 0; r < R; ++r) buf[lds_pad(base + r * ostride)] = v[i][r];
         }
     }
     __syncthreads();
 }
 
-// Radix schedule handed to kernels by value.
-struct FftDesc {
-    int L;
-    int nstages;
-    int radix[16];
-};
-
 // Runs all stages.  LDS_ELEMS bounds W * L (the LDS budget in complex elements).  POW2 plans
 // (radices 2, 4, 8 only) get a stage switch without the odd radices, whose register arrays
 // would otherwise set the kernel's VGPR count.
 template <class C, int LDS_ELEMS, bool POW2, int B = kFftBlock>
-__device__ void fft_lds(C* buf, const FftDesc& d, int W, const C* __restrict__ tw, bool inv) {
-    int Ns = 1;
+__device__ void fft_lds(C* buf, const FftDesc& d, const FastDiv& W, const C* __restrict__ tw, bool inv) {
     for (int s = 0; s < d.nstages; ++s) {
         const int R = d.radix[s];
+        const FastDiv& Ns = d.ns[s];
+        const int ts = d.twstep[s];
         switch (R) {
-        case 2: fft_stage<C, 2, (LDS_ELEMS / 2 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
-        case 4: fft_stage<C, 4, (LDS_ELEMS / 4 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
-        case 8: fft_stage<C, 8, (LDS_ELEMS / 8 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
+        case 2: fft_stage<C, 2, (LDS_ELEMS / 2 + B - 1) / B, B>(buf, d.L, W, Ns, ts, tw, inv); break;
+        case 4: fft_stage<C, 4, (LDS_ELEMS / 4 + B - 1) / B, B>(buf, d.L, W, Ns, ts, tw, inv); break;
+        case 8: fft_stage<C, 8, (LDS_ELEMS / 8 + B - 1) / B, B>(buf, d.L, W, Ns, ts, tw, inv); break;
         default:
             if constexpr (!POW2) {
                 switch (R) {
-                case 3: fft_stage<C, 3, (LDS_ELEMS / 3 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
-                case 5: fft_stage<C, 5, (LDS_ELEMS / 5 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
-                case 7: fft_stage<C, 7, (LDS_ELEMS / 7 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
-                case 11: fft_stage<C, 11, (LDS_ELEMS / 11 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
-                case 13: fft_stage<C, 13, (LDS_ELEMS / 13 + B - 1) / B, B>(buf, d.L, W, Ns, tw, inv); break;
+                case 3: fft_stage<C, 3, (LDS_ELEMS / 3 + B - 1) / B, B>(buf, d.L, W, Ns, ts, tw, inv); break;
+                case 5: fft_stage<C, 5, (LDS_ELEMS / 5 + B - 1) / B, B>(buf, d.L, W, Ns, ts, tw, inv); break;
+                case 7: fft_stage<C, 7, (LDS_ELEMS / 7 + B - 1) / B, B>(buf, d.L, W, Ns, ts, tw, inv); break;
+                case 11: fft_stage<C, 11, (LDS_ELEMS / 11 + B - 1) / B, B>(buf, d.L, W, Ns, ts, tw, inv); break;
+                case 13: fft_stage<C, 13, (LDS_ELEMS / 13 + B - 1) / B, B>(buf, d.L, W, Ns, ts, tw, inv); break;
                 default: break; // unreachable: FftPlan::build rejects other factors
                 }
             }
             break;
         }
-        Ns *= R;
     }
 }
 

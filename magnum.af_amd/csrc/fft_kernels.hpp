@@ -6,16 +6,39 @@
 namespace mafx {
 
 FftDesc desc_of(const FftPlan& p);
-// rows (c, z, y) of the real input -> x half spectrum at out[((c n2o + z) n1 + y) hx + k]
+
+// twiddle table of a plan in the precision of C
+template <class C> inline const C* tw_of(const FftPlan& p) {
+    if constexpr (sizeof(C) == sizeof(double2))
+        return reinterpret_cast<const C*>(p.tw64.p);
+    else
+        return reinterpret_cast<const C*>(p.tw32.p);
+}
+
+// complex elements of a 64 KiB LDS budget
+template <class C> constexpr int lds_elems() { return 65536 / static_cast<int>(sizeof(C)); }
+
+// rows (c, z, y) of the real input (row length Xi <= n0) -> x half spectrum at
+// out[((c n2o + z) n1 + y) hxs + k], k <= n0/2, through one FFT of length n0/2 per row
 template <class C>
-void fft_x_forward(mafx_ctx* ctx, const FftPlan& p, const double* in, const double* ms, double scale, int64_t Xi,
-                   int64_t Yi, int64_t Zi, int ncomp, int64_t Yr, int64_t Zr, C* out, int64_t n2o);
+void fft_x_forward(mafx_ctx* ctx, const double* in, const double* ms, double scale, int64_t Xi, int64_t Yi,
+                   int64_t Zi, int ncomp, int64_t Yr, int64_t Zr, C* out, int64_t n2o);
+// general strided complex transform (full length, optional zero padding / cropping)
 template <class C>
 void fft_strided(mafx_ctx* ctx, const FftPlan& p, C* data, int64_t sa, int na, int64_t sb, int nb, int64_t stride,
                  int n_in, int n_out, bool inv, const char* name);
+// y transform of the zero-padded spectrum (inputs / outputs y < ny) as two length-ny transforms
+template <class C>
+void fft_y_split(mafx_ctx* ctx, C* data, int64_t sa, int na, int64_t sb, int nb, int64_t stride, bool inv);
 template <class C> void kspace_mul(mafx_ctx* ctx, C* spec, const C* tensor, int64_t K);
 // z DFT (length 2 nz) + tensor product + inverse z DFT, 2 <= nz <= 8, spectrum planes [3][nz]
 template <class C> void zfused(mafx_ctx* ctx, C* spec, const C* tensor, int nz);
-void convert_tensor_f32(mafx_ctx* ctx, const double2* a, float2* b, int64_t n);
+// FP32 path, real mirror-symmetric tensor spectrum (fft.hip): zero the unpaired planes of the padded
+// real-space tensor [ncomp][n2][n1][n0]; keep the real quarter Ts[6][n2h][n1h][hxs] of its spectrum;
+// apply it to a spectrum of n2 z-planes; fused z stage for 2 <= nz <= 8
+void zero_unpaired_planes(mafx_ctx* ctx, double* N, int ncomp);
+void tensor_sym_f32(mafx_ctx* ctx, const double2* T, float* Ts);
+void kspace_mul_sym(mafx_ctx* ctx, float2* spec, const float* ts, int64_t n2);
+void zfused_sym(mafx_ctx* ctx, float2* spec, const float* ts, int nz);
 
 } // namespace mafx

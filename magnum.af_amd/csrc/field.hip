@@ -14,6 +14,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "fft_device.hpp"
 #include "fft_kernels.hpp"
@@ -359,51 +360,99 @@ __device__ void finish_block(const EpiParams& P, double* acc) {
     block_reduce_store(acc, P.nterms + 2, is_max, P.partials, P.nblocks);
 }
 
-// One work-group per (y, z) row: inverse x transform of the three components (C2R with the
-// Hermitian extension X[L-k] = conj X[k]), crop to x < nx, scale, then the per-cell epilogue.
-template <class C, class T, bool POW2>
+// Packs row k of the half spectrum X[0..M] for the half-length inverse transform of a real
+// sequence of length 2M: Zp[k] = (X[k] + conj X[M-k]) + i w_{2M}^-k (X[k] - conj X[M-k]); the
+// unnormalised inverse FFT_M of Zp then holds x[2n] + i x[2n+1].
+template <class C>
+__device__ __forceinline__ C c2r_pack(const C* __restrict__ row, int k, int M, const C* __restrict__ twL) {
+    const C Xk = row[k], Xm = row[M - k];
+    const C S{Xk.x + Xm.x, Xk.y - Xm.y};
+    const C D{Xk.x - Xm.x, Xk.y + Xm.y};
+    const C wD = cmul(conj_if(twL[k], true), D);
+    return C{S.x - wD.y, S.y + wD.x};
+}
+
+// Inverse x transform + local terms.  A work-group owns R consecutive rows of the z-fastest row
+// order (row = y nz + z) after the XCD renumbering, so the y and z stencil neighbours of a row are
+// read by work-groups behind the same L2.  The three
+// components of each row are W = 3 R lines of one LDS transform of length M = nx (C2R through the
+// half-length packing), so a row costs one barrier sequence instead of three.  After the
+// transform the real field value of cell x of line w sits in buf[(x / 2) W + w] (.x for even x,
+// .y for odd x).  SEQ handles rows too wide for 3 lines in LDS: one component at a time, the
+// cropped real values parked in a second LDS array.
+template <class C, class T, bool POW2, bool SEQ>
 __global__ void __launch_bounds__(kFftBlock)
-    k_epi_rows(EpiParams P, FftDesc d, const C* __restrict__ tw, const C* __restrict__ spec, int64_t hx, int64_t n1,
-               int64_t n2, double hscale) {
+    k_epi_rows(EpiParams P, FftDesc dh, const C* __restrict__ twh, const C* __restrict__ twL,
+               const C* __restrict__ spec, int64_t hxs, int64_t n1, int64_t n2s, double hscale, int R, FastDiv Md,
+               FastDiv Wd, FastDiv nxd, FastDiv nzd) {
     extern __shared__ unsigned char smem_raw[];
-    C* line = reinterpret_cast<C*>(smem_raw);
-    T* res = reinterpret_cast<T*>(smem_raw + sizeof(C) * d.L);
-    const int L = d.L;
-    const int64_t y = blockIdx.x % P.ny, z = blockIdx.x / P.ny;
-    for (int c = 0; c < 3; ++c) {
-        const C* row = spec + ((c * n2 + z) * n1 + y) * hx;
-        for (int k = threadIdx.x; k < L; k += kFftBlock) {
-            C v;
-            if (k < hx) {
-                v = row[k];
-            } else {
-                const C w = row[L - k];
-                v = C{w.x, -w.y};
-            }
-            line[k] = v;
-        }
-        __syncthreads();
-        fft_lds<C, 65536 / static_cast<int>(sizeof(C)), POW2>(line, d, 1, tw, true);
-        for (int x = threadIdx.x; x < P.nx; x += kFftBlock)
-            res[c * P.nx + x] = static_cast<T>(static_cast<double>(line[x].x) * hscale);
-        __syncthreads();
-    }
+    C* buf = reinterpret_cast<C*>(smem_raw);
+    const int M = dh.L;
+    const int nx = static_cast<int>(P.nx), ny = static_cast<int>(P.ny);
+    const int nrows = ny * static_cast<int>(P.nz);
+    const int nz = static_cast<int>(P.nz);
+    const int row0 = static_cast<int>(xcd_swizzle(blockIdx.x, gridDim.x)) * R;
     double acc[kMaxTerms + 2];
     for (int s = 0; s < kMaxTerms + 2; ++s) acc[s] = 0;
-    for (int64_t x = threadIdx.x; x < P.nx; x += kFftBlock) {
-        const T hd[3] = {res[x], res[P.nx + x], res[2 * P.nx + x]};
-        cell_epilogue<T>(P, x, y, z, hd, acc);
+    if constexpr (!SEQ) {
+        const int W = static_cast<int>(Wd.d); // 3 R lines
+        for (int e = threadIdx.x; e < M * W; e += kFftBlock) {
+            const int w = static_cast<int>(fdiv(e, Md)), k = e - w * M;
+            const int r = w / 3, c = w - 3 * r;
+            const int row = row0 + r;
+            C v{0, 0};
+            if (row < nrows) {
+                const int y = static_cast<int>(fdiv(row, nzd)), z = row - y * nz;
+                v = c2r_pack(spec + ((c * n2s + z) * n1 + y) * hxs, k, M, twL);
+            }
+            buf[lds_pad(k * W + w)] = v;
+        }
+        __syncthreads();
+        fft_lds<C, 65536 / static_cast<int>(sizeof(C)), POW2>(buf, dh, Wd, twh, true);
+        for (int e = threadIdx.x; e < R * nx; e += kFftBlock) {
+            const int r = static_cast<int>(fdiv(e, nxd)), x = e - r * nx;
+            const int row = row0 + r;
+            if (row >= nrows) continue;
+            const int y = static_cast<int>(fdiv(row, nzd)), z = row - y * nz;
+            T hd[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const C v = buf[lds_pad((x >> 1) * W + 3 * r + c)];
+                hd[c] = static_cast<T>(static_cast<double>((x & 1) ? v.y : v.x) * hscale);
+            }
+            cell_epilogue<T>(P, x, y, z, hd, acc);
+        }
+    } else {
+        T* res = reinterpret_cast<T*>(smem_raw + sizeof(C) * lds_padded(M));
+        const int row = row0;
+        const int y = static_cast<int>(fdiv(row, nzd)), z = row - y * nz;
+        for (int c = 0; c < 3; ++c) {
+            for (int k = threadIdx.x; k < M; k += kFftBlock)
+                buf[lds_pad(k)] = c2r_pack(spec + ((c * n2s + z) * n1 + y) * hxs, k, M, twL);
+            __syncthreads();
+            fft_lds<C, 65536 / static_cast<int>(sizeof(C)), POW2>(buf, dh, Wd, twh, true);
+            for (int64_t x = threadIdx.x; x < P.nx; x += kFftBlock) {
+                const C v = buf[lds_pad(x >> 1)];
+                res[c * P.nx + x] = static_cast<T>(static_cast<double>((x & 1) ? v.y : v.x) * hscale);
+            }
+            __syncthreads();
+        }
+        for (int64_t x = threadIdx.x; x < P.nx; x += kFftBlock) {
+            const T hd[3] = {res[x], res[P.nx + x], res[2 * P.nx + x]};
+            cell_epilogue<T>(P, x, y, z, hd, acc);
+        }
     }
     finish_block(P, acc);
 }
 
 // Local terms only (no FFT term): one thread per cell.
-template <class T> __global__ void __launch_bounds__(kBlock) k_epi_cells(EpiParams P) {
+template <class T> __global__ void __launch_bounds__(kBlock) k_epi_cells(EpiParams P, FastDiv nxd, FastDiv nyd) {
     double acc[kMaxTerms + 2];
     for (int s = 0; s < kMaxTerms + 2; ++s) acc[s] = 0;
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
     if (i < P.N) {
-        const int64_t x = i % P.nx, yz = i / P.nx, y = yz % P.ny, z = yz / P.ny;
+        const int yz = static_cast<int>(fdiv(i, nxd)), x = i - yz * static_cast<int>(P.nx);
+        const int z = static_cast<int>(fdiv(yz, nyd)), y = yz - z * static_cast<int>(P.ny);
         const T hd[3] = {0, 0, 0};
         cell_epilogue<T>(P, x, y, z, hd, acc);
     }
@@ -497,8 +546,14 @@ EpiParams make_params(mafx_ctx* ctx, const EvalArgs& a, int64_t nblocks) {
 template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a) {
     const mafx_mesh& g = ctx->mesh;
     const bool fft = ctx->demag_term >= 0 && (a.mask & (1u << ctx->demag_term));
+    // cell and row indices are 32-bit inside the kernels
+    require(ctx->N < (int64_t{1} << 31), "meshes of 2^31 cells or more are not supported");
     const int64_t nrows = g.ny * g.nz;
-    const int64_t nblocks = fft ? nrows : (ctx->N + kBlock - 1) / kBlock;
+    // epilogue rows per work-group: 3 R lines of length nx must fit the 64 KiB LDS transform;
+    // R == 0 selects the one-component-at-a-time variant
+    constexpr int kElems = 65536 / static_cast<int>(sizeof(C));
+    const int R = fft ? static_cast<int>(std::min<int64_t>(4, kElems / (3 * g.nx))) : 0;
+    const int64_t nblocks = fft ? (R > 0 ? (nrows + R - 1) / R : nrows) : (ctx->N + kBlock - 1) / kBlock;
     require((ctx->partials.bytes / sizeof(double)) >= static_cast<size_t>((kMaxTerms + 2) * nblocks),
             "partials buffer too small");
     EpiParams P = make_params(ctx, a, nblocks);
@@ -506,56 +561,75 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
                                : (a.mode == OutMode::ENERGY) ? 0.0 : 24.0;
     if (fft) {
         C* spec = spec_of<C>(ctx);
-        const int64_t hx = ctx->hx, n1 = ctx->n1, n2 = ctx->n2, nzs = ctx->nzs;
-        const int64_t sa = nzs * n1 * hx;
+        const int64_t hx = ctx->hx, hxs = ctx->hxs, n1 = ctx->n1, n2 = ctx->n2, nzs = ctx->nzs;
+        const int64_t sa = nzs * n1 * hxs;
         // Ms * m (micro; Ms_field per cell) or m (atomistic dipole) -> x half spectrum
         const double* ms = (!ctx->atom_dipole && ctx->has_ms_field) ? ctx->ms_field.as<double>() : nullptr;
         const double scale = ctx->atom_dipole ? 1.0 : (ctx->has_ms_field ? 1.0 : ctx->Ms);
-        fft_x_forward<C>(ctx, ctx->px, a.m, ms, scale, g.nx, g.ny, g.nz, 3, g.ny, g.nz, spec, nzs);
-        fft_strided<C>(ctx, ctx->py, spec, sa, 3, n1 * hx, static_cast<int>(g.nz), hx, static_cast<int>(g.ny),
-                       static_cast<int>(n1), false, sizeof(C) == 8 ? "fft_y_fwd_f32" : "fft_y_fwd_f64");
+        fft_x_forward<C>(ctx, a.m, ms, scale, g.nx, g.ny, g.nz, 3, g.ny, g.nz, spec, nzs);
+        fft_y_split<C>(ctx, spec, sa, 3, n1 * hxs, static_cast<int>(g.nz), hxs, false);
+        // tensor product: complex reference spectrum (FP64) or its real symmetric quarter (FP32)
+        auto kmul = [&](int64_t planes) {
+            if constexpr (std::is_same<C, float2>::value)
+                kspace_mul_sym(ctx, spec, ctx->tensor.as<float>(), planes);
+            else
+                kspace_mul<C>(ctx, spec, ctx->tensor.as<C>(), sa);
+        };
         if (n2 == 1) {
-            kspace_mul<C>(ctx, spec, ctx->tensor.as<C>(), sa);
+            kmul(1);
         } else if (nzs == g.nz) {
-            zfused<C>(ctx, spec, ctx->tensor.as<C>(), static_cast<int>(g.nz));
+            if constexpr (std::is_same<C, float2>::value)
+                zfused_sym(ctx, spec, ctx->tensor.as<float>(), static_cast<int>(g.nz));
+            else
+                zfused<C>(ctx, spec, ctx->tensor.as<C>(), static_cast<int>(g.nz));
         } else {
-            fft_strided<C>(ctx, ctx->pz, spec, sa, 3, hx, static_cast<int>(n1), n1 * hx, static_cast<int>(g.nz),
+            fft_strided<C>(ctx, ctx->pz, spec, sa, 3, hxs, static_cast<int>(n1), n1 * hxs, static_cast<int>(g.nz),
                            static_cast<int>(n2), false, sizeof(C) == 8 ? "fft_z_fwd_f32" : "fft_z_fwd_f64");
-            kspace_mul<C>(ctx, spec, ctx->tensor.as<C>(), sa);
-            fft_strided<C>(ctx, ctx->pz, spec, sa, 3, hx, static_cast<int>(n1), n1 * hx, static_cast<int>(n2),
+            kmul(n2);
+            fft_strided<C>(ctx, ctx->pz, spec, sa, 3, hxs, static_cast<int>(n1), n1 * hxs, static_cast<int>(n2),
                            static_cast<int>(g.nz), true, sizeof(C) == 8 ? "fft_z_inv_f32" : "fft_z_inv_f64");
         }
-        fft_strided<C>(ctx, ctx->py, spec, sa, 3, n1 * hx, static_cast<int>(g.nz), hx, static_cast<int>(n1),
-                       static_cast<int>(g.ny), true, sizeof(C) == 8 ? "fft_y_inv_f32" : "fft_y_inv_f64");
+        fft_y_split<C>(ctx, spec, sa, 3, n1 * hxs, static_cast<int>(g.nz), hxs, true);
         const double P_total = static_cast<double>(ctx->n0) * n1 * n2;
         const double hscale = (ctx->atom_dipole ? ctx->Ms : 1.0) / P_total;
-        const size_t lds = sizeof(C) * ctx->n0 + sizeof(T) * 3 * g.nx;
+        const int M = ctx->pxh.L;
+        const bool seq = R == 0;
+        const size_t lds = seq ? sizeof(C) * lds_padded(M) + sizeof(T) * 3 * g.nx : sizeof(C) * lds_padded(M * 3 * R);
+        require(M <= kElems, "x transform length exceeds the LDS FFT budget");
         require(lds <= 160 * 1024, "epilogue LDS exceeds 160 KiB (mesh too wide in x)");
-        require(ctx->n0 <= 65536 / static_cast<int64_t>(sizeof(C)), "x transform length exceeds the LDS FFT budget");
         // algorithmic bytes: the three cropped spectrum rows, m once, the output once
         const double bytes = static_cast<double>(ctx->N) * (3.0 * sizeof(C) * hx / g.nx + 24.0 + field_bytes);
         ProfScope ps(ctx, sizeof(C) == 8 ? "epilogue_rows_f32" : "epilogue_rows_f64", bytes);
-        const bool pow2 = is_pow2_plan(desc_of(ctx->px));
-        MAFX_HIP(hipFuncSetAttribute(pow2 ? reinterpret_cast<const void*>(&k_epi_rows<C, T, true>)
-                                          : reinterpret_cast<const void*>(&k_epi_rows<C, T, false>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        FftDesc d{};
-        d.L = ctx->px.L;
-        d.nstages = ctx->px.nstages;
-        for (int s = 0; s < d.nstages; ++s) d.radix[s] = ctx->px.radix[s];
-        const C* tw = sizeof(C) == 8 ? reinterpret_cast<const C*>(ctx->px.tw32.p)
-                                     : reinterpret_cast<const C*>(ctx->px.tw64.p);
-        if (pow2)
-            hipLaunchKernelGGL((k_epi_rows<C, T, true>), dim3(static_cast<unsigned>(nrows)), dim3(kFftBlock), lds,
-                               ctx->stream, P, d, tw, spec, hx, n1, nzs, hscale);
+        const FftDesc dh = desc_of(ctx->pxh);
+        const bool pow2 = is_pow2_plan(dh);
+        const void* fn = pow2 ? (seq ? reinterpret_cast<const void*>(&k_epi_rows<C, T, true, true>)
+                                     : reinterpret_cast<const void*>(&k_epi_rows<C, T, true, false>))
+                              : (seq ? reinterpret_cast<const void*>(&k_epi_rows<C, T, false, true>)
+                                     : reinterpret_cast<const void*>(&k_epi_rows<C, T, false, false>));
+        MAFX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        const C* twh = tw_of<C>(ctx->pxh);
+        const C* twL = tw_of<C>(ctx->px);
+        const dim3 grid(static_cast<unsigned>(nblocks)), blk(kFftBlock);
+        const FastDiv Md = make_fastdiv(M), Wd = make_fastdiv(seq ? 1 : 3 * R);
+        const FastDiv nxd = make_fastdiv(static_cast<uint32_t>(g.nx)), nzd = make_fastdiv(static_cast<uint32_t>(g.nz));
+        if (pow2 && !seq)
+            hipLaunchKernelGGL((k_epi_rows<C, T, true, false>), grid, blk, lds, ctx->stream, P, dh, twh, twL, spec, hxs,
+                               n1, nzs, hscale, R, Md, Wd, nxd, nzd);
+        else if (pow2)
+            hipLaunchKernelGGL((k_epi_rows<C, T, true, true>), grid, blk, lds, ctx->stream, P, dh, twh, twL, spec, hxs,
+                               n1, nzs, hscale, 1, Md, Wd, nxd, nzd);
+        else if (!seq)
+            hipLaunchKernelGGL((k_epi_rows<C, T, false, false>), grid, blk, lds, ctx->stream, P, dh, twh, twL, spec,
+                               hxs, n1, nzs, hscale, R, Md, Wd, nxd, nzd);
         else
-            hipLaunchKernelGGL((k_epi_rows<C, T, false>), dim3(static_cast<unsigned>(nrows)), dim3(kFftBlock), lds,
-                               ctx->stream, P, d, tw, spec, hx, n1, nzs, hscale);
+            hipLaunchKernelGGL((k_epi_rows<C, T, false, true>), grid, blk, lds, ctx->stream, P, dh, twh, twL, spec,
+                               hxs, n1, nzs, hscale, 1, Md, Wd, nxd, nzd);
         MAFX_HIP(hipGetLastError());
     } else {
         ProfScope ps(ctx, sizeof(T) == 4 ? "local_cells_f32" : "local_cells_f64",
                      static_cast<double>(ctx->N) * (24.0 + field_bytes));
-        hipLaunchKernelGGL(k_epi_cells<T>, dim3(static_cast<unsigned>(nblocks)), dim3(kBlock), 0, ctx->stream, P);
+        hipLaunchKernelGGL(k_epi_cells<T>, dim3(static_cast<unsigned>(nblocks)), dim3(kBlock), 0, ctx->stream, P,
+                           make_fastdiv(static_cast<uint32_t>(g.nx)), make_fastdiv(static_cast<uint32_t>(g.ny)));
         MAFX_HIP(hipGetLastError());
     }
     ctx->scal_nblocks = nblocks;
@@ -636,20 +710,24 @@ std::shared_ptr<std::vector<double>> host_tensor(const mafx_mesh& g, bool dipole
 // Demag / dipole tensor: real-space tensor on the host in the reference's arithmetic, then a
 // full FP64 R2C transform on the GPU with the same kernels, stored as the complex half spectrum.
 void build_tensor(mafx_ctx* ctx) {
-    const int64_t n0 = ctx->n0, n1 = ctx->n1, n2 = ctx->n2, hx = ctx->hx;
-    const int64_t P = n0 * n1 * n2, K = n2 * n1 * hx;
+    const int64_t n0 = ctx->n0, n1 = ctx->n1, n2 = ctx->n2, hxs = ctx->hxs;
+    const int64_t P = n0 * n1 * n2, K = n2 * n1 * hxs;
     auto Nh = host_tensor(ctx->mesh, ctx->atom_dipole, P);
     DevBuf Nd, T64;
     Nd.alloc(sizeof(double) * 6 * P);
     MAFX_HIP(hipMemcpy(Nd.p, Nh->data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
     Nh.reset();
+    ctx->n1h = n1 / 2 + 1;
+    ctx->n2h = n2 / 2 + 1;
+    if (ctx->precision != MAFX_F64) zero_unpaired_planes(ctx, Nd.as<double>(), 6);
     T64.alloc(sizeof(double2) * 6 * K);
+    MAFX_HIP(hipMemset(T64.p, 0, T64.bytes)); // padding columns stay defined
     double2* t = T64.as<double2>();
-    fft_x_forward<double2>(ctx, ctx->px, Nd.as<double>(), nullptr, 1.0, n0, n1, n2, 6, n1, n2, t, n2);
-    fft_strided<double2>(ctx, ctx->py, t, K, 6, n1 * hx, static_cast<int>(n2), hx, static_cast<int>(n1),
+    fft_x_forward<double2>(ctx, Nd.as<double>(), nullptr, 1.0, n0, n1, n2, 6, n1, n2, t, n2);
+    fft_strided<double2>(ctx, ctx->py, t, K, 6, n1 * hxs, static_cast<int>(n2), hxs, static_cast<int>(n1),
                          static_cast<int>(n1), false, "tensor_fft_y");
     if (n2 > 1)
-        fft_strided<double2>(ctx, ctx->pz, t, K, 6, hx, static_cast<int>(n1), n1 * hx, static_cast<int>(n2),
+        fft_strided<double2>(ctx, ctx->pz, t, K, 6, hxs, static_cast<int>(n1), n1 * hxs, static_cast<int>(n2),
                              static_cast<int>(n2), false, "tensor_fft_z");
     MAFX_HIP(hipStreamSynchronize(ctx->stream));
     Nd.release();
@@ -657,8 +735,8 @@ void build_tensor(mafx_ctx* ctx) {
         std::swap(ctx->tensor.p, T64.p);
         std::swap(ctx->tensor.bytes, T64.bytes);
     } else {
-        ctx->tensor.alloc(sizeof(float2) * 6 * K);
-        convert_tensor_f32(ctx, t, ctx->tensor.as<float2>(), 6 * K);
+        ctx->tensor.alloc(sizeof(float) * 6 * ctx->n2h * ctx->n1h * hxs);
+        tensor_sym_f32(ctx, t, ctx->tensor.as<float>());
         MAFX_HIP(hipStreamSynchronize(ctx->stream));
     }
 }

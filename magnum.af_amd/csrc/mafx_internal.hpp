@@ -126,9 +126,14 @@ struct mafx_ctx {
     // FFT geometry: expanded lengths (demag_field.cpp:14-16) and half-spectrum width
     int64_t n0{0}, n1{0}, n2{0}, hx{0};
     int64_t nzs{0};           // z planes of the spectrum workspace: nz when the z stage is fused, else n2
-    mafx::FftPlan px, py, pz;
-    mafx::DevBuf tensor;      // [6][n2][n1][hx] complex, precision of the field path
-    mafx::DevBuf spec;        // [3][n2][n1][hx] complex workspace
+    int64_t hxs{0};           // padded row stride of spectra (>= hx, multiple of 16)
+    int64_t n1h{0}, n2h{0};   // n1/2 + 1, n2/2 + 1: ky, kz extent of the symmetric FP32 tensor
+    mafx::FftPlan px, py, pz; // lengths n0, n1, n2 (px, py also supply the packing / split twiddles)
+    mafx::FftPlan pxh, pyh;   // lengths nx, ny: the half-length x and split y transforms
+    // FP64 path: [6][n2][n1][hxs] complex f64, the transformed reference tensor as it is.
+    // FP32 path: [6][n2h][n1h][hxs] real f32, the mirror-symmetric spectrum (fft.hip).
+    mafx::DevBuf tensor;
+    mafx::DevBuf spec;        // [3][nzs][n1][hxs] complex workspace
     mafx::DevBuf hbuf;        // 3N f64 scratch (field output for helpers)
     mafx::DevBuf partials;    // reduction partials (f64)
     mafx::DevBuf scal;        // small device scalars

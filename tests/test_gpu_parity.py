@@ -162,15 +162,57 @@ def test_demag_cube_energy_gpu():
     assert abs(E - analytic) <= (E + analytic) / 2 * 5.3e-8
 
 
-@pytest.mark.parametrize("mesh", [(16, 8, 4, 2e-9, 2e-9, 2e-9), (64, 32, 1, 3e-9, 3e-9, 3e-9)])
+# FP32 field path: nz = 1 (k-space product), 2 <= nz <= 8 (fused z stage), nz > 8 (strided z
+# transforms), odd and prime sizes, one cell.  Tolerance: FP32 rounding of the FFTs and stencils.
+FP32_TOL = 1e-5
+
+
+@pytest.mark.parametrize("mesh", [(16, 8, 4, 2e-9, 2e-9, 2e-9), (64, 32, 1, 3e-9, 3e-9, 3e-9),
+                                  (5, 4, 3, 1e-9, 1.2e-9, 0.8e-9), (7, 3, 1, 2e-9, 1e-9, 3e-9),
+                                  (33, 7, 5, 1e-9, 2e-9, 1.5e-9), (12, 6, 10, 2e-9, 2e-9, 2e-9),
+                                  (20, 9, 8, 1e-9, 1e-9, 1e-9), (1, 1, 1, 2e-9, 2e-9, 2e-9)])
 def test_heff_fp32_field_error(mesh):
+    print(f"\n[heff fp32] mesh={mesh}", flush=True)
     m = rand_m(mesh[:3], 5)
     state, terms, sim = build(mesh, 1.1e6, m, MICRO_ALL)
     M = mf()
     h32 = M.LLGIntegrator(0.1, terms, precision="f32").H_in_Apm(state)
     err = normwise(h32, sim.heff())
-    print(f"\n[fp32 field] mesh={mesh} normwise error={err:.3e}", flush=True)
-    assert err <= 1e-5
+    print(f"[fp32 field] mesh={mesh} normwise error={err:.3e}", flush=True)
+    assert err <= FP32_TOL
+    # the demag term alone, where the symmetric tensor spectrum acts
+    hd = M.LLGIntegrator(0.1, [terms[0]], precision="f32").H_in_Apm(state)
+    err_d = normwise(hd, sim.heff(mask=1))
+    assert err_d <= FP32_TOL, err_d
+
+
+def test_fp32_ms_field_airbox():
+    x = y = z = 1e-9
+    nx, ny, nz = 6, 5, 4
+    m = np.zeros((2 * nx, 2 * ny, 2 * nz, 3))
+    m[nx // 2:2 * nx - nx // 2, ny // 2:2 * ny - ny // 2, nz // 2:2 * nz - nz // 2, 0] = 1
+    mesh = (2 * nx, 2 * ny, 2 * nz, x / nx, y / ny, z / nz)
+    print(f"\n[fp32 Ms_field] mesh={mesh}", flush=True)
+    M = mf()
+    g = M.Mesh(*mesh)
+    state = M.State(g, 1e5, m)
+    sim = O.Sim(mesh, 1e5, m, [O.term(O.DEMAG), O.term(O.EXCHANGE, 1e-11)])
+    llg = M.LLGIntegrator(0.1, [M.DemagField(g), M.ExchangeField(1e-11)], precision="f32")
+    assert normwise(llg.H_in_Apm(state), sim.heff()) <= FP32_TOL
+
+
+@pytest.mark.parametrize("mesh", [(16, 12, 1, 2.7e-10, 2.7e-10, 2.7e-10), (8, 6, 3, 2.7e-10, 2.7e-10, 2.7e-10)])
+def test_fp32_atomistic_dipole(mesh):
+    print(f"\n[fp32 atomistic] mesh={mesh}", flush=True)
+    J = 1.6e-21
+    specs = [("atom_dipole", {}), ("atom_exchange", {"J": J})]
+    m = rand_m(mesh[:3], 17)
+    state, terms, sim = build(mesh, 2.6 * 9.274009994e-24, m, specs)
+    M = mf()
+    llg = M.LLGIntegrator(0.1, terms, precision="f32")
+    assert normwise(llg.H_in_Apm(state), sim.heff()) <= FP32_TOL
+    hd = M.LLGIntegrator(0.1, [terms[0]], precision="f32").H_in_Apm(state)
+    assert normwise(hd, sim.heff(mask=1)) <= FP32_TOL
 
 
 def test_rkky_energy_sweep_gpu():
