@@ -198,7 +198,9 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
             }
             ctx->terms.push_back(std::move(t));
         }
-        int64_t nblk = std::max<int64_t>(mesh->ny * mesh->nz, (N + 255) / 256);
+        // partial-sum blocks: epilogue row groups, per-cell tiles (row tiles of 256 cells or
+        // memory order), vector reductions
+        int64_t nblk = std::max<int64_t>(mesh->ny * mesh->nz * ((mesh->nx + 255) / 256), (N + 255) / 256);
         nblk = std::max<int64_t>(nblk, reduction_partials_needed(3 * N));
         ctx->partials.alloc(sizeof(double) * 10 * nblk);
         ctx->scal.alloc(sizeof(double) * 64);
@@ -220,10 +222,12 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
             const size_t csz = precision == MAFX_F64 ? sizeof(double2) : sizeof(float2);
             ctx->spec.alloc(csz * 3 * ctx->nzs * ctx->n1 * ctx->hxs);
             MAFX_HIP(hipMemset(ctx->spec.p, 0, ctx->spec.bytes));
+            ctx->hdem.alloc((precision == MAFX_F64 ? sizeof(double) : sizeof(float)) * 3 * N);
             build_tensor(ctx.get());
         }
         ctx->device_bytes = static_cast<int64_t>(ctx->ms_field.bytes + ctx->partials.bytes + ctx->scal.bytes +
-                                                 ctx->hbuf.bytes + ctx->spec.bytes + ctx->tensor.bytes);
+                                                 ctx->hbuf.bytes + ctx->spec.bytes + ctx->tensor.bytes +
+                                                 ctx->hdem.bytes);
         for (auto& t : ctx->terms)
             ctx->device_bytes += static_cast<int64_t>(t.zee.bytes + t.rkky.bytes + t.aex.bytes + t.ridx.bytes);
         *out = ctx.release();

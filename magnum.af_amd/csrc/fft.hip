@@ -84,6 +84,9 @@ namespace {
 // 512-thread work-groups for the axis transforms: W L <= 64 KiB of complex data spread over twice
 // the lanes halves the butterflies each lane keeps in registers across a stage.
 constexpr int kAxisBlock = 512;
+// Minimum waves per SIMD requested for the power-of-two axis kernels: 4 caps them at 128 VGPRs, so
+// two 512-lane work-groups (the 64 KiB LDS budget allows two) are resident per CU.
+constexpr int kMinWaves = 4;
 
 // Forward real-to-complex transform along x of rows (c, z, y): padded length L = 2 M, input
 // x[t] = in[((c Zi + z) Yi + y) Xi + t] * scale * (ms ? ms[cell] : 1) for t < Xi, 0 beyond.
@@ -92,7 +95,7 @@ constexpr int kAxisBlock = 512;
 // Lanes walk along rows (consecutive lanes, consecutive addresses) for the global accesses.  The
 // (c, z, y) decomposition of each of the block's W <= 64 rows is done once, into LDS.
 template <class C, bool POW2>
-__global__ void __launch_bounds__(kAxisBlock)
+__global__ void __launch_bounds__(kAxisBlock, POW2 ? kMinWaves : 1)
     k_fft_x_fwd(const double* __restrict__ in, const double* __restrict__ ms, double scale, int Xi, int64_t Yi,
                 int64_t Zi, FastDiv Yr, FastDiv Zr, int rows, FastDiv Wd, FftDesc dh, FastDiv Md, FastDiv M1d,
                 const C* __restrict__ twh, const C* __restrict__ twL, C* __restrict__ out, int64_t hxs, int64_t n1,
@@ -152,11 +155,58 @@ __global__ void __launch_bounds__(kAxisBlock)
     }
 }
 
+// Inverse real transform along x of spectrum rows (c, z, y), c < 3, z < Zr, y < Yr, read at
+// spec[((c n2s + z) n1 + y) hxs + k]: C2R through the half-length packing (c2r_pack), an inverse
+// FFT of length M = n0 / 2, the crop to x < Xo and the 1 / P scale,
+// out[c No + (z Yr + y) Xo + x] = T(hscale * x_real[x]).
+template <class C, class T, bool POW2>
+__global__ void __launch_bounds__(kAxisBlock, POW2 ? kMinWaves : 1)
+    k_fft_x_inv(const C* __restrict__ spec, int64_t hxs, int64_t n1, int64_t n2s, FastDiv Yr, FastDiv Zr, int rows,
+                FastDiv Wd, FftDesc dh, FastDiv Md, FastDiv Mhd, const C* __restrict__ twh, const C* __restrict__ twL,
+                T* __restrict__ out, int Xo, int64_t No, double hscale) {
+    extern __shared__ unsigned char smem_raw[];
+    __shared__ int64_t s_src[64], s_dst[64];
+    C* buf = reinterpret_cast<C*>(smem_raw);
+    const int M = dh.L;
+    const int W = static_cast<int>(Wd.d);
+    const int row0 = static_cast<int>(blockIdx.x) * W;
+    if (static_cast<int>(threadIdx.x) < W) {
+        const int r = row0 + static_cast<int>(threadIdx.x);
+        int64_t src = -1, dst = -1;
+        if (r < rows) {
+            const uint32_t zc = fdiv(r, Yr), y = r - zc * Yr.d;
+            const uint32_t c = fdiv(zc, Zr), z = zc - c * Zr.d;
+            src = ((c * n2s + z) * n1 + y) * hxs;
+            dst = c * No + (static_cast<int64_t>(z) * Yr.d + y) * Xo;
+        }
+        s_src[threadIdx.x] = src;
+        s_dst[threadIdx.x] = dst;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < M * W; e += kAxisBlock) {
+        const int w = static_cast<int>(fdiv(e, Md)), k = e - w * M;
+        const int64_t so = s_src[w];
+        buf[lds_pad(k * W + w)] = so >= 0 ? c2r_pack(spec + so, k, M, twL) : C{0, 0};
+    }
+    __syncthreads();
+    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, dh, Wd, twh, true);
+    const int Mh = static_cast<int>(Mhd.d);
+    for (int e = threadIdx.x; e < Mh * W; e += kAxisBlock) {
+        const int w = static_cast<int>(fdiv(e, Mhd)), n = e - w * Mh;
+        const int64_t dst = s_dst[w];
+        if (dst < 0) continue;
+        const C v = buf[lds_pad(n * W + w)];
+        const int x = 2 * n;
+        out[dst + x] = static_cast<T>(static_cast<double>(v.x) * hscale);
+        if (x + 1 < Xo) out[dst + x + 1] = static_cast<T>(static_cast<double>(v.y) * hscale);
+    }
+}
+
 // Complex transform of strided lines: line (tile, b, a) covers columns kx0 .. kx0+W-1 and holds
 // data[a sa + b sb + kx + pos * stride].  Inputs pos >= n_in are taken as zero; outputs pos >= n_out
 // are not written.
 template <class C, bool POW2>
-__global__ void __launch_bounds__(kAxisBlock)
+__global__ void __launch_bounds__(kAxisBlock, POW2 ? kMinWaves : 1)
     k_fft_strided(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, int n_in, int n_out,
                   FastDiv Wd, FftDesc d, const C* __restrict__ tw, int inv) {
     extern __shared__ unsigned char smem_raw[];
@@ -184,11 +234,11 @@ __global__ void __launch_bounds__(kAxisBlock)
 // Forward transform of zero-padded strided lines (inputs pos < M, L = 2 M outputs) as two
 // length-M transforms: X[2k] = FFT_M(x)[k], X[2k+1] = FFT_M(x[n] w_L^n)[k].  The inputs are kept
 // in registers between the halves, so rows may be overwritten in place.
-template <class C, bool POW2>
-__global__ void __launch_bounds__(kAxisBlock)
+template <class C, bool POW2, int BS, int ELEMS, int MINW = kMinWaves>
+__global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
     k_fft_y_fwd_split(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, FastDiv Wd, FftDesc dh,
                       const C* __restrict__ twh, const C* __restrict__ twL) {
-    constexpr int MAXE = (lds_elems<C>() + kAxisBlock - 1) / kAxisBlock;
+    constexpr int MAXE = (ELEMS + BS - 1) / BS;
     extern __shared__ unsigned char smem_raw[];
     C* buf = reinterpret_cast<C*>(smem_raw);
     const int M = dh.L;
@@ -200,7 +250,7 @@ __global__ void __launch_bounds__(kAxisBlock)
     C v[MAXE];
 #pragma unroll
     for (int i = 0; i < MAXE; ++i) {
-        const int e = threadIdx.x + i * kAxisBlock;
+        const int e = threadIdx.x + i * BS;
         if (e < M * W) {
             const int n = static_cast<int>(fdiv(e, Wd)), w = e - n * W;
             v[i] = w < wv ? base[n * stride + w] : C{0, 0};
@@ -208,20 +258,20 @@ __global__ void __launch_bounds__(kAxisBlock)
         }
     }
     __syncthreads();
-    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, dh, Wd, twh, false);
-    for (int e = threadIdx.x; e < M * W; e += kAxisBlock) {
+    fft_lds<C, ELEMS, POW2, BS>(buf, dh, Wd, twh, false);
+    for (int e = threadIdx.x; e < M * W; e += BS) {
         const int k = static_cast<int>(fdiv(e, Wd)), w = e - k * W;
         if (w < wv) base[(2 * k) * stride + w] = buf[lds_pad(e)];
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < MAXE; ++i) {
-        const int e = threadIdx.x + i * kAxisBlock;
+        const int e = threadIdx.x + i * BS;
         if (e < M * W) buf[lds_pad(e)] = cmul(v[i], twL[fdiv(e, Wd)]);
     }
     __syncthreads();
-    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, dh, Wd, twh, false);
-    for (int e = threadIdx.x; e < M * W; e += kAxisBlock) {
+    fft_lds<C, ELEMS, POW2, BS>(buf, dh, Wd, twh, false);
+    for (int e = threadIdx.x; e < M * W; e += BS) {
         const int k = static_cast<int>(fdiv(e, Wd)), w = e - k * W;
         if (w < wv) base[(2 * k + 1) * stride + w] = buf[lds_pad(e)];
     }
@@ -229,11 +279,11 @@ __global__ void __launch_bounds__(kAxisBlock)
 
 // Inverse transform of strided lines of length L = 2 M keeping outputs n < M:
 // x[n] = IFFT_M(X_even)[n] + w_L^-n IFFT_M(X_odd)[n].
-template <class C, bool POW2>
-__global__ void __launch_bounds__(kAxisBlock)
+template <class C, bool POW2, int BS, int ELEMS, int MINW = kMinWaves>
+__global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
     k_fft_y_inv_split(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, FastDiv Wd, FftDesc dh,
                       const C* __restrict__ twh, const C* __restrict__ twL) {
-    constexpr int MAXE = (lds_elems<C>() + kAxisBlock - 1) / kAxisBlock;
+    constexpr int MAXE = (ELEMS + BS - 1) / BS;
     extern __shared__ unsigned char smem_raw[];
     C* buf = reinterpret_cast<C*>(smem_raw);
     const int M = dh.L;
@@ -242,28 +292,28 @@ __global__ void __launch_bounds__(kAxisBlock)
     const int64_t kx0 = static_cast<int64_t>(blk.x) * W;
     C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + kx0;
     const int wv = static_cast<int>(hx - kx0 < W ? hx - kx0 : W);
-    for (int e = threadIdx.x; e < M * W; e += kAxisBlock) {
+    for (int e = threadIdx.x; e < M * W; e += BS) {
         const int k = static_cast<int>(fdiv(e, Wd)), w = e - k * W;
         buf[lds_pad(e)] = w < wv ? base[(2 * k) * stride + w] : C{0, 0};
     }
     __syncthreads();
-    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, dh, Wd, twh, true);
+    fft_lds<C, ELEMS, POW2, BS>(buf, dh, Wd, twh, true);
     C v[MAXE];
 #pragma unroll
     for (int i = 0; i < MAXE; ++i) {
-        const int e = threadIdx.x + i * kAxisBlock;
+        const int e = threadIdx.x + i * BS;
         if (e < M * W) v[i] = buf[lds_pad(e)];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < M * W; e += kAxisBlock) {
+    for (int e = threadIdx.x; e < M * W; e += BS) {
         const int k = static_cast<int>(fdiv(e, Wd)), w = e - k * W;
         buf[lds_pad(e)] = w < wv ? base[(2 * k + 1) * stride + w] : C{0, 0};
     }
     __syncthreads();
-    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, dh, Wd, twh, true);
+    fft_lds<C, ELEMS, POW2, BS>(buf, dh, Wd, twh, true);
 #pragma unroll
     for (int i = 0; i < MAXE; ++i) {
-        const int e = threadIdx.x + i * kAxisBlock;
+        const int e = threadIdx.x + i * BS;
         if (e < M * W) {
             const int n = static_cast<int>(fdiv(e, Wd)), w = e - n * W;
             if (w < wv) base[n * stride + w] = cadd(v[i], cmul(conj_if(twL[n], true), buf[lds_pad(e)]));
@@ -500,6 +550,30 @@ void fft_x_forward(mafx_ctx* ctx, const double* in, const double* ms, double sca
     MAFX_HIP(hipGetLastError());
 }
 
+template <class C, class T>
+void fft_x_inverse(mafx_ctx* ctx, const C* spec, int64_t n2s, T* out, double hscale) {
+    const FftPlan& ph = ctx->pxh;
+    const int M = ph.L;
+    const mafx_mesh& g = ctx->mesh;
+    require(M <= lds_elems<C>(), "x transform length exceeds the LDS budget");
+    const int W = lines_per_block(M, lds_elems<C>());
+    const int64_t rows = 3 * g.nz * g.ny;
+    require(rows * (M + 1) < (int64_t{1} << 31), "x transform: too many rows");
+    const unsigned grid = static_cast<unsigned>((rows + W - 1) / W);
+    const double bytes = static_cast<double>(rows) * (ctx->hx * sizeof(C) + g.nx * sizeof(T));
+    const FftDesc d = desc_of(ph);
+    ProfScope ps(ctx, nm("fft_x_inv_f32", "fft_x_inv_f64", sizeof(C)), bytes);
+    auto fn = is_pow2_plan(d) ? k_fft_x_inv<C, T, true> : k_fft_x_inv<C, T, false>;
+    const size_t lds = lds_padded(W * M) * sizeof(C);
+    allow_lds(reinterpret_cast<const void*>(fn), lds);
+    const int Mh = static_cast<int>((g.nx + 1) / 2);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kAxisBlock), lds, ctx->stream, spec, ctx->hxs, ctx->n1, n2s,
+                       make_fastdiv(static_cast<uint32_t>(g.ny)), make_fastdiv(static_cast<uint32_t>(g.nz)),
+                       static_cast<int>(rows), make_fastdiv(W), d, make_fastdiv(M), make_fastdiv(Mh), tw_of<C>(ph),
+                       tw_of<C>(ctx->px), out, static_cast<int>(g.nx), ctx->N, hscale);
+    MAFX_HIP(hipGetLastError());
+}
+
 template <class C>
 void fft_strided(mafx_ctx* ctx, const FftPlan& p, C* data, int64_t sa, int na, int64_t sb, int nb, int64_t stride,
                  int n_in, int n_out, bool inv, const char* name) {
@@ -525,18 +599,34 @@ void fft_y_split(mafx_ctx* ctx, C* data, int64_t sa, int na, int64_t sb, int nb,
     const int M = ph.L;
     require(M <= lds_elems<C>(), "y transform length exceeds the LDS budget");
     require(nb <= 65535 && na <= 65535, "y transform grid too large");
-    const int W = lines_per_block(M, lds_elems<C>());
+    const FftDesc d = desc_of(ph);
+    // Tile shapes, measured at 1024 x 1024 x 4: power-of-two FP32 plans run best with a 32 KiB
+    // LDS tile per 512-lane work-group and a 6-wave register cap (three resident work-groups per
+    // CU, 8 elements per lane, no spills); FP64 with a 64 KiB tile and the 4-wave cap (two
+    // work-groups).  Lines longer than the small tile and other plans use the 64 KiB shape.
+    constexpr int E64 = lds_elems<C>();
+    constexpr bool kF32 = sizeof(C) == sizeof(float2);
+    constexpr int ET = kF32 ? E64 / 2 : E64;
+    constexpr int MW = kF32 ? 6 : kMinWaves;
+    void (*fn)(C*, int64_t, int64_t, int64_t, int64_t, FastDiv, FftDesc, const C*, const C*) = nullptr;
+    int elems = E64;
+    if (is_pow2_plan(d) && M <= ET) {
+        elems = ET;
+        fn = inv ? k_fft_y_inv_split<C, true, kAxisBlock, ET, MW> : k_fft_y_fwd_split<C, true, kAxisBlock, ET, MW>;
+    } else if (is_pow2_plan(d)) {
+        fn = inv ? k_fft_y_inv_split<C, true, kAxisBlock, E64> : k_fft_y_fwd_split<C, true, kAxisBlock, E64>;
+    } else {
+        fn = inv ? k_fft_y_inv_split<C, false, kAxisBlock, E64> : k_fft_y_fwd_split<C, false, kAxisBlock, E64>;
+    }
+    require(M <= elems, "y transform length exceeds the LDS budget");
+    const int W = lines_per_block(M, elems);
     const unsigned gx = static_cast<unsigned>((ctx->hx + W - 1) / W);
     const double bytes = static_cast<double>(na) * nb * ctx->hx * (3.0 * M) * sizeof(C);
-    const FftDesc d = desc_of(ph);
     const C* twL = tw_of<C>(ctx->py);
     ProfScope ps(ctx, inv ? nm("fft_y_inv_f32", "fft_y_inv_f64", sizeof(C)) : nm("fft_y_fwd_f32", "fft_y_fwd_f64", sizeof(C)),
                  bytes);
     const dim3 g(gx, nb, na), b(kAxisBlock);
     const size_t lds = lds_padded(W * M) * sizeof(C);
-    const bool p2 = is_pow2_plan(d);
-    auto fn = inv ? (p2 ? k_fft_y_inv_split<C, true> : k_fft_y_inv_split<C, false>)
-                  : (p2 ? k_fft_y_fwd_split<C, true> : k_fft_y_fwd_split<C, false>);
     allow_lds(reinterpret_cast<const void*>(fn), lds);
     hipLaunchKernelGGL(fn, g, b, lds, ctx->stream, data, sa, sb, stride, ctx->hx, make_fastdiv(W), d, tw_of<C>(ph),
                        twL);
@@ -635,6 +725,8 @@ template void fft_x_forward<double2>(mafx_ctx*, const double*, const double*, do
                                      int64_t, int64_t, double2*, int64_t);
 template void fft_x_forward<float2>(mafx_ctx*, const double*, const double*, double, int64_t, int64_t, int64_t, int,
                                     int64_t, int64_t, float2*, int64_t);
+template void fft_x_inverse<double2, double>(mafx_ctx*, const double2*, int64_t, double*, double);
+template void fft_x_inverse<float2, float>(mafx_ctx*, const float2*, int64_t, float*, double);
 template void fft_strided<double2>(mafx_ctx*, const FftPlan&, double2*, int64_t, int, int64_t, int, int64_t, int,
                                    int, bool, const char*);
 template void fft_strided<float2>(mafx_ctx*, const FftPlan&, float2*, int64_t, int, int64_t, int, int64_t, int, int,

@@ -98,6 +98,18 @@ __device__ __forceinline__ void dft(C (&v)[R], bool inv, const C* __restrict__ t
     }
 }
 
+// Packs row k of the half spectrum X[0..M] for the half-length inverse transform of a real
+// sequence of length 2M: Zp[k] = (X[k] + conj X[M-k]) + i w_{2M}^-k (X[k] - conj X[M-k]); the
+// unnormalised inverse FFT_M of Zp then holds x[2n] + i x[2n+1].
+template <class C>
+__device__ __forceinline__ C c2r_pack(const C* __restrict__ row, int k, int M, const C* __restrict__ twL) {
+    const C Xk = row[k], Xm = row[M - k];
+    const C S{Xk.x + Xm.x, Xk.y - Xm.y};
+    const C D{Xk.x - Xm.x, Xk.y + Xm.y};
+    const C wD = cmul(conj_if(twL[k], true), D);
+    return C{S.x - wD.y, S.y + wD.x};
+}
+
 // Radix schedule handed to kernels by value, with the per-stage constants precomputed on the
 // host: ns[s] = Ns (product of the radices before stage s) as a divisor, twstep[s] = L / (Ns R).
 struct FftDesc {

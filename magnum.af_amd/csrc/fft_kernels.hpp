@@ -23,6 +23,9 @@ template <class C> constexpr int lds_elems() { return 65536 / static_cast<int>(s
 template <class C>
 void fft_x_forward(mafx_ctx* ctx, const double* in, const double* ms, double scale, int64_t Xi, int64_t Yi,
                    int64_t Zi, int ncomp, int64_t Yr, int64_t Zr, C* out, int64_t n2o);
+// inverse real transform along x of the spectrum rows (c < 3, z < nz, y < ny; n2s z-planes per
+// component) cropped to x < nx and scaled: out[c N + (z ny + y) nx + x]
+template <class C, class T> void fft_x_inverse(mafx_ctx* ctx, const C* spec, int64_t n2s, T* out, double hscale);
 // general strided complex transform (full length, optional zero padding / cropping)
 template <class C>
 void fft_strided(mafx_ctx* ctx, const FftPlan& p, C* data, int64_t sa, int na, int64_t sb, int nb, int64_t stride,

@@ -1,7 +1,10 @@
-// Effective-field evaluation: demag tensor setup, the zero-padded FFT convolution, and one fused
-// epilogue that finishes the inverse x transform (C2R + crop + 1/P) and adds every local term in
-// term order, then writes H, the LLG right-hand side, or the LBFGS gradient, and accumulates
-// per-term energy partial sums.
+// Effective-field evaluation: demag tensor setup, the zero-padded FFT convolution (its last pass,
+// the inverse x transform with C2R + crop + 1/P, writes the convolution field), and one per-cell
+// kernel that adds every local term in term order, then writes H, the LLG right-hand side, or the
+// LBFGS gradient, and accumulates per-term energy partial sums.  Running the inverse x transform
+// and the stencils as two kernels lets each hold its best occupancy (an LDS transform and a
+// register-heavy stencil in one kernel left two waves per SIMD); the extra field round trip costs
+// less than that loss at 1024 x 1024 x 4.
 //
 // Replaces (reference paths relative to magnumaf/src):
 //   field_terms/micro/demag_field.cpp:104-264, exchange_field.cpp:14-60,115-157,
@@ -56,6 +59,7 @@ struct EpiParams {
     int want_energy;
     double* partials; // [nslots][nblocks]
     int64_t nblocks;
+    const void* hd; // FFT-convolution field [3][N] of type T for the per-cell kernel, or null
 };
 
 namespace {
@@ -89,9 +93,11 @@ __device__ void block_reduce_store(double* vals, int nv, const int* is_max, doub
 }
 
 // All local terms and the output stage for one cell.  hd holds the FFT-convolution field of the
-// cell (ignored when P.fft_slot < 0).  acc[] collects the energy partials (slots 0..nterms-1),
-// acc[nterms] = max |m x H|, acc[nterms+1] = max |H| (gradient mode).
-template <class T>
+// cell (ignored when P.fft_slot < 0).  With RED (energies or the gradient mode's maxima wanted)
+// acc[] collects the energy partials (slots 0..nterms-1), acc[nterms] = max |m x H|,
+// acc[nterms+1] = max |H| (gradient mode); without RED acc is not touched, which keeps the
+// accumulators out of the register budget of the plain field and LLG evaluations.
+template <class T, bool RED>
 __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t z, const T hd[3], double* acc) {
     const int64_t N = P.N;
     const int64_t i = x + P.nx * (y + P.ny * z);
@@ -152,7 +158,7 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
                 h0 = c * l0;
                 h1 = c * l1;
                 h2 = c * l2;
-                if (P.want_energy) {
+                if (RED && P.want_energy) {
                     // Neumann correction of exchange_field.cpp:44-51: + m / d^2 on every boundary face
                     T corr = 0;
                     const T mm = m0 * m0 + m1 * m1 + m2 * m2;
@@ -297,7 +303,7 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
             H1 = H1 + h1;
             H2 = H2 + h2;
         }
-        if (P.want_energy) {
+        if (RED && P.want_energy) {
             const double w = (P.ms && tm.kind < MAFX_ATOM_DIPOLE) ? msi : 1.0;
             acc[t] += w * (static_cast<double>(h0) * mc0 + static_cast<double>(h1) * mc1 +
                            static_cast<double>(h2) * mc2 + static_cast<double>(e_extra));
@@ -343,16 +349,18 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
         o[i] = mc1 * a2 - mc2 * a1;
         o[N + i] = mc2 * a0 - mc0 * a2;
         o[2 * N + i] = mc0 * a1 - mc1 * a0;
-        acc[P.nterms] = fmax(acc[P.nterms], fmax(fabs(a0), fmax(fabs(a1), fabs(a2))));
-        acc[P.nterms + 1] = fmax(acc[P.nterms + 1], fmax(fabs(Hx), fmax(fabs(Hy), fabs(Hz))));
+        if constexpr (RED) {
+            acc[P.nterms] = fmax(acc[P.nterms], fmax(fabs(a0), fmax(fabs(a1), fabs(a2))));
+            acc[P.nterms + 1] = fmax(acc[P.nterms + 1], fmax(fabs(Hx), fmax(fabs(Hy), fabs(Hz))));
+        }
         break;
     }
     default: break;
     }
 }
 
-__device__ void finish_block(const EpiParams& P, double* acc) {
-    if (!P.want_energy && P.mode != static_cast<int>(OutMode::GRAD)) return;
+template <bool RED> __device__ void finish_block(const EpiParams& P, double* acc) {
+    if constexpr (!RED) return;
     int is_max[kMaxTerms + 2];
     for (int s = 0; s < P.nterms; ++s) is_max[s] = 0;
     is_max[P.nterms] = 1;
@@ -360,103 +368,45 @@ __device__ void finish_block(const EpiParams& P, double* acc) {
     block_reduce_store(acc, P.nterms + 2, is_max, P.partials, P.nblocks);
 }
 
-// Packs row k of the half spectrum X[0..M] for the half-length inverse transform of a real
-// sequence of length 2M: Zp[k] = (X[k] + conj X[M-k]) + i w_{2M}^-k (X[k] - conj X[M-k]); the
-// unnormalised inverse FFT_M of Zp then holds x[2n] + i x[2n+1].
-template <class C>
-__device__ __forceinline__ C c2r_pack(const C* __restrict__ row, int k, int M, const C* __restrict__ twL) {
-    const C Xk = row[k], Xm = row[M - k];
-    const C S{Xk.x + Xm.x, Xk.y - Xm.y};
-    const C D{Xk.x - Xm.x, Xk.y + Xm.y};
-    const C wD = cmul(conj_if(twL[k], true), D);
-    return C{S.x - wD.y, S.y + wD.x};
-}
-
-// Inverse x transform + local terms.  A work-group owns R consecutive rows of the z-fastest row
-// order (row = y nz + z) after the XCD renumbering, so the y and z stencil neighbours of a row are
-// read by work-groups behind the same L2.  The three
-// components of each row are W = 3 R lines of one LDS transform of length M = nx (C2R through the
-// half-length packing), so a row costs one barrier sequence instead of three.  After the
-// transform the real field value of cell x of line w sits in buf[(x / 2) W + w] (.x for even x,
-// .y for odd x).  SEQ handles rows too wide for 3 lines in LDS: one component at a time, the
-// cropped real values parked in a second LDS array.
-template <class C, class T, bool POW2, bool SEQ>
-__global__ void __launch_bounds__(kFftBlock)
-    k_epi_rows(EpiParams P, FftDesc dh, const C* __restrict__ twh, const C* __restrict__ twL,
-               const C* __restrict__ spec, int64_t hxs, int64_t n1, int64_t n2s, double hscale, int R, FastDiv Md,
-               FastDiv Wd, FastDiv nxd, FastDiv nzd) {
-    extern __shared__ unsigned char smem_raw[];
-    C* buf = reinterpret_cast<C*>(smem_raw);
-    const int M = dh.L;
-    const int nx = static_cast<int>(P.nx), ny = static_cast<int>(P.ny);
-    const int nrows = ny * static_cast<int>(P.nz);
-    const int nz = static_cast<int>(P.nz);
-    const int row0 = static_cast<int>(xcd_swizzle(blockIdx.x, gridDim.x)) * R;
+// One thread per cell; the FFT-convolution field, if any, comes from P.hd ([3][N], type T).
+// ROWS maps work-group b (after the XCD renumbering) to a tile of kBlock cells of row
+// b / bpr in the z-fastest row order (row = y nz + z), so the y and z stencil neighbours of a
+// tile are read behind the same L2; otherwise cells are taken in memory order.
+template <class T, bool RED, bool ROWS>
+__global__ void __launch_bounds__(kBlock) k_epi_cells(EpiParams P, FastDiv nxd, FastDiv nyd, FastDiv nzd,
+                                                      FastDiv bprd) {
     double acc[kMaxTerms + 2];
     for (int s = 0; s < kMaxTerms + 2; ++s) acc[s] = 0;
-    if constexpr (!SEQ) {
-        const int W = static_cast<int>(Wd.d); // 3 R lines
-        for (int e = threadIdx.x; e < M * W; e += kFftBlock) {
-            const int w = static_cast<int>(fdiv(e, Md)), k = e - w * M;
-            const int r = w / 3, c = w - 3 * r;
-            const int row = row0 + r;
-            C v{0, 0};
-            if (row < nrows) {
-                const int y = static_cast<int>(fdiv(row, nzd)), z = row - y * nz;
-                v = c2r_pack(spec + ((c * n2s + z) * n1 + y) * hxs, k, M, twL);
-            }
-            buf[lds_pad(k * W + w)] = v;
-        }
-        __syncthreads();
-        fft_lds<C, 65536 / static_cast<int>(sizeof(C)), POW2>(buf, dh, Wd, twh, true);
-        for (int e = threadIdx.x; e < R * nx; e += kFftBlock) {
-            const int r = static_cast<int>(fdiv(e, nxd)), x = e - r * nx;
-            const int row = row0 + r;
-            if (row >= nrows) continue;
-            const int y = static_cast<int>(fdiv(row, nzd)), z = row - y * nz;
-            T hd[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const C v = buf[lds_pad((x >> 1) * W + 3 * r + c)];
-                hd[c] = static_cast<T>(static_cast<double>((x & 1) ? v.y : v.x) * hscale);
-            }
-            cell_epilogue<T>(P, x, y, z, hd, acc);
-        }
+    const int nx = static_cast<int>(P.nx), ny = static_cast<int>(P.ny), nz = static_cast<int>(P.nz);
+    int x, y, z;
+    bool valid;
+    if constexpr (ROWS) {
+        const int lb = static_cast<int>(xcd_swizzle(blockIdx.x, gridDim.x));
+        const int row = static_cast<int>(fdiv(lb, bprd)), xc = lb - row * static_cast<int>(bprd.d);
+        y = static_cast<int>(fdiv(row, nzd));
+        z = row - y * nz;
+        x = xc * kBlock + static_cast<int>(threadIdx.x);
+        valid = x < nx;
     } else {
-        T* res = reinterpret_cast<T*>(smem_raw + sizeof(C) * lds_padded(M));
-        const int row = row0;
-        const int y = static_cast<int>(fdiv(row, nzd)), z = row - y * nz;
-        for (int c = 0; c < 3; ++c) {
-            for (int k = threadIdx.x; k < M; k += kFftBlock)
-                buf[lds_pad(k)] = c2r_pack(spec + ((c * n2s + z) * n1 + y) * hxs, k, M, twL);
-            __syncthreads();
-            fft_lds<C, 65536 / static_cast<int>(sizeof(C)), POW2>(buf, dh, Wd, twh, true);
-            for (int64_t x = threadIdx.x; x < P.nx; x += kFftBlock) {
-                const C v = buf[lds_pad(x >> 1)];
-                res[c * P.nx + x] = static_cast<T>(static_cast<double>((x & 1) ? v.y : v.x) * hscale);
-            }
-            __syncthreads();
-        }
-        for (int64_t x = threadIdx.x; x < P.nx; x += kFftBlock) {
-            const T hd[3] = {res[x], res[P.nx + x], res[2 * P.nx + x]};
-            cell_epilogue<T>(P, x, y, z, hd, acc);
-        }
+        const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+        valid = i < P.N;
+        const int yz = static_cast<int>(fdiv(i, nxd));
+        x = i - yz * nx;
+        z = static_cast<int>(fdiv(yz, nyd));
+        y = yz - z * ny;
     }
-    finish_block(P, acc);
-}
-
-// Local terms only (no FFT term): one thread per cell.
-template <class T> __global__ void __launch_bounds__(kBlock) k_epi_cells(EpiParams P, FastDiv nxd, FastDiv nyd) {
-    double acc[kMaxTerms + 2];
-    for (int s = 0; s < kMaxTerms + 2; ++s) acc[s] = 0;
-    const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
-    if (i < P.N) {
-        const int yz = static_cast<int>(fdiv(i, nxd)), x = i - yz * static_cast<int>(P.nx);
-        const int z = static_cast<int>(fdiv(yz, nyd)), y = yz - z * static_cast<int>(P.ny);
-        const T hd[3] = {0, 0, 0};
-        cell_epilogue<T>(P, x, y, z, hd, acc);
+    if (valid) {
+        T hd[3] = {0, 0, 0};
+        if (P.hd) {
+            const T* h = static_cast<const T*>(P.hd);
+            const int64_t i = x + P.nx * (y + P.ny * static_cast<int64_t>(z));
+            hd[0] = h[i];
+            hd[1] = h[P.N + i];
+            hd[2] = h[2 * P.N + i];
+        }
+        cell_epilogue<T, RED>(P, x, y, z, hd, acc);
     }
-    finish_block(P, acc);
+    finish_block<RED>(P, acc);
 }
 
 // Fixed-order reduction of partials[s][0..nb) into out[s].
@@ -549,11 +499,10 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
     // cell and row indices are 32-bit inside the kernels
     require(ctx->N < (int64_t{1} << 31), "meshes of 2^31 cells or more are not supported");
     const int64_t nrows = g.ny * g.nz;
-    // epilogue rows per work-group: 3 R lines of length nx must fit the 64 KiB LDS transform;
-    // R == 0 selects the one-component-at-a-time variant
-    constexpr int kElems = 65536 / static_cast<int>(sizeof(C));
-    const int R = fft ? static_cast<int>(std::min<int64_t>(4, kElems / (3 * g.nx))) : 0;
-    const int64_t nblocks = fft ? (R > 0 ? (nrows + R - 1) / R : nrows) : (ctx->N + kBlock - 1) / kBlock;
+    // per-cell kernel: row tiles when rows are at least one work-group long
+    const bool rowmap = g.nx >= kBlock;
+    const int64_t bpr = (g.nx + kBlock - 1) / kBlock;
+    const int64_t nblocks = rowmap ? nrows * bpr : (ctx->N + kBlock - 1) / kBlock;
     require((ctx->partials.bytes / sizeof(double)) >= static_cast<size_t>((kMaxTerms + 2) * nblocks),
             "partials buffer too small");
     EpiParams P = make_params(ctx, a, nblocks);
@@ -561,7 +510,7 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
                                : (a.mode == OutMode::ENERGY) ? 0.0 : 24.0;
     if (fft) {
         C* spec = spec_of<C>(ctx);
-        const int64_t hx = ctx->hx, hxs = ctx->hxs, n1 = ctx->n1, n2 = ctx->n2, nzs = ctx->nzs;
+        const int64_t hxs = ctx->hxs, n1 = ctx->n1, n2 = ctx->n2, nzs = ctx->nzs;
         const int64_t sa = nzs * n1 * hxs;
         // Ms * m (micro; Ms_field per cell) or m (atomistic dipole) -> x half spectrum
         const double* ms = (!ctx->atom_dipole && ctx->has_ms_field) ? ctx->ms_field.as<double>() : nullptr;
@@ -592,44 +541,22 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
         fft_y_split<C>(ctx, spec, sa, 3, n1 * hxs, static_cast<int>(g.nz), hxs, true);
         const double P_total = static_cast<double>(ctx->n0) * n1 * n2;
         const double hscale = (ctx->atom_dipole ? ctx->Ms : 1.0) / P_total;
-        const int M = ctx->pxh.L;
-        const bool seq = R == 0;
-        const size_t lds = seq ? sizeof(C) * lds_padded(M) + sizeof(T) * 3 * g.nx : sizeof(C) * lds_padded(M * 3 * R);
-        require(M <= kElems, "x transform length exceeds the LDS FFT budget");
-        require(lds <= 160 * 1024, "epilogue LDS exceeds 160 KiB (mesh too wide in x)");
-        // algorithmic bytes: the three cropped spectrum rows, m once, the output once
-        const double bytes = static_cast<double>(ctx->N) * (3.0 * sizeof(C) * hx / g.nx + 24.0 + field_bytes);
-        ProfScope ps(ctx, sizeof(C) == 8 ? "epilogue_rows_f32" : "epilogue_rows_f64", bytes);
-        const FftDesc dh = desc_of(ctx->pxh);
-        const bool pow2 = is_pow2_plan(dh);
-        const void* fn = pow2 ? (seq ? reinterpret_cast<const void*>(&k_epi_rows<C, T, true, true>)
-                                     : reinterpret_cast<const void*>(&k_epi_rows<C, T, true, false>))
-                              : (seq ? reinterpret_cast<const void*>(&k_epi_rows<C, T, false, true>)
-                                     : reinterpret_cast<const void*>(&k_epi_rows<C, T, false, false>));
-        MAFX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        const C* twh = tw_of<C>(ctx->pxh);
-        const C* twL = tw_of<C>(ctx->px);
-        const dim3 grid(static_cast<unsigned>(nblocks)), blk(kFftBlock);
-        const FastDiv Md = make_fastdiv(M), Wd = make_fastdiv(seq ? 1 : 3 * R);
-        const FastDiv nxd = make_fastdiv(static_cast<uint32_t>(g.nx)), nzd = make_fastdiv(static_cast<uint32_t>(g.nz));
-        if (pow2 && !seq)
-            hipLaunchKernelGGL((k_epi_rows<C, T, true, false>), grid, blk, lds, ctx->stream, P, dh, twh, twL, spec, hxs,
-                               n1, nzs, hscale, R, Md, Wd, nxd, nzd);
-        else if (pow2)
-            hipLaunchKernelGGL((k_epi_rows<C, T, true, true>), grid, blk, lds, ctx->stream, P, dh, twh, twL, spec, hxs,
-                               n1, nzs, hscale, 1, Md, Wd, nxd, nzd);
-        else if (!seq)
-            hipLaunchKernelGGL((k_epi_rows<C, T, false, false>), grid, blk, lds, ctx->stream, P, dh, twh, twL, spec,
-                               hxs, n1, nzs, hscale, R, Md, Wd, nxd, nzd);
-        else
-            hipLaunchKernelGGL((k_epi_rows<C, T, false, true>), grid, blk, lds, ctx->stream, P, dh, twh, twL, spec,
-                               hxs, n1, nzs, hscale, 1, Md, Wd, nxd, nzd);
-        MAFX_HIP(hipGetLastError());
-    } else {
-        ProfScope ps(ctx, sizeof(T) == 4 ? "local_cells_f32" : "local_cells_f64",
-                     static_cast<double>(ctx->N) * (24.0 + field_bytes));
-        hipLaunchKernelGGL(k_epi_cells<T>, dim3(static_cast<unsigned>(nblocks)), dim3(kBlock), 0, ctx->stream, P,
-                           make_fastdiv(static_cast<uint32_t>(g.nx)), make_fastdiv(static_cast<uint32_t>(g.ny)));
+        T* hd = ctx->hdem.as<T>();
+        fft_x_inverse<C, T>(ctx, spec, nzs, hd, hscale);
+        P.hd = hd;
+    }
+    {
+        // per-cell kernel: local terms, plus the FFT field read back from P.hd
+        const double hd_bytes = P.hd ? 3.0 * sizeof(T) : 0.0;
+        ProfScope ps(ctx, P.hd ? (sizeof(T) == 4 ? "epilogue_cells_f32" : "epilogue_cells_f64")
+                               : (sizeof(T) == 4 ? "local_cells_f32" : "local_cells_f64"),
+                     static_cast<double>(ctx->N) * (24.0 + field_bytes + hd_bytes));
+        const bool red = P.want_energy || P.mode == static_cast<int>(OutMode::GRAD);
+        auto kern = red ? (rowmap ? k_epi_cells<T, true, true> : k_epi_cells<T, true, false>)
+                        : (rowmap ? k_epi_cells<T, false, true> : k_epi_cells<T, false, false>);
+        hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nblocks)), dim3(kBlock), 0, ctx->stream, P,
+                           make_fastdiv(static_cast<uint32_t>(g.nx)), make_fastdiv(static_cast<uint32_t>(g.ny)),
+                           make_fastdiv(static_cast<uint32_t>(g.nz)), make_fastdiv(static_cast<uint32_t>(bpr)));
         MAFX_HIP(hipGetLastError());
     }
     ctx->scal_nblocks = nblocks;

@@ -134,6 +134,7 @@ struct mafx_ctx {
     // FP32 path: [6][n2h][n1h][hxs] real f32, the mirror-symmetric spectrum (fft.hip).
     mafx::DevBuf tensor;
     mafx::DevBuf spec;        // [3][nzs][n1][hxs] complex workspace
+    mafx::DevBuf hdem;        // [3][N] FFT-convolution field (f64 or f32) between the x inverse and the cell pass
     mafx::DevBuf hbuf;        // 3N f64 scratch (field output for helpers)
     mafx::DevBuf partials;    // reduction partials (f64)
     mafx::DevBuf scal;        // small device scalars
