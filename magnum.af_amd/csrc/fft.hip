@@ -94,8 +94,8 @@ constexpr int kMinWaves = 4;
 // O = -i (Z[k] - conj Z[M-k]) / 2, k = 0..M, written to out[((c n2o + z) n1 + y) hxs + k].
 // Lanes walk along rows (consecutive lanes, consecutive addresses) for the global accesses.  The
 // (c, z, y) decomposition of each of the block's W <= 64 rows is done once, into LDS.
-template <class C, bool POW2>
-__global__ void __launch_bounds__(kAxisBlock, POW2 ? kMinWaves : 1)
+template <class C, bool POW2, int ELEMS = lds_elems<C>(), int MINW = kMinWaves>
+__global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
     k_fft_x_fwd(const double* __restrict__ in, const double* __restrict__ ms, double scale, int Xi, int64_t Yi,
                 int64_t Zi, FastDiv Yr, FastDiv Zr, int rows, FastDiv Wd, FftDesc dh, FastDiv Md, FastDiv M1d,
                 const C* __restrict__ twh, const C* __restrict__ twL, C* __restrict__ out, int64_t hxs, int64_t n1,
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? kMinWaves : 1)
         buf[lds_pad(n * W + w)] = C{re, im};
     }
     __syncthreads();
-    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, dh, Wd, twh, false);
+    fft_lds<C, ELEMS, POW2, kAxisBlock>(buf, dh, Wd, twh, false);
     const T half = static_cast<T>(0.5);
     for (int e = threadIdx.x; e < (M + 1) * W; e += kAxisBlock) {
         const int w = static_cast<int>(fdiv(e, M1d)), k = e - w * (M + 1);
@@ -159,8 +159,8 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? kMinWaves : 1)
 // spec[((c n2s + z) n1 + y) hxs + k]: C2R through the half-length packing (c2r_pack), an inverse
 // FFT of length M = n0 / 2, the crop to x < Xo and the 1 / P scale,
 // out[c No + (z Yr + y) Xo + x] = T(hscale * x_real[x]).
-template <class C, class T, bool POW2>
-__global__ void __launch_bounds__(kAxisBlock, POW2 ? kMinWaves : 1)
+template <class C, class T, bool POW2, int ELEMS = lds_elems<C>(), int MINW = kMinWaves>
+__global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
     k_fft_x_inv(const C* __restrict__ spec, int64_t hxs, int64_t n1, int64_t n2s, FastDiv Yr, FastDiv Zr, int rows,
                 FastDiv Wd, FftDesc dh, FastDiv Md, FastDiv Mhd, const C* __restrict__ twh, const C* __restrict__ twL,
                 T* __restrict__ out, int Xo, int64_t No, double hscale) {
@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? kMinWaves : 1)
         buf[lds_pad(k * W + w)] = so >= 0 ? c2r_pack(spec + so, k, M, twL) : C{0, 0};
     }
     __syncthreads();
-    fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, dh, Wd, twh, true);
+    fft_lds<C, ELEMS, POW2, kAxisBlock>(buf, dh, Wd, twh, true);
     const int Mh = static_cast<int>(Mhd.d);
     for (int e = threadIdx.x; e < Mh * W; e += kAxisBlock) {
         const int w = static_cast<int>(fdiv(e, Mhd)), n = e - w * Mh;
@@ -532,7 +532,11 @@ void fft_x_forward(mafx_ctx* ctx, const double* in, const double* ms, double sca
     const FftPlan& ph = ctx->pxh;
     const int M = ph.L;
     require(M <= lds_elems<C>(), "x transform length exceeds the LDS budget");
-    const int W = lines_per_block(M, lds_elems<C>());
+    // Power-of-two rows that fit use the 32 KiB tile under a 6-wave register cap (three resident
+    // work-groups per CU), measured faster than the 64 KiB tile at 1024 x 1024 x 4 in both precisions.
+    constexpr int E64 = lds_elems<C>();
+    const bool small = is_pow2_plan(desc_of(ph)) && M <= E64 / 2;
+    const int W = lines_per_block(M, small ? E64 / 2 : E64);
     const int64_t rows = static_cast<int64_t>(ncomp) * Zr * Yr;
     require(rows * (M + 1) < (int64_t{1} << 31) && Xi < (int64_t{1} << 30), "x transform: too many rows");
     const unsigned grid = static_cast<unsigned>((rows + W - 1) / W);
@@ -541,6 +545,7 @@ void fft_x_forward(mafx_ctx* ctx, const double* in, const double* ms, double sca
     ProfScope ps(ctx, nm("fft_x_fwd_f32", "fft_x_fwd_f64", sizeof(C)), bytes);
     const C* twL = tw_of<C>(ctx->px);
     auto fn = is_pow2_plan(d) ? k_fft_x_fwd<C, true> : k_fft_x_fwd<C, false>;
+    if (small) fn = k_fft_x_fwd<C, true, E64 / 2, 6>;
     const size_t lds = lds_padded(W * M) * sizeof(C);
     allow_lds(reinterpret_cast<const void*>(fn), lds);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kAxisBlock), lds, ctx->stream, in, ms, scale, static_cast<int>(Xi), Yi, Zi,
@@ -556,7 +561,9 @@ void fft_x_inverse(mafx_ctx* ctx, const C* spec, int64_t n2s, T* out, double hsc
     const int M = ph.L;
     const mafx_mesh& g = ctx->mesh;
     require(M <= lds_elems<C>(), "x transform length exceeds the LDS budget");
-    const int W = lines_per_block(M, lds_elems<C>());
+    constexpr int E64 = lds_elems<C>();
+    const bool small = is_pow2_plan(desc_of(ph)) && M <= E64 / 2; // as in fft_x_forward
+    const int W = lines_per_block(M, small ? E64 / 2 : E64);
     const int64_t rows = 3 * g.nz * g.ny;
     require(rows * (M + 1) < (int64_t{1} << 31), "x transform: too many rows");
     const unsigned grid = static_cast<unsigned>((rows + W - 1) / W);
@@ -564,6 +571,7 @@ void fft_x_inverse(mafx_ctx* ctx, const C* spec, int64_t n2s, T* out, double hsc
     const FftDesc d = desc_of(ph);
     ProfScope ps(ctx, nm("fft_x_inv_f32", "fft_x_inv_f64", sizeof(C)), bytes);
     auto fn = is_pow2_plan(d) ? k_fft_x_inv<C, T, true> : k_fft_x_inv<C, T, false>;
+    if (small) fn = k_fft_x_inv<C, T, true, E64 / 2, 6>;
     const size_t lds = lds_padded(W * M) * sizeof(C);
     allow_lds(reinterpret_cast<const void*>(fn), lds);
     const int Mh = static_cast<int>((g.nx + 1) / 2);

@@ -133,13 +133,20 @@ struct LinArgs {
 };
 
 // out = ((base + c0 v0) + c1 v1) + ...  (the element-wise evaluation order of the reference's
-// stage inputs, adaptive_runge_kutta.cpp:139-165)
+// stage inputs, adaptive_runge_kutta.cpp:139-165).  N is the vector count, so the loads of one
+// element are issued together; terms with a zero weight are skipped (x + 0 v == x for finite v).
+template <int N>
 __global__ void __launch_bounds__(kB) k_lincomb(double* __restrict__ out, const double* __restrict__ base, LinArgs a,
                                                int64_t len) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
     if (i >= len) return;
+    double v[N > 0 ? N : 1];
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] = a.c[j] != 0.0 ? a.v[j][i] : 0.0;
     double s = base[i];
-    for (int j = 0; j < a.n; ++j) s = s + a.c[j] * a.v[j][i];
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        if (a.c[j] != 0.0) s = s + a.c[j] * v[j];
     out[i] = s;
 }
 
@@ -165,24 +172,35 @@ struct ErrArgs {
 // sumbk = sum b_j k_j, rk_error = sum e_j k_j (left to right, adaptive_runge_kutta.cpp:105-106,168-169),
 // err = max |rk_error / (atol + rtol |max(m, m + sumbk)|)| (:108,171) as per-block maxima.
 // For RKF45 the reference forms rk_error = sumbk - (sum c_j k_j); `e` then holds c_j and
-// rkf_form = 1 selects that shape.
-__global__ void __launch_bounds__(kB) k_rk_error(const double* __restrict__ m, ErrArgs a, int rkf_form,
-                                                double* __restrict__ sumbk, int64_t len, double* __restrict__ part) {
+// RKF selects that shape.  NS is the stage count, so one element's stage loads are issued
+// together; stages whose b and e weights are both zero are not read.
+template <int NS, bool RKF>
+__global__ void __launch_bounds__(kB) k_rk_error(const double* __restrict__ m, ErrArgs a, double* __restrict__ sumbk,
+                                                int64_t len, double* __restrict__ part) {
     double emax = 0;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < len; i += static_cast<int64_t>(gridDim.x) * kB) {
-        double sb = a.b[1] * a.k[1][i];
-        for (int j = 2; j <= a.ns; ++j) sb = sb + a.b[j] * a.k[j][i];
+        double k[NS + 1];
+#pragma unroll
+        for (int j = 1; j <= NS; ++j) k[j] = (a.b[j] != 0.0 || a.e[j] != 0.0) ? a.k[j][i] : 0.0;
+        const double mi = m[i];
+        double sb = a.b[1] * k[1];
+#pragma unroll
+        for (int j = 2; j <= NS; ++j)
+            if (a.b[j] != 0.0) sb = sb + a.b[j] * k[j];
         double re;
-        if (rkf_form) {
-            double t = a.e[1] * a.k[1][i];
-            for (int j = 2; j <= a.ns; ++j) t = t + a.e[j] * a.k[j][i];
+        if constexpr (RKF) {
+            double t = a.e[1] * k[1];
+#pragma unroll
+            for (int j = 2; j <= NS; ++j)
+                if (a.e[j] != 0.0) t = t + a.e[j] * k[j];
             re = sb - t;
         } else {
-            re = a.e[1] * a.k[1][i];
-            for (int j = 2; j <= a.ns; ++j) re = re + a.e[j] * a.k[j][i];
+            re = a.e[1] * k[1];
+#pragma unroll
+            for (int j = 2; j <= NS; ++j)
+                if (a.e[j] != 0.0) re = re + a.e[j] * k[j];
         }
         sumbk[i] = sb;
-        const double mi = m[i];
         const double mm = fmax(mi, mi + sb);
         emax = fmax(emax, fabs(re / (a.atol + a.rtol * fabs(mm))));
     }
@@ -290,8 +308,21 @@ void lincomb(mafx_ctx* ctx, double* out, const double* base, int n, const double
         a.v[j] = v[j];
         a.c[j] = c[j];
     }
-    ProfScope ps(ctx, "rk_stage_f64", 8.0 * len * (n + 2));
-    hipLaunchKernelGGL(k_lincomb, dim3(grid_for(len, kB)), dim3(kB), 0, ctx->stream, out, base, a, len);
+    int nz = 0;
+    for (int j = 0; j < n; ++j) nz += c[j] != 0.0;
+    ProfScope ps(ctx, "rk_stage_f64", 8.0 * len * (nz + 2));
+    const dim3 g(grid_for(len, kB)), b(kB);
+    switch (n) {
+    case 0: hipLaunchKernelGGL(k_lincomb<0>, g, b, 0, ctx->stream, out, base, a, len); break;
+    case 1: hipLaunchKernelGGL(k_lincomb<1>, g, b, 0, ctx->stream, out, base, a, len); break;
+    case 2: hipLaunchKernelGGL(k_lincomb<2>, g, b, 0, ctx->stream, out, base, a, len); break;
+    case 3: hipLaunchKernelGGL(k_lincomb<3>, g, b, 0, ctx->stream, out, base, a, len); break;
+    case 4: hipLaunchKernelGGL(k_lincomb<4>, g, b, 0, ctx->stream, out, base, a, len); break;
+    case 5: hipLaunchKernelGGL(k_lincomb<5>, g, b, 0, ctx->stream, out, base, a, len); break;
+    case 6: hipLaunchKernelGGL(k_lincomb<6>, g, b, 0, ctx->stream, out, base, a, len); break;
+    case 7: hipLaunchKernelGGL(k_lincomb<7>, g, b, 0, ctx->stream, out, base, a, len); break;
+    default: hipLaunchKernelGGL(k_lincomb<8>, g, b, 0, ctx->stream, out, base, a, len); break;
+    }
     MAFX_HIP(hipGetLastError());
 }
 
@@ -314,10 +345,35 @@ double rk_error(mafx_ctx* ctx, const double* m, const double* const* k, const do
     a.rtol = rtol;
     const int64_t len = 3 * ctx->N;
     const unsigned nb = red_blocks(len);
+    require(ns >= 1 && ns <= 7, "rk_error: 1..7 stages");
+    int nread = 0;
+    for (int j = 1; j <= ns; ++j) nread += (b[j] != 0.0 || e[j] != 0.0);
     {
-        ProfScope ps(ctx, "rk_error_f64", 8.0 * len * (ns + 2));
-        hipLaunchKernelGGL(k_rk_error, dim3(nb), dim3(kB), 0, ctx->stream, m, a, rkf_form ? 1 : 0, sumbk, len,
-                           ctx->partials.as<double>());
+        ProfScope ps(ctx, "rk_error_f64", 8.0 * len * (nread + 2));
+        double* part = ctx->partials.as<double>();
+        const dim3 g(nb), bl(kB);
+        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, bl, 0, ctx->stream, m, a, sumbk, len, part); };
+        if (rkf_form) {
+            switch (ns) {
+            case 1: go(k_rk_error<1, true>); break;
+            case 2: go(k_rk_error<2, true>); break;
+            case 3: go(k_rk_error<3, true>); break;
+            case 4: go(k_rk_error<4, true>); break;
+            case 5: go(k_rk_error<5, true>); break;
+            case 6: go(k_rk_error<6, true>); break;
+            default: go(k_rk_error<7, true>); break;
+            }
+        } else {
+            switch (ns) {
+            case 1: go(k_rk_error<1, false>); break;
+            case 2: go(k_rk_error<2, false>); break;
+            case 3: go(k_rk_error<3, false>); break;
+            case 4: go(k_rk_error<4, false>); break;
+            case 5: go(k_rk_error<5, false>); break;
+            case 6: go(k_rk_error<6, false>); break;
+            default: go(k_rk_error<7, false>); break;
+            }
+        }
         hipLaunchKernelGGL(k_final, dim3(1), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(),
                            static_cast<int>(nb), 1, 0, ctx->scal.as<double>());
     }
