@@ -27,6 +27,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
 MU0 = 4e-7 * math.pi
+COPY_CAL_BYTES = 1 << 28  # calibration copy of the profiler runs: 256 MiB read + 256 MiB written
 
 
 def workload(nx, ny, nz, seed=2024):
@@ -50,14 +51,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--heff-only", choices=["f32", "f64"], default=None,
                     help="run --steps H_eff evaluations of one path and exit (for profiler runs)")
+    ap.add_argument("--config", choices=["C1", "C2traj", "C3", "C4", "C5"], default=None,
+                    help="run one secondary configuration and print its JSON line instead of the main one")
+    ap.add_argument("--c4-samples", type=int, default=4096, help="C4: training films (64x64x1)")
+    ap.add_argument("--c4-epochs", type=int, default=20, help="C4: training epochs")
+    ap.add_argument("--c4-eval", type=int, default=4, help="C4: 256x256x1 evaluation films")
+    ap.add_argument("--c4-weights", default=None, help="C4: write the trained parameters here (.npy)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (the sharding design is "
-              "described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
     import magnumaf as M
+
+    if args.config:
+        print(json.dumps(CONFIGS[args.config](M, torch, args)))
+        return
 
     nx, ny, nz = (int(v) for v in args.size.split("x"))
     d = 2e-9
@@ -74,7 +84,14 @@ def main():
         for _ in range(args.warmup + args.steps):
             ctx.heff(state.m_dev, out=h)
         torch.cuda.synchronize()
-        log(f"[bench] {args.steps} {args.heff_only} H_eff evaluations done ({time.time() - t0:.1f} s)")
+        # a plain device-to-device copy of known size: the reference point for HBM byte counters
+        src = torch.ones(COPY_CAL_BYTES // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] {args.steps} {args.heff_only} H_eff evaluations and 3 copies of {COPY_CAL_BYTES} B done "
+            f"({time.time() - t0:.1f} s)")
         return
     ctx32 = state._context(terms, "f32")
     log(f"[bench] FP32 context ready in {time.time() - t0:.1f} s (includes host Newell tensor)")
@@ -125,6 +142,22 @@ def main():
     kt32 = kernel_table(ctx32, 20)
     kt64 = kernel_table(ctx64, 10)
     dom = max(kt32.items(), key=lambda kv: kv[1]["avg_ms"] * kv[1]["launches"])
+    # HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # runs at this workload (profiles/heff_*_1024x1024x4_hbm_traffic.md), when it is the default one
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "heff_1024x1024x4_traffic.json")
+    if (nx, ny, nz) == (1024, 1024, 4) and os.path.exists(tfile):
+        with open(tfile) as f:
+            t = json.load(f).get("f32", {}).get(dom[0])
+        if t:
+            traffic = t["read_bytes"] + t["write_bytes"]
+    # FP32-field H_eff error against the FP64 path (normwise: max|dH| / max|H64|) on the initial state
+    h64 = torch.empty_like(k32)
+    ctx64.heff(state.m_dev, out=h64)
+    ctx32.heff(state.m_dev, out=k32)
+    torch.cuda.synchronize()
+    heff_err32 = ((k32 - h64).abs().max() / h64.abs().max()).item()
+    del h64
     per_heff_ms32 = sum(v["avg_ms"] * v["launches"] for v in kt32.values()) / 20
     bytes_heff32 = sum(v["bytes"] * v["launches"] for v in kt32.values()) / 20
     per_heff_ms64 = sum(v["avg_ms"] * v["launches"] for v in kt64.values()) / 10
@@ -201,13 +234,15 @@ def main():
         "config": {"workload": f"C2 film {nx}x{ny}x{nz} cells of 2 nm, demag+exchange+DMI+uniaxial, alpha=0.1",
                    "cells": N, "path": "FP32-field H_eff"},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": dom[1]["GBps"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom[1]["GBps"] / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": dom[1]["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": dom[1]["bytes"], "avg_ms": dom[1]["avg_ms"]},
         "cpu_baseline": cpu,
         "extra": {
             "heff_per_s_fp32": heff32, "heff_per_s_fp64": heff64,
             "dp45_steps_per_s_fp32": dp32, "dp45_steps_per_s_fp64": dp64,
             "dp45_attempts_per_s_fp32": at32, "dp45_attempts_per_s_fp64": at64,
             "dp45_fp32_over_fp64": dp32 / dp64 if dp64 > 0 else None,
+            "heff_fp32_normwise_error_vs_fp64": heff_err32,
             "heff_algorithmic_GB_fp32": bytes_heff32 / 1e9, "heff_algorithmic_GB_fp64": bytes_heff64 / 1e9,
             "heff_kernel_ms_fp32": per_heff_ms32, "heff_kernel_ms_fp64": per_heff_ms64,
             "heff_frac_of_hbm_peak_fp32": bytes_heff32 / (per_heff_ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -217,6 +252,177 @@ def main():
         },
     }
     print(json.dumps(out))
+
+
+def config_c1(M, torch, args):
+    """128x32x1 film, DP45 (rtol = atol = 1e-6), 1 ns of dense output every 10 ps on the FP64 path,
+    against the oracle on the host cores (the reference's CPU-runnable configuration)."""
+    import oracle_lib as O
+    nx, ny, nz = 128, 32, 1
+    mesh_t = (nx, ny, nz, 4e-9, 4e-9, 3e-9)
+    rng = np.random.default_rng(11)
+    m0 = np.zeros((nx, ny, nz, 3))
+    m0[..., 0] = 1.0
+    m0 += 0.1 * rng.standard_normal(m0.shape)
+    m0 /= np.linalg.norm(m0, axis=-1, keepdims=True)
+    h = 25e-3 / MU0
+    H = (h * math.cos(math.radians(170)), h * math.sin(math.radians(170)), 0.0)
+    mesh = M.Mesh(*mesh_t)
+    state = M.State(mesh, 8e5, m0)
+    llg = M.LLGIntegrator(0.02, [M.DemagField(mesh), M.ExchangeField(1.3e-11), M.ExternalField(homogeneous=H)], "DP45")
+    llg.integrate_dense(state, 1e-11, 1e-11)  # warm-up (context, first steps)
+    state = M.State(mesh, 8e5, m0)
+    llg = M.LLGIntegrator(0.02, llg.terms, "DP45")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rows = llg.integrate_dense(state, 1e-9, 1e-11)
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    steps = llg.stats()["accepted"]
+    log(f"[C1] GPU: {len(rows)} rows, {steps} steps in {t_gpu:.2f} s")
+    threads = int(os.environ.get("OMP_NUM_THREADS", str(os.cpu_count() or 1)))
+    sim = O.Sim(mesh_t, 8e5, m0, [O.term(O.DEMAG), O.term(O.EXCHANGE, 1.3e-11), O.term(O.ZEEMAN, field=H)],
+                nthreads=threads)
+    sim.llg_init(0.02, "DP45")
+    t0 = time.perf_counter()
+    ref = sim.integrate_dense(1e-9, 1e-11)
+    t_cpu = time.perf_counter() - t0
+    dev = float(np.max(np.abs(rows[:, 1:] - ref[:, 1:]))) if rows.shape == ref.shape else None
+    log(f"[C1] oracle: {len(ref)} rows in {t_cpu:.2f} s; max |<m>_gpu - <m>_oracle| = {dev}")
+    return {"config": "C1 128x32x1 DP45 1 ns dense output every 10 ps, FP64 path vs oracle",
+            "gpu_s": t_gpu, "gpu_steps": steps, "gpu_steps_per_s": steps / t_gpu, "rows": len(rows),
+            "max_abs_mean_m_deviation_vs_oracle": dev, "oracle_s": t_cpu, "oracle_threads": threads,
+            "final_mean_m_gpu": rows[-1, 1:].tolist(), "final_mean_m_oracle": ref[-1, 1:].tolist()}
+
+
+def config_c2traj(M, torch, args):
+    """1024x1024x4 film, DP45 for 100 ps on the FP64 path and on the FP32-field path from the same
+    seeded state; <m> every 1 ps through dense output; max deviation of <m> between the paths."""
+    nx, ny, nz, d = 1024, 1024, 4, 2e-9
+    mesh = M.Mesh(nx, ny, nz, d, d, d)
+    m0 = workload(nx, ny, nz)
+    terms = [M.DemagField(mesh), M.ExchangeField(1.5e-11), M.DmiField(3e-3, [0, 0, 1]),
+             M.UniaxialAnisotropyField(8e5, [0, 0, 1])]
+    out = {}
+    rows = {}
+    for prec in ["f64", "f32"]:
+        state = M.State(mesh, 1.1e6, m0)
+        llg = M.LLGIntegrator(0.1, terms, "DP45", precision=prec)
+        t0 = time.perf_counter()
+        chunks = []
+        for c in range(10):  # 10 chunks of 10 ps, dense output every 1 ps
+            r = llg.integrate_dense(state, 1e-11, 1e-12)
+            chunks.append(r if c == 0 else r[1:])
+            log(f"[C2traj] {prec}: t = {state.t:.3e} s, steps {llg.stats()['accepted']}, "
+                f"{time.perf_counter() - t0:.1f} s")
+        torch.cuda.synchronize()
+        rows[prec] = np.concatenate(chunks)
+        st = llg.stats()
+        out[prec] = {"wall_s": time.perf_counter() - t0, "steps": st["accepted"], "rejected": st["rejected"],
+                     "heff_evals": st["heff_evals"], "final_mean_m": rows[prec][-1, 1:].tolist()}
+    a, b = rows["f64"], rows["f32"]
+    n = min(len(a), len(b))
+    dev = float(np.max(np.abs(a[:n, 1:] - b[:n, 1:])))
+    tdev = float(np.max(np.abs(a[:n, 0] - b[:n, 0])))
+    log(f"[C2traj] max_t |<m>_f32 - <m>_f64|_inf = {dev:.3e} over {n} samples (sample-time mismatch {tdev:.1e} s)")
+    return {"config": "C2 1024x1024x4 DP45 100 ps, FP32-field vs FP64 path, <m> every 1 ps",
+            "max_abs_mean_m_deviation": dev, "samples": n, "paths": out}
+
+
+def config_c3(M, torch, args):
+    """Atomistic 512x512x1 spins, LBFGS from seeded random spins until max|m x H| / max|H| <= 1e-6."""
+    nx, ny, nz, a = 512, 512, 1, 0.27e-9
+    J = 1.6e-21
+    mus = 2.6 * 9.274009994e-24
+    mesh = M.Mesh(nx, ny, nz, a, a, a)
+    m0 = workload(nx, ny, nz, seed=3)
+    terms = [M.AtomisticExchangeField(J), M.AtomisticUniaxialAnisotropyField(0.02 * J, (0, 0, 1)),
+             M.AtomisticDmiField(0.2 * J, (0, 0, -1)), M.AtomisticDipoleDipoleField(mesh),
+             M.AtomisticExternalField(homogeneous=(0, 0, 1.0 / MU0))]
+    res = {}
+    for prec in ["f64"]:
+        state = M.State(mesh, mus, m0)
+        mini = M.LBFGS_Minimizer(terms, tol=1e-20, maxiter=200000, torque_tol=1e-6, precision=prec)
+        state._context(terms, prec)  # tensor setup outside the timed region
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E = mini.minimize(state)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[prec] = dict(mini.last_stats, wall_s=dt, E=E)
+        log(f"[C3] {prec}: {dt:.2f} s, {mini.last_stats}")
+    return {"config": "C3 atomistic 512x512x1 LBFGS to max|m x H|/max|H| <= 1e-6 (DMI axis (0,0,-1))",
+            "lbfgs": res}
+
+
+def config_c5(M, torch, args):
+    """2048x1024x8 multilayer (RKKY between layers), H_eff/s of both paths and DP45 steps/s (FP32 field)."""
+    nx, ny, nz, d = 2048, 1024, 8, 2e-9
+    mesh = M.Mesh(nx, ny, nz, d, d, d)
+    m0 = workload(nx, ny, nz, seed=5)
+    rk = np.full((nx, ny, nz), -1e-3 * d / 2.0)
+    ex = np.full((nx, ny, nz), 1.3e-11)
+    terms = [M.DemagField(mesh), M.RKKYExchangeField(rk, ex, mesh), M.DmiField(2.5e-3, [0, 0, 1]),
+             M.UniaxialAnisotropyField(7e5, [0, 0, 1]), M.ExternalField(homogeneous=(50e-3 / MU0, 0, 0))]
+    state = M.State(mesh, 1.0e6, m0)
+    out = {}
+    for prec in ["f32", "f64"]:
+        t0 = time.time()
+        ctx = state._context(terms, prec)
+        log(f"[C5] {prec} context in {time.time() - t0:.1f} s")
+        h = torch.empty(3 * mesh.n, dtype=torch.float64, device="cuda")
+        for _ in range(3):
+            ctx.heff(state.m_dev, out=h)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            ctx.heff(state.m_dev, out=h)
+        torch.cuda.synchronize()
+        out[f"heff_per_s_{prec}"] = 20 / (time.perf_counter() - t0)
+        log(f"[C5] {prec} H_eff/s {out[f'heff_per_s_{prec}']:.1f}")
+    st = M.State(mesh, 1.0e6, m0)
+    st._ctx = state._ctx
+    llg = M.LLGIntegrator(0.05, terms, "DP45", precision="f32")
+    for _ in range(2):
+        llg.step(st)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(6):
+        llg.step(st)
+    torch.cuda.synchronize()
+    out["dp45_steps_per_s_fp32"] = 6 / (time.perf_counter() - t0)
+    for key, c in state._ctx.items():
+        out[f"device_bytes_{key[1]}_context"] = c.device_bytes()
+    log(f"[C5] DP45 FP32-field steps/s {out['dp45_steps_per_s_fp32']:.2f}")
+    return {"config": "C5 2048x1024x8 RKKY multilayer, demag+RKKY+DMI+uniaxial+50 mT", **out}
+
+
+def config_c4(M, torch, args):
+    """Learned warm start: FP64 LBFGS relaxations of seeded 64x64x1 films as targets, training of
+    the encoder-decoder on the GPU, then warm start + LBFGS vs plain LBFGS at 256x256x1."""
+    import magnumaf.warmstart as WS
+    t0 = time.perf_counter()
+    data = WS.generate_dataset(args.c4_samples, seed=1, size=64, torque_tol=1e-4, log=log)
+    t_data = time.perf_counter() - t0
+    its = [mm["iterations"] for mm in data["meta"]]
+    log(f"[C4] dataset: {args.c4_samples} films in {t_data:.1f} s, LBFGS iterations median {np.median(its):.0f}")
+    net = WS.WarmStartNet()
+    t0 = time.perf_counter()
+    hist, hold = WS.train(net, data, epochs=args.c4_epochs, batch=32, lr=1e-3, log=log)
+    t_train = time.perf_counter() - t0
+    if args.c4_weights:
+        net.save(args.c4_weights)
+    ev = WS.evaluate(net, n=args.c4_eval, size=256, log=log)
+    sp = [e["speedup"] for e in ev]
+    same = [e["E_rel_diff"] <= 1e-9 for e in ev]
+    return {"config": "C4 learned warm start: 64x64x1 training films, 256x256x1 evaluation, torque <= 1e-6",
+            "samples": args.c4_samples, "dataset_s": t_data, "train_s": t_train, "epochs": args.c4_epochs,
+            "train_loss": hist, "heldout_loss": hold, "eval": ev,
+            "speedup_median": float(np.median(sp)), "speedup_geomean": float(np.exp(np.mean(np.log(sp)))),
+            "equal_final_energy_count": int(sum(same)), "lbfgs_iterations_median_dataset": float(np.median(its))}
+
+
+CONFIGS = {"C1": config_c1, "C2traj": config_c2traj, "C3": config_c3, "C4": config_c4, "C5": config_c5}
 
 
 if __name__ == "__main__":

@@ -191,6 +191,35 @@ typedef struct mafx_lbfgs_stats {
 int mafx_lbfgs_minimize(mafx_ctx* ctx, double* m_dev, double tolerance, int64_t max_iter,
                         int verbose, double torque_tol, double* E_out, mafx_lbfgs_stats* stats);
 
+/* ------------------------------------------------------------------ learned warm start */
+/* No counterpart in the reference (magnum.af has no learned component).  A 4-level
+ * convolutional encoder-decoder (3x3 convolutions 8->32->64->128->128 and back with skip
+ * concatenation, BF16 weights, FP32 activations and accumulation) predicts a relaxed
+ * magnetisation from m0 plus five parameter planes; LBFGS (mafx_lbfgs_minimize) then refines
+ * it.  Layouts: inputs x [B][8][H][W] and outputs / targets [B][3][H][W], f32 device arrays, x
+ * fastest; H and W multiples of 8.  Parameters: per layer W [Cout][Cin][3][3] then b [Cout],
+ * concatenated in layer order (FP32 masters; the kernels use their BF16 rounding). */
+typedef struct mafx_net mafx_net;
+/* stream: a hipStream_t or NULL for the null stream.  Weights start He-normal (fixed seed). */
+int mafx_net_create(void* stream, mafx_net** out);
+int mafx_net_destroy(mafx_net* net);
+int mafx_net_param_count(const mafx_net* net, int64_t* n);
+int mafx_net_set_params(mafx_net* net, const float* host);
+int mafx_net_get_params(const mafx_net* net, float* host);
+/* m_dev [B][3][H][W] <- unit-vector prediction */
+int mafx_net_forward(mafx_net* net, const float* x_dev, int B, int H, int W, float* m_dev);
+/* loss = sum |m - target|^2 / (B H W); gradients (same layout as the parameters) to grad_host */
+int mafx_net_gradients(mafx_net* net, const float* x_dev, const float* target_dev, int B, int H, int W,
+                       float* grad_host, double* loss);
+/* forward + backward + one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with learning rate lr */
+int mafx_net_train_step(mafx_net* net, const float* x_dev, const float* target_dev, int B, int H, int W,
+                        float lr, double* loss);
+/* Warm start for a film (nz = 1, nx and ny multiples of 8): m_out_dev (SoA f64 [3][ny][nx],
+ * renormalised in FP64) from m0_dev (same layout) and the five normalised parameter values
+ * broadcast as input planes. */
+int mafx_net_warm_start(mafx_net* net, const mafx_mesh* mesh, const double* m0_dev, const float planes[5],
+                        double* m_out_dev);
+
 /* ------------------------------------------------------------------ profiling */
 /* When enabled, every kernel launch is bracketed by HIP events on the context stream and
  * its duration accumulated per kernel name. */

@@ -21,7 +21,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", 
              "-packed-fp32-ops", "-Wall", "-Wno-unused-result", "-ffp-contract=fast"]
 GXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra", "-pthread"]
 
-HIP_SOURCES = ["fft.hip", "field.hip", "vec.hip", "llg.cpp", "lbfgs.cpp", "capi.cpp"]
+HIP_SOURCES = ["fft.hip", "field.hip", "vec.hip", "net.hip", "llg.cpp", "lbfgs.cpp", "capi.cpp"]
 GXX_SOURCES = ["newell_host.cpp"]
 
 

@@ -9,6 +9,7 @@
 #include "fft_kernels.hpp"
 #include "mafx_internal.hpp"
 #include "mafx_llg.hpp"
+#include "net.hpp"
 #include "vec_kernels.hpp"
 
 namespace mafx {
@@ -400,6 +401,83 @@ int mafx_lbfgs_minimize(mafx_ctx* ctx, double* m_dev, double tolerance, int64_t 
     int rc = MAFX_OK;
     const int g = guarded([&] { rc = lbfgs_minimize(ctx, m_dev, tolerance, max_iter, verbose, torque_tol, E_out, stats); });
     return g != MAFX_OK ? g : rc;
+}
+
+int mafx_net_create(void* stream, mafx_net** out) {
+    return guarded([&] {
+        require(out != nullptr, "mafx_net_create: null output");
+        int n = 0;
+        MAFX_HIP(hipGetDeviceCount(&n));
+        require(n > 0, "no GPU");
+        *out = reinterpret_cast<mafx_net*>(new Net(static_cast<hipStream_t>(stream)));
+    });
+}
+
+int mafx_net_destroy(mafx_net* net) {
+    return guarded([&] { delete reinterpret_cast<Net*>(net); });
+}
+
+int mafx_net_param_count(const mafx_net* net, int64_t* n) {
+    return guarded([&] { *n = reinterpret_cast<const Net*>(net)->nparams; });
+}
+
+int mafx_net_set_params(mafx_net* net, const float* host) {
+    return guarded([&] { reinterpret_cast<Net*>(net)->set_params(host); });
+}
+
+int mafx_net_get_params(const mafx_net* net, float* host) {
+    return guarded([&] { reinterpret_cast<const Net*>(net)->get_params(host); });
+}
+
+int mafx_net_forward(mafx_net* net_, const float* x_dev, int B, int H, int W, float* m_dev) {
+    return guarded([&] {
+        Net* net = reinterpret_cast<Net*>(net_);
+        net->ensure_shape(B, H, W);
+        net->forward(x_dev);
+        MAFX_HIP(hipMemcpyAsync(m_dev, net->output(), sizeof(float) * 3 * static_cast<int64_t>(B) * H * W,
+                                hipMemcpyDeviceToDevice, net->stream));
+        MAFX_HIP(hipStreamSynchronize(net->stream));
+    });
+}
+
+int mafx_net_gradients(mafx_net* net_, const float* x_dev, const float* target_dev, int B, int H, int W,
+                       float* grad_host, double* loss) {
+    return guarded([&] {
+        Net* net = reinterpret_cast<Net*>(net_);
+        net->ensure_shape(B, H, W);
+        net->forward(x_dev);
+        *loss = net->backward(target_dev);
+        MAFX_HIP(hipMemcpyAsync(grad_host, net->grad.p, sizeof(float) * net->nparams, hipMemcpyDeviceToHost,
+                                net->stream));
+        MAFX_HIP(hipStreamSynchronize(net->stream));
+    });
+}
+
+int mafx_net_train_step(mafx_net* net_, const float* x_dev, const float* target_dev, int B, int H, int W, float lr,
+                        double* loss) {
+    return guarded([&] {
+        Net* net = reinterpret_cast<Net*>(net_);
+        net->ensure_shape(B, H, W);
+        net->forward(x_dev);
+        *loss = net->backward(target_dev);
+        net->adam(lr, 0.9f, 0.999f, 1e-8f);
+    });
+}
+
+int mafx_net_warm_start(mafx_net* net_, const mafx_mesh* mesh, const double* m0_dev, const float planes[5],
+                        double* m_out_dev) {
+    return guarded([&] {
+        Net* net = reinterpret_cast<Net*>(net_);
+        require(mesh->nz == 1, "warm start: films only (nz = 1)");
+        const int nx = static_cast<int>(mesh->nx), ny = static_cast<int>(mesh->ny);
+        net->ensure_shape(1, ny, nx);
+        DevBuf x;
+        x.alloc(sizeof(float) * kNetInputs * static_cast<int64_t>(nx) * ny);
+        net_pack_input(net, m0_dev, nx, ny, planes, x.as<float>());
+        net->forward(x.as<float>());
+        net_unpack_output(net, net->output(), nx, ny, m_out_dev);
+        MAFX_HIP(hipStreamSynchronize(net->stream));
+    });
 }
 
 int mafx_prof_enable(mafx_ctx* ctx, int on) {

@@ -121,8 +121,15 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
         s_cell[threadIdx.x] = cell0;
         s_dst[threadIdx.x] = dst;
     }
+    C* tws = buf + lds_padded(W * M); // stage twiddles, staged next to the data
+    for (int t = threadIdx.x; t < M; t += kAxisBlock) tws[t] = twh[t];
     __syncthreads();
-    for (int e = threadIdx.x; e < M * W; e += kAxisBlock) {
+    // fixed trip count (ELEMS bounds M W), so a lane's loads are all issued before the first use
+    constexpr int IT = (ELEMS + kAxisBlock - 1) / kAxisBlock;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = static_cast<int>(threadIdx.x) + i * kAxisBlock;
+        if (e >= M * W) break;
         const int w = static_cast<int>(fdiv(e, Md)), n = e - w * M;
         const int64_t so = s_src[w];
         T re = 0, im = 0;
@@ -141,7 +148,7 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
         buf[lds_pad(n * W + w)] = C{re, im};
     }
     __syncthreads();
-    fft_lds<C, ELEMS, POW2, kAxisBlock>(buf, dh, Wd, twh, false);
+    fft_lds<C, ELEMS, POW2, kAxisBlock>(buf, dh, Wd, tws, false);
     const T half = static_cast<T>(0.5);
     for (int e = threadIdx.x; e < (M + 1) * W; e += kAxisBlock) {
         const int w = static_cast<int>(fdiv(e, M1d)), k = e - w * (M + 1);
@@ -182,14 +189,20 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
         s_src[threadIdx.x] = src;
         s_dst[threadIdx.x] = dst;
     }
+    C* tws = buf + lds_padded(W * M); // stage twiddles, staged next to the data
+    for (int t = threadIdx.x; t < M; t += kAxisBlock) tws[t] = twh[t];
     __syncthreads();
-    for (int e = threadIdx.x; e < M * W; e += kAxisBlock) {
+    constexpr int IT = (ELEMS + kAxisBlock - 1) / kAxisBlock;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = static_cast<int>(threadIdx.x) + i * kAxisBlock;
+        if (e >= M * W) break;
         const int w = static_cast<int>(fdiv(e, Md)), k = e - w * M;
         const int64_t so = s_src[w];
         buf[lds_pad(k * W + w)] = so >= 0 ? c2r_pack(spec + so, k, M, twL) : C{0, 0};
     }
     __syncthreads();
-    fft_lds<C, ELEMS, POW2, kAxisBlock>(buf, dh, Wd, twh, true);
+    fft_lds<C, ELEMS, POW2, kAxisBlock>(buf, dh, Wd, tws, true);
     const int Mh = static_cast<int>(Mhd.d);
     for (int e = threadIdx.x; e < Mh * W; e += kAxisBlock) {
         const int w = static_cast<int>(fdiv(e, Mhd)), n = e - w * Mh;
@@ -231,93 +244,97 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? kMinWaves : 1)
     }
 }
 
-// Forward transform of zero-padded strided lines (inputs pos < M, L = 2 M outputs) as two
-// length-M transforms: X[2k] = FFT_M(x)[k], X[2k+1] = FFT_M(x[n] w_L^n)[k].  The inputs are kept
-// in registers between the halves, so rows may be overwritten in place.
-template <class C, bool POW2, int BS, int ELEMS, int MINW = kMinWaves>
+// Both halves of the zero-padded y transform in one LDS pass.  The work-group's W columns become
+// 2W lines of one length-M transform: line w holds x[n] (its outputs are X[2k]) and line W + w
+// holds x[n] w_L^n (outputs X[2k+1]), so the stage sequence, and its barriers, run once per
+// column pair instead of twice.  With TWL the stage twiddles (M entries) are staged in LDS behind
+// the data; the inputs are kept in registers only between their loads and the LDS writes.
+template <class C, bool POW2, int BS, int ELEMS, int MINW, bool TWL>
 __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
-    k_fft_y_fwd_split(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, FastDiv Wd, FftDesc dh,
-                      const C* __restrict__ twh, const C* __restrict__ twL) {
-    constexpr int MAXE = (ELEMS + BS - 1) / BS;
+    k_fft_y_fwd2(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, FastDiv Wd, FastDiv W2d,
+                 FftDesc dh, const C* __restrict__ twh, const C* __restrict__ twL) {
     extern __shared__ unsigned char smem_raw[];
     C* buf = reinterpret_cast<C*>(smem_raw);
     const int M = dh.L;
-    const int W = static_cast<int>(Wd.d);
+    const int W = static_cast<int>(Wd.d), W2 = 2 * W;
+    C* tws = buf + lds_padded(W2 * M);
+    if constexpr (TWL)
+        for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
+    const C* tw = TWL ? tws : twh;
     const uint3 blk = swizzled_block();
     const int64_t kx0 = static_cast<int64_t>(blk.x) * W;
     C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + kx0;
     const int wv = static_cast<int>(hx - kx0 < W ? hx - kx0 : W);
-    C v[MAXE];
+    // all of a lane's loads are issued before the first use (IT is a compile-time bound)
+    constexpr int IT = (ELEMS / 2 + BS - 1) / BS;
+    C v[IT], t[IT];
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i) {
+    for (int i = 0; i < IT; ++i) {
         const int e = threadIdx.x + i * BS;
         if (e < M * W) {
             const int n = static_cast<int>(fdiv(e, Wd)), w = e - n * W;
             v[i] = w < wv ? base[n * stride + w] : C{0, 0};
-            buf[lds_pad(e)] = v[i];
+            t[i] = twL[n];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * BS;
+        if (e < M * W) {
+            const int n = static_cast<int>(fdiv(e, Wd)), w = e - n * W;
+            buf[lds_pad(n * W2 + w)] = v[i];
+            buf[lds_pad(n * W2 + W + w)] = cmul(v[i], t[i]);
         }
     }
     __syncthreads();
-    fft_lds<C, ELEMS, POW2, BS>(buf, dh, Wd, twh, false);
-    for (int e = threadIdx.x; e < M * W; e += BS) {
-        const int k = static_cast<int>(fdiv(e, Wd)), w = e - k * W;
-        if (w < wv) base[(2 * k) * stride + w] = buf[lds_pad(e)];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < MAXE; ++i) {
-        const int e = threadIdx.x + i * BS;
-        if (e < M * W) buf[lds_pad(e)] = cmul(v[i], twL[fdiv(e, Wd)]);
-    }
-    __syncthreads();
-    fft_lds<C, ELEMS, POW2, BS>(buf, dh, Wd, twh, false);
-    for (int e = threadIdx.x; e < M * W; e += BS) {
-        const int k = static_cast<int>(fdiv(e, Wd)), w = e - k * W;
-        if (w < wv) base[(2 * k + 1) * stride + w] = buf[lds_pad(e)];
+    fft_lds<C, ELEMS, POW2, BS>(buf, dh, W2d, tw, false);
+    for (int e = threadIdx.x; e < M * W2; e += BS) {
+        const int k = static_cast<int>(fdiv(e, W2d)), l = e - k * W2;
+        const int odd = l >= W ? 1 : 0, w = l - odd * W;
+        if (w < wv) base[(2 * k + odd) * stride + w] = buf[lds_pad(e)];
     }
 }
 
-// Inverse transform of strided lines of length L = 2 M keeping outputs n < M:
-// x[n] = IFFT_M(X_even)[n] + w_L^-n IFFT_M(X_odd)[n].
-template <class C, bool POW2, int BS, int ELEMS, int MINW = kMinWaves>
+// Inverse counterpart: lines w and W + w take X[2k] and X[2k+1] of column w; after one inverse
+// LDS pass x[n] = IFFT_M(X_even)[n] + w_L^-n IFFT_M(X_odd)[n] is written for n < M.
+template <class C, bool POW2, int BS, int ELEMS, int MINW, bool TWL>
 __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
-    k_fft_y_inv_split(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, FastDiv Wd, FftDesc dh,
-                      const C* __restrict__ twh, const C* __restrict__ twL) {
-    constexpr int MAXE = (ELEMS + BS - 1) / BS;
+    k_fft_y_inv2(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, FastDiv Wd, FastDiv W2d,
+                 FftDesc dh, const C* __restrict__ twh, const C* __restrict__ twL) {
     extern __shared__ unsigned char smem_raw[];
     C* buf = reinterpret_cast<C*>(smem_raw);
     const int M = dh.L;
-    const int W = static_cast<int>(Wd.d);
+    const int W = static_cast<int>(Wd.d), W2 = 2 * W;
+    C* tws = buf + lds_padded(W2 * M);
+    if constexpr (TWL)
+        for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
+    const C* tw = TWL ? tws : twh;
     const uint3 blk = swizzled_block();
     const int64_t kx0 = static_cast<int64_t>(blk.x) * W;
     C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + kx0;
     const int wv = static_cast<int>(hx - kx0 < W ? hx - kx0 : W);
-    for (int e = threadIdx.x; e < M * W; e += BS) {
-        const int k = static_cast<int>(fdiv(e, Wd)), w = e - k * W;
-        buf[lds_pad(e)] = w < wv ? base[(2 * k) * stride + w] : C{0, 0};
-    }
-    __syncthreads();
-    fft_lds<C, ELEMS, POW2, BS>(buf, dh, Wd, twh, true);
-    C v[MAXE];
+    constexpr int IT = (ELEMS + BS - 1) / BS;
+    C v[IT];
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i) {
+    for (int i = 0; i < IT; ++i) {
         const int e = threadIdx.x + i * BS;
-        if (e < M * W) v[i] = buf[lds_pad(e)];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < M * W; e += BS) {
-        const int k = static_cast<int>(fdiv(e, Wd)), w = e - k * W;
-        buf[lds_pad(e)] = w < wv ? base[(2 * k + 1) * stride + w] : C{0, 0};
-    }
-    __syncthreads();
-    fft_lds<C, ELEMS, POW2, BS>(buf, dh, Wd, twh, true);
-#pragma unroll
-    for (int i = 0; i < MAXE; ++i) {
-        const int e = threadIdx.x + i * BS;
-        if (e < M * W) {
-            const int n = static_cast<int>(fdiv(e, Wd)), w = e - n * W;
-            if (w < wv) base[n * stride + w] = cadd(v[i], cmul(conj_if(twL[n], true), buf[lds_pad(e)]));
+        if (e < M * W2) {
+            const int k = static_cast<int>(fdiv(e, W2d)), l = e - k * W2;
+            const int odd = l >= W ? 1 : 0, w = l - odd * W;
+            v[i] = w < wv ? base[(2 * k + odd) * stride + w] : C{0, 0};
         }
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * BS;
+        if (e < M * W2) buf[lds_pad(e)] = v[i];
+    }
+    __syncthreads();
+    fft_lds<C, ELEMS, POW2, BS>(buf, dh, W2d, tw, true);
+    for (int e = threadIdx.x; e < M * W; e += BS) {
+        const int n = static_cast<int>(fdiv(e, Wd)), w = e - n * W;
+        if (w < wv)
+            base[n * stride + w] = cadd(buf[lds_pad(n * W2 + w)], cmul(conj_if(twL[n], true), buf[lds_pad(n * W2 + W + w)]));
     }
 }
 
@@ -546,7 +563,7 @@ void fft_x_forward(mafx_ctx* ctx, const double* in, const double* ms, double sca
     const C* twL = tw_of<C>(ctx->px);
     auto fn = is_pow2_plan(d) ? k_fft_x_fwd<C, true> : k_fft_x_fwd<C, false>;
     if (small) fn = k_fft_x_fwd<C, true, E64 / 2, 6>;
-    const size_t lds = lds_padded(W * M) * sizeof(C);
+    const size_t lds = (lds_padded(W * M) + M) * sizeof(C);
     allow_lds(reinterpret_cast<const void*>(fn), lds);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kAxisBlock), lds, ctx->stream, in, ms, scale, static_cast<int>(Xi), Yi, Zi,
                        make_fastdiv(static_cast<uint32_t>(Yr)), make_fastdiv(static_cast<uint32_t>(Zr)),
@@ -572,7 +589,7 @@ void fft_x_inverse(mafx_ctx* ctx, const C* spec, int64_t n2s, T* out, double hsc
     ProfScope ps(ctx, nm("fft_x_inv_f32", "fft_x_inv_f64", sizeof(C)), bytes);
     auto fn = is_pow2_plan(d) ? k_fft_x_inv<C, T, true> : k_fft_x_inv<C, T, false>;
     if (small) fn = k_fft_x_inv<C, T, true, E64 / 2, 6>;
-    const size_t lds = lds_padded(W * M) * sizeof(C);
+    const size_t lds = (lds_padded(W * M) + M) * sizeof(C);
     allow_lds(reinterpret_cast<const void*>(fn), lds);
     const int Mh = static_cast<int>((g.nx + 1) / 2);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kAxisBlock), lds, ctx->stream, spec, ctx->hxs, ctx->n1, n2s,
@@ -605,39 +622,44 @@ template <class C>
 void fft_y_split(mafx_ctx* ctx, C* data, int64_t sa, int na, int64_t sb, int nb, int64_t stride, bool inv) {
     const FftPlan& ph = ctx->pyh;
     const int M = ph.L;
-    require(M <= lds_elems<C>(), "y transform length exceeds the LDS budget");
     require(nb <= 65535 && na <= 65535, "y transform grid too large");
     const FftDesc d = desc_of(ph);
-    // Tile shapes, measured at 1024 x 1024 x 4: power-of-two FP32 plans run best with a 32 KiB
-    // LDS tile per 512-lane work-group and a 6-wave register cap (three resident work-groups per
-    // CU, 8 elements per lane, no spills); FP64 with a 64 KiB tile and the 4-wave cap (two
-    // work-groups).  Lines longer than the small tile and other plans use the 64 KiB shape.
     constexpr int E64 = lds_elems<C>();
     constexpr bool kF32 = sizeof(C) == sizeof(float2);
-    constexpr int ET = kF32 ? E64 / 2 : E64;
-    constexpr int MW = kF32 ? 6 : kMinWaves;
-    void (*fn)(C*, int64_t, int64_t, int64_t, int64_t, FastDiv, FftDesc, const C*, const C*) = nullptr;
-    int elems = E64;
-    if (is_pow2_plan(d) && M <= ET) {
-        elems = ET;
-        fn = inv ? k_fft_y_inv_split<C, true, kAxisBlock, ET, MW> : k_fft_y_fwd_split<C, true, kAxisBlock, ET, MW>;
+    constexpr size_t kLdsMax = 160 * 1024;
+    // Tile shapes, measured at 1024 x 1024 x 4 (one column pair = 2 lines): FP32 power-of-two
+    // lines of up to E64/2 run 1024-lane work-groups on a 64 KiB tile under an 8-wave register cap
+    // (two groups per CU); FP64 ones 1024-lane groups on a 128 KiB tile; both stage the twiddles in
+    // LDS.  Longer or non-power-of-two lines use 512-lane groups, a 128 KiB tile and global twiddles.
+    using KT = void (*)(C*, int64_t, int64_t, int64_t, int64_t, FastDiv, FastDiv, FftDesc, const C*, const C*);
+    KT fn = nullptr;
+    int bs = 512, elems = 2 * E64;
+    bool twl = false;
+    if (is_pow2_plan(d) && M <= E64 / 2) {
+        constexpr int EL = kF32 ? E64 : 2 * E64, MW = kF32 ? 8 : 4;
+        bs = 1024;
+        twl = true;
+        elems = EL;
+        fn = inv ? k_fft_y_inv2<C, true, 1024, EL, MW, true> : k_fft_y_fwd2<C, true, 1024, EL, MW, true>;
     } else if (is_pow2_plan(d)) {
-        fn = inv ? k_fft_y_inv_split<C, true, kAxisBlock, E64> : k_fft_y_fwd_split<C, true, kAxisBlock, E64>;
+        fn = inv ? k_fft_y_inv2<C, true, 512, 2 * E64, 1, false> : k_fft_y_fwd2<C, true, 512, 2 * E64, 1, false>;
     } else {
-        fn = inv ? k_fft_y_inv_split<C, false, kAxisBlock, E64> : k_fft_y_fwd_split<C, false, kAxisBlock, E64>;
+        fn = inv ? k_fft_y_inv2<C, false, 512, 2 * E64, 1, false> : k_fft_y_fwd2<C, false, 512, 2 * E64, 1, false>;
     }
-    require(M <= elems, "y transform length exceeds the LDS budget");
-    const int W = lines_per_block(M, elems);
+    // lines per work-group: even, within the tile and, with the twiddles, within the LDS
+    auto lds_of = [&](int w2) { return (lds_padded(w2 * M) + (twl ? M : 0)) * sizeof(C); };
+    int W2 = std::min(elems / M, 64) & ~1;
+    while (W2 >= 2 && lds_of(W2) > kLdsMax) W2 -= 2;
+    require(W2 >= 2, "y transform length exceeds the LDS budget");
+    const int W = W2 / 2;
     const unsigned gx = static_cast<unsigned>((ctx->hx + W - 1) / W);
     const double bytes = static_cast<double>(na) * nb * ctx->hx * (3.0 * M) * sizeof(C);
-    const C* twL = tw_of<C>(ctx->py);
     ProfScope ps(ctx, inv ? nm("fft_y_inv_f32", "fft_y_inv_f64", sizeof(C)) : nm("fft_y_fwd_f32", "fft_y_fwd_f64", sizeof(C)),
                  bytes);
-    const dim3 g(gx, nb, na), b(kAxisBlock);
-    const size_t lds = lds_padded(W * M) * sizeof(C);
+    const size_t lds = lds_of(W2);
     allow_lds(reinterpret_cast<const void*>(fn), lds);
-    hipLaunchKernelGGL(fn, g, b, lds, ctx->stream, data, sa, sb, stride, ctx->hx, make_fastdiv(W), d, tw_of<C>(ph),
-                       twL);
+    hipLaunchKernelGGL(fn, dim3(gx, nb, na), dim3(bs), lds, ctx->stream, data, sa, sb, stride, ctx->hx, make_fastdiv(W),
+                       make_fastdiv(W2), d, tw_of<C>(ph), tw_of<C>(ctx->py));
     MAFX_HIP(hipGetLastError());
 }
 

@@ -50,6 +50,7 @@ struct EpiParams {
     double rk_pref[3];       // mu0 * d^2 of the RKKY couplings: dx^2 for x AND y, dz^2 for z
     int nterms;
     int fft_slot; // term index fed by the FFT convolution, -1 if none
+    int needs_nb; // some selected term reads the six neighbours (loaded with the centre cell)
     EpiTerm t[kMaxTerms];
     int mode;
     void* out;
@@ -123,6 +124,9 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
             }
         }
     };
+    // the neighbour loads are issued with the centre cell's, not behind the first stencil term,
+    // so a cell costs one memory round trip
+    if (P.needs_nb) load_nb();
     const double msi = P.ms ? P.ms[i] : P.Ms;
     T H0 = 0, H1 = 0, H2 = 0;
     for (int t = 0; t < P.nterms; ++t) {
@@ -479,6 +483,9 @@ EpiParams make_params(mafx_ctx* ctx, const EvalArgs& a, int64_t nblocks) {
         default: break;
         }
         if (static_cast<int>(k) == ctx->demag_term) P.fft_slot = P.nterms;
+        if (td.kind == MAFX_EXCHANGE || td.kind == MAFX_DMI || td.kind == MAFX_ATOM_DMI ||
+            td.kind == MAFX_ATOM_EXCHANGE || td.kind == MAFX_RKKY)
+            P.needs_nb = 1;
         P.nterms++;
     }
     P.mode = static_cast<int>(a.mode);
@@ -634,16 +641,112 @@ std::shared_ptr<std::vector<double>> host_tensor(const mafx_mesh& g, bool dipole
 }
 } // namespace
 
-// Demag / dipole tensor: real-space tensor on the host in the reference's arithmetic, then a
-// full FP64 R2C transform on the GPU with the same kernels, stored as the complex half spectrum.
+// ---- real-space tensors on the GPU (FP32 field path) ---------------------------------------
+// The same formulas as newell_host.cpp / demag_field.cpp:20-121 evaluated in FP64 on the device.
+// The device libm rounds differently from glibc, so the far-field cancellation noise differs from
+// the reference's in its last bits; the FP32 path (whose own rounding is ~1e-7) uses this, the FP64
+// path keeps the bit-exact host tensor.
+namespace {
+
+__device__ double nf_dev(double x, double y, double z) { // demag_field.cpp:20-40
+    x = fabs(x);
+    y = fabs(y);
+    z = fabs(z);
+    const double xx = x * x, yy = y * y, zz = z * z;
+    const double R = sqrt(xx + yy + zz);
+    double r = 1.0 / 6.0 * (2.0 * xx - yy - zz) * R;
+    if (xx + zz > 0) r += y / 2.0 * (zz - xx) * asinh(y / sqrt(xx + zz));
+    if (xx + yy > 0) r += z / 2.0 * (yy - xx) * asinh(z / sqrt(xx + yy));
+    if (x * R > 0) r += -x * y * z * atan(y * z / (x * R));
+    return r;
+}
+
+__device__ double ng_dev(double x, double y, double z) { // demag_field.cpp:42-69
+    z = fabs(z);
+    const double xx = x * x, yy = y * y, zz = z * z;
+    const double R = sqrt(xx + yy + zz);
+    double r = -x * y * R / 3.0;
+    if (xx + yy > 0) r += x * y * z * asinh(z / sqrt(xx + yy));
+    if (yy + zz > 0) r += y / 6.0 * (3.0 * zz - yy) * asinh(x / sqrt(yy + zz));
+    if (xx + zz > 0) r += x / 6.0 * (3.0 * zz - xx) * asinh(y / sqrt(xx + zz));
+    if (z * R > 0) r += -zz * z / 6.0 * atan(x * y / (z * R));
+    if (y * R != 0) r += -z * yy / 2.0 * atan(x * z / (y * R));
+    if (x * R != 0) r += -z * xx / 2.0 * atan(y * z / (x * R));
+    return r;
+}
+
+// 27-point second difference in the order of demag_field.cpp:75-83
+template <bool G>
+__device__ double second_diff_dev(int ix, int iy, int iz, double dx, double dy, double dz) {
+    const double x = dx * ix, y = dy * iy, z = dz * iz;
+    const double xs[3] = {x, x + dx, x - dx}, ys[3] = {y, y + dy, y - dy}, zs[3] = {z, z + dz, z - dz};
+    auto F = [&](int a, int b, int c) { return G ? ng_dev(xs[a], ys[b], zs[c]) : nf_dev(xs[a], ys[b], zs[c]); };
+    double s = 8.0 * F(0, 0, 0);
+    s = s - 4.0 * F(1, 0, 0) - 4.0 * F(2, 0, 0) - 4.0 * F(0, 1, 0) - 4.0 * F(0, 2, 0) - 4.0 * F(0, 0, 1) -
+        4.0 * F(0, 0, 2);
+    s = s + 2.0 * F(1, 1, 0) + 2.0 * F(1, 2, 0) + 2.0 * F(2, 1, 0) + 2.0 * F(2, 2, 0) + 2.0 * F(1, 0, 1) +
+        2.0 * F(1, 0, 2) + 2.0 * F(2, 0, 1) + 2.0 * F(2, 0, 2) + 2.0 * F(0, 1, 1) + 2.0 * F(0, 1, 2) +
+        2.0 * F(0, 2, 1) + 2.0 * F(0, 2, 2);
+    s = s - F(1, 1, 1) - F(1, 1, 2) - F(1, 2, 1) - F(1, 2, 2) - F(2, 1, 1) - F(2, 1, 2) - F(2, 2, 1) - F(2, 2, 2);
+    return -s / (4.0 * 3.14159265358979323846 * dx * dy * dz);
+}
+
+// [6][n2][n1][n0] with the wrapped index j = (i + n/2) % n - n/2 (demag_field.cpp:104-121); one
+// thread per position.  dipole: atomistic_dipole_dipole_field.cpp:55-113 instead.
+__global__ void __launch_bounds__(256)
+    k_real_tensor(double* __restrict__ N, int64_t n0, int64_t n1, int64_t n2, double dx, double dy, double dz,
+                  int dipole) {
+    const int64_t P = n0 * n1 * n2;
+    const int64_t at = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (at >= P) return;
+    const int64_t i0 = at % n0, r = at / n0, i1 = r % n1, i2 = r / n1;
+    const int j0 = static_cast<int>((i0 + n0 / 2) % n0 - n0 / 2);
+    const int j1 = static_cast<int>((i1 + n1 / 2) % n1 - n1 / 2);
+    const int j2 = static_cast<int>((i2 + n2 / 2) % n2 - n2 / 2);
+    if (!dipole) {
+        N[0 * P + at] = second_diff_dev<false>(j0, j1, j2, dx, dy, dz);
+        N[1 * P + at] = second_diff_dev<true>(j0, j1, j2, dx, dy, dz);
+        N[2 * P + at] = second_diff_dev<true>(j0, j2, j1, dx, dz, dy);
+        N[3 * P + at] = second_diff_dev<false>(j1, j2, j0, dy, dz, dx);
+        N[4 * P + at] = second_diff_dev<true>(j1, j2, j0, dy, dz, dx);
+        N[5 * P + at] = second_diff_dev<false>(j2, j0, j1, dz, dx, dy);
+        return;
+    }
+    const double rx = j0 * dx, ry = j1 * dy, rz = j2 * dz;
+    const double rr = sqrt(rx * rx + ry * ry + rz * rz);
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (rr != 0.0) {
+        const double r3 = rr * rr * rr, r5 = r3 * rr * rr;
+        v[0] = 3.0 * rx * rx / r5 - 1.0 / r3;
+        v[1] = 3.0 * rx * ry / r5;
+        v[2] = 3.0 * rx * rz / r5;
+        v[3] = 3.0 * ry * ry / r5 - 1.0 / r3;
+        v[4] = 3.0 * ry * rz / r5;
+        v[5] = 3.0 * rz * rz / r5 - 1.0 / r3;
+    }
+    const double s = 1.0 / (4.0 * 3.14159265358979323846);
+    for (int c = 0; c < 6; ++c) N[c * P + at] = v[c] * s;
+}
+
+} // namespace
+
+// Demag / dipole tensor: the real-space tensor (FP64 path: on the host in the reference's
+// arithmetic; FP32 path: on the GPU), then a full FP64 R2C transform on the GPU with the same
+// kernels, stored as the complex half spectrum (FP64) or its real symmetric quarter (FP32).
 void build_tensor(mafx_ctx* ctx) {
     const int64_t n0 = ctx->n0, n1 = ctx->n1, n2 = ctx->n2, hxs = ctx->hxs;
     const int64_t P = n0 * n1 * n2, K = n2 * n1 * hxs;
-    auto Nh = host_tensor(ctx->mesh, ctx->atom_dipole, P);
     DevBuf Nd, T64;
     Nd.alloc(sizeof(double) * 6 * P);
-    MAFX_HIP(hipMemcpy(Nd.p, Nh->data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
-    Nh.reset();
+    if (ctx->precision == MAFX_F64) {
+        auto Nh = host_tensor(ctx->mesh, ctx->atom_dipole, P);
+        MAFX_HIP(hipMemcpy(Nd.p, Nh->data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
+    } else {
+        const mafx_mesh& g = ctx->mesh;
+        hipLaunchKernelGGL(k_real_tensor, dim3(grid_for(P, 256)), dim3(256), 0, ctx->stream, Nd.as<double>(), n0, n1,
+                           n2, g.dx, g.dy, g.dz, ctx->atom_dipole ? 1 : 0);
+        MAFX_HIP(hipGetLastError());
+    }
     ctx->n1h = n1 / 2 + 1;
     ctx->n2h = n2 / 2 + 1;
     if (ctx->precision != MAFX_F64) zero_unpaired_planes(ctx, Nd.as<double>(), 6);

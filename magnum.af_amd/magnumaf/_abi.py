@@ -17,7 +17,9 @@ EXPORTS = [
     "mafx_dot", "mafx_max_abs", "mafx_normalize", "mafx_mean_m", "mafx_llg_create", "mafx_llg_destroy",
     "mafx_llg_set_alpha", "mafx_llg_step", "mafx_llg_run_until", "mafx_llg_integrate_dense", "mafx_llg_get_stats",
     "mafx_llg_set_controller_state", "mafx_llg_get_controller_state", "mafx_lbfgs_minimize", "mafx_prof_enable",
-    "mafx_prof_reset", "mafx_prof_report",
+    "mafx_prof_reset", "mafx_prof_report", "mafx_net_create", "mafx_net_destroy", "mafx_net_param_count",
+    "mafx_net_set_params", "mafx_net_get_params", "mafx_net_forward", "mafx_net_gradients", "mafx_net_train_step",
+    "mafx_net_warm_start",
 ]
 
 
@@ -86,6 +88,16 @@ def lib():
     L.mafx_prof_enable.argtypes = [vp, C.c_int]
     L.mafx_prof_reset.argtypes = [vp]
     L.mafx_prof_report.argtypes = [vp, C.c_char_p, i64]
+    fp = C.POINTER(C.c_float)
+    L.mafx_net_create.argtypes = [vp, C.POINTER(vp)]
+    L.mafx_net_destroy.argtypes = [vp]
+    L.mafx_net_param_count.argtypes = [vp, C.POINTER(i64)]
+    L.mafx_net_set_params.argtypes = [vp, fp]
+    L.mafx_net_get_params.argtypes = [vp, fp]
+    L.mafx_net_forward.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, vp]
+    L.mafx_net_gradients.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, fp, dp]
+    L.mafx_net_train_step.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, dp]
+    L.mafx_net_warm_start.argtypes = [vp, C.POINTER(Mesh), vp, fp, vp]
     _lib = L
     return L
 

@@ -160,6 +160,9 @@ def test_demag_cube_energy_gpu():
     E = M.DemagField(g).Energy_in_J(M.State(g, 1e5, m))
     analytic = 1.0 / 6.0 * x * y * z * 1e5**2 * MU0
     assert abs(E - analytic) <= (E + analytic) / 2 * 5.3e-8
+    # the FP32 path (tensor built on the GPU, symmetric real spectrum, FP32 transforms)
+    E32 = M.DemagField(g).Energy_in_J(M.State(g, 1e5, m), precision="f32")
+    assert abs(E32 - analytic) <= analytic * 1e-5, (E32, analytic)
 
 
 # FP32 field path: nz = 1 (k-space product), 2 <= nz <= 8 (fused z stage), nz > 8 (strided z
