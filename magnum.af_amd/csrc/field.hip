@@ -65,7 +65,15 @@ struct EpiParams {
 
 namespace {
 
-__device__ __forceinline__ double ld(const double* m, int64_t i) { return m[i]; }
+// Cell arrays are read and written as (uniform plane base, 32-bit byte offset): the address
+// then lives in one VGPR next to an SGPR base (global_load/store with saddr) instead of a
+// 64-bit VGPR pair per access.  Valid while one component plane is below 4 GiB (N < 2^29).
+template <class U> __device__ __forceinline__ U ldb(const U* base, uint32_t boff) {
+    return *reinterpret_cast<const U*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <class U> __device__ __forceinline__ void stb(U* base, uint32_t boff, U v) {
+    *reinterpret_cast<U*>(reinterpret_cast<char*>(base) + boff) = v;
+}
 
 template <class T> struct V3 {
     T x, y, z;
@@ -102,31 +110,34 @@ template <class T, bool RED>
 __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t z, const T hd[3], double* acc) {
     const int64_t N = P.N;
     const int64_t i = x + P.nx * (y + P.ny * z);
-    const double* m = P.m;
-    const double mc0 = ld(m, i), mc1 = ld(m, N + i), mc2 = ld(m, 2 * N + i);
+    const double* mp[3] = {P.m, P.m + N, P.m + 2 * N};
+    const uint32_t o8 = static_cast<uint32_t>(i) * 8u;
+    const double mc0 = ldb(mp[0], o8), mc1 = ldb(mp[1], o8), mc2 = ldb(mp[2], o8);
     const T m0 = static_cast<T>(mc0), m1 = static_cast<T>(mc1), m2 = static_cast<T>(mc2);
 
-    // neighbours with zero padding (af::convolve AF_CONV_DEFAULT semantics)
-    V3<T> nb[6]; // x-, x+, y-, y+, z-, z+
-    bool have_nb = false;
-    auto load_nb = [&]() {
-        if (have_nb) return;
-        have_nb = true;
-        const int64_t off[6] = {-1, 1, -P.nx, P.nx, -P.nx * P.ny, P.nx * P.ny};
+    // Neighbours with zero padding (af::convolve AF_CONV_DEFAULT semantics), x-, x+, y-, y+,
+    // z-, z+.  A missing neighbour reads the centre cell and is then replaced by 0, so all 18
+    // loads are issued together, without branches, with the centre cell's: one memory round
+    // trip per cell.  P.needs_nb is set whenever a selected term reads nb.
+    V3<T> nb[6];
+    if (P.needs_nb) {
         const bool ok[6] = {x > 0, x + 1 < P.nx, y > 0, y + 1 < P.ny, z > 0, z + 1 < P.nz};
+        const uint32_t sy = static_cast<uint32_t>(P.nx) * 8u, sz = static_cast<uint32_t>(P.nx * P.ny) * 8u;
+        const uint32_t jo[6] = {ok[0] ? o8 - 8u : o8, ok[1] ? o8 + 8u : o8, ok[2] ? o8 - sy : o8,
+                                ok[3] ? o8 + sy : o8, ok[4] ? o8 - sz : o8, ok[5] ? o8 + sz : o8};
+        double v[6][3];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            if (ok[k]) {
-                const int64_t j = i + off[k];
-                nb[k] = V3<T>{static_cast<T>(ld(m, j)), static_cast<T>(ld(m, N + j)), static_cast<T>(ld(m, 2 * N + j))};
-            } else {
-                nb[k] = V3<T>{0, 0, 0};
-            }
-        }
-    };
-    // the neighbour loads are issued with the centre cell's, not behind the first stencil term,
-    // so a cell costs one memory round trip
-    if (P.needs_nb) load_nb();
+        for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[k][c] = ldb(mp[c], jo[k]);
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            nb[k] = ok[k] ? V3<T>{static_cast<T>(v[k][0]), static_cast<T>(v[k][1]), static_cast<T>(v[k][2])}
+                          : V3<T>{0, 0, 0};
+    } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nb[k] = V3<T>{0, 0, 0};
+    }
     const double msi = P.ms ? P.ms[i] : P.Ms;
     T H0 = 0, H1 = 0, H2 = 0;
     for (int t = 0; t < P.nterms; ++t) {
@@ -151,7 +162,6 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
             h2 = hd[2];
             break;
         case MAFX_EXCHANGE: {
-            load_nb();
             T c;
             if (micro_coef(c)) {
                 const T wx = static_cast<T>(P.ex_w[0]), wy = static_cast<T>(P.ex_w[1]), wz = static_cast<T>(P.ex_w[2]);
@@ -179,7 +189,6 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
         }
         case MAFX_DMI:
         case MAFX_ATOM_DMI: {
-            load_nb();
             T c;
             bool ok;
             T cx, cy, cz;
@@ -240,7 +249,6 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
             }
             break;
         case MAFX_ATOM_EXCHANGE: {
-            load_nb();
             const T c = static_cast<T>(tm.c0);
             h0 = c * (nb[0].x + nb[1].x + nb[2].x + nb[3].x + nb[4].x + nb[5].x);
             h1 = c * (nb[0].y + nb[1].y + nb[2].y + nb[3].y + nb[4].y + nb[5].y);
@@ -249,7 +257,6 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
         }
         case MAFX_RKKY: {
             // rkky_exchange_field.cpp:195-337 entries, applied as a stencil, then / Ms (:19-29)
-            load_nb();
             const int64_t sxy = P.nx * P.ny;
             const double Ai = tm.aex[i];
             double a0 = 0, a1 = 0, a2 = 0;
@@ -318,14 +325,15 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
     case static_cast<int>(OutMode::H):
         if (P.out_f32) {
             float* o = static_cast<float*>(P.out);
-            o[i] = static_cast<float>(H0);
-            o[N + i] = static_cast<float>(H1);
-            o[2 * N + i] = static_cast<float>(H2);
+            const uint32_t o4 = static_cast<uint32_t>(i) * 4u;
+            stb(o, o4, static_cast<float>(H0));
+            stb(o + N, o4, static_cast<float>(H1));
+            stb(o + 2 * N, o4, static_cast<float>(H2));
         } else {
             double* o = static_cast<double*>(P.out);
-            o[i] = Hx;
-            o[N + i] = Hy;
-            o[2 * N + i] = Hz;
+            stb(o, o8, Hx);
+            stb(o + N, o8, Hy);
+            stb(o + 2 * N, o8, Hz);
         }
         break;
     case static_cast<int>(OutMode::LLG): {
@@ -337,22 +345,22 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
         const double cd = -P.alpha * kGamma / (1. + a2p);
         double* o = static_cast<double*>(P.out);
         if (P.diss_only) {
-            o[i] = P.dt * (cd * b0);
-            o[N + i] = P.dt * (cd * b1);
-            o[2 * N + i] = P.dt * (cd * b2);
+            stb(o, o8, P.dt * (cd * b0));
+            stb(o + N, o8, P.dt * (cd * b1));
+            stb(o + 2 * N, o8, P.dt * (cd * b2));
         } else {
-            o[i] = P.dt * (cp * a0 + cd * b0);
-            o[N + i] = P.dt * (cp * a1 + cd * b1);
-            o[2 * N + i] = P.dt * (cp * a2 + cd * b2);
+            stb(o, o8, P.dt * (cp * a0 + cd * b0));
+            stb(o + N, o8, P.dt * (cp * a1 + cd * b1));
+            stb(o + 2 * N, o8, P.dt * (cp * a2 + cd * b2));
         }
         break;
     }
     case static_cast<int>(OutMode::GRAD): {
         const double a0 = mc1 * Hz - mc2 * Hy, a1 = mc2 * Hx - mc0 * Hz, a2 = mc0 * Hy - mc1 * Hx;
         double* o = static_cast<double*>(P.out);
-        o[i] = mc1 * a2 - mc2 * a1;
-        o[N + i] = mc2 * a0 - mc0 * a2;
-        o[2 * N + i] = mc0 * a1 - mc1 * a0;
+        stb(o, o8, mc1 * a2 - mc2 * a1);
+        stb(o + N, o8, mc2 * a0 - mc0 * a2);
+        stb(o + 2 * N, o8, mc0 * a1 - mc1 * a0);
         if constexpr (RED) {
             acc[P.nterms] = fmax(acc[P.nterms], fmax(fabs(a0), fmax(fabs(a1), fabs(a2))));
             acc[P.nterms + 1] = fmax(acc[P.nterms + 1], fmax(fabs(Hx), fmax(fabs(Hy), fabs(Hz))));
@@ -376,9 +384,11 @@ template <bool RED> __device__ void finish_block(const EpiParams& P, double* acc
 // ROWS maps work-group b (after the XCD renumbering) to a tile of kBlock cells of row
 // b / bpr in the z-fastest row order (row = y nz + z), so the y and z stencil neighbours of a
 // tile are read behind the same L2; otherwise cells are taken in memory order.
-template <class T, bool RED, bool ROWS>
-__global__ void __launch_bounds__(kBlock) k_epi_cells(EpiParams P, FastDiv nxd, FastDiv nyd, FastDiv nzd,
-                                                      FastDiv bprd) {
+// MINW is the waves per SIMD the register budget must allow: 5 (96 VGPRs, no spills) for the
+// FP32 field and LLG evaluations, which run 7 times per RK attempt; 1 (uncapped) otherwise.
+template <class T, bool RED, bool ROWS, int MINW>
+__global__ void __launch_bounds__(kBlock, MINW) k_epi_cells(EpiParams P, FastDiv nxd, FastDiv nyd, FastDiv nzd,
+                                                            FastDiv bprd) {
     double acc[kMaxTerms + 2];
     for (int s = 0; s < kMaxTerms + 2; ++s) acc[s] = 0;
     const int nx = static_cast<int>(P.nx), ny = static_cast<int>(P.ny), nz = static_cast<int>(P.nz);
@@ -403,10 +413,10 @@ __global__ void __launch_bounds__(kBlock) k_epi_cells(EpiParams P, FastDiv nxd, 
         T hd[3] = {0, 0, 0};
         if (P.hd) {
             const T* h = static_cast<const T*>(P.hd);
-            const int64_t i = x + P.nx * (y + P.ny * static_cast<int64_t>(z));
-            hd[0] = h[i];
-            hd[1] = h[P.N + i];
-            hd[2] = h[2 * P.N + i];
+            const uint32_t ob = static_cast<uint32_t>(x + nx * (y + ny * z)) * static_cast<uint32_t>(sizeof(T));
+            hd[0] = ldb(h, ob);
+            hd[1] = ldb(h + P.N, ob);
+            hd[2] = ldb(h + 2 * P.N, ob);
         }
         cell_epilogue<T, RED>(P, x, y, z, hd, acc);
     }
@@ -503,8 +513,9 @@ EpiParams make_params(mafx_ctx* ctx, const EvalArgs& a, int64_t nblocks) {
 template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a) {
     const mafx_mesh& g = ctx->mesh;
     const bool fft = ctx->demag_term >= 0 && (a.mask & (1u << ctx->demag_term));
-    // cell and row indices are 32-bit inside the kernels
-    require(ctx->N < (int64_t{1} << 31), "meshes of 2^31 cells or more are not supported");
+    // cell and row indices are 32-bit inside the kernels, and the per-cell kernel addresses a
+    // component plane with 32-bit byte offsets (8 N < 2^32)
+    require(ctx->N < (int64_t{1} << 29), "meshes of 2^29 cells or more are not supported");
     const int64_t nrows = g.ny * g.nz;
     // per-cell kernel: row tiles when rows are at least one work-group long
     const bool rowmap = g.nx >= kBlock;
@@ -559,8 +570,9 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
                                : (sizeof(T) == 4 ? "local_cells_f32" : "local_cells_f64"),
                      static_cast<double>(ctx->N) * (24.0 + field_bytes + hd_bytes));
         const bool red = P.want_energy || P.mode == static_cast<int>(OutMode::GRAD);
-        auto kern = red ? (rowmap ? k_epi_cells<T, true, true> : k_epi_cells<T, true, false>)
-                        : (rowmap ? k_epi_cells<T, false, true> : k_epi_cells<T, false, false>);
+        constexpr int MW = sizeof(T) == 4 ? 5 : 1;
+        auto kern = red ? (rowmap ? k_epi_cells<T, true, true, 1> : k_epi_cells<T, true, false, 1>)
+                        : (rowmap ? k_epi_cells<T, false, true, MW> : k_epi_cells<T, false, false, MW>);
         hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nblocks)), dim3(kBlock), 0, ctx->stream, P,
                            make_fastdiv(static_cast<uint32_t>(g.nx)), make_fastdiv(static_cast<uint32_t>(g.ny)),
                            make_fastdiv(static_cast<uint32_t>(g.nz)), make_fastdiv(static_cast<uint32_t>(bpr)));

@@ -526,7 +526,8 @@ double Net::backward(const float* target) {
     MAFX_HIP(hipGetLastError());
     MAFX_HIP(hipMemcpyAsync(host_scal, scal.p, sizeof(double), hipMemcpyDeviceToHost, stream));
     MAFX_HIP(hipStreamSynchronize(stream));
-    return host_scal[0];
+    // the partials hold sum |m - target|^2; the loss is its mean over the B H W cells
+    return host_scal[0] / static_cast<double>(f);
 }
 
 void Net::adam(float lr, float beta1, float beta2, float eps) {
