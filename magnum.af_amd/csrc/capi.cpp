@@ -212,7 +212,7 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
             ctx->n1 = 2 * mesh->ny;
             ctx->n2 = mesh->nz == 1 ? 1 : 2 * mesh->nz;
             ctx->hx = ctx->n0 / 2 + 1;
-            ctx->hxs = (ctx->hx + 15) / 16 * 16; // rows start on 128 B (f32) / 256 B (f64) boundaries
+            ctx->hxs = (ctx->hx + 15) / 16 * 16; // whole column tiles; planes start on 512 B boundaries
             ctx->px.build(static_cast<int>(ctx->n0));  // twiddles of the length-2 nx packing
             ctx->pxh.build(static_cast<int>(mesh->nx)); // the half-length x transform
             ctx->py.build(static_cast<int>(ctx->n1));

@@ -2,10 +2,11 @@
 // (replaces af::fftR2C / af::fftC2R in demag_field.cpp:229-257 and
 // atomistic_dipole_dipole_field.cpp:20-51).
 //
-// Spectrum layout: [comp][kz][ky][kx] complex with kx in [0, hx), hx = n0/2 + 1 (R2C half
-// spectrum along x) and a row stride hxs >= hx padded to 16 elements; ky in [0, n1); kz in
-// [0, nzs) where nzs = nz when the z transform is fused into the tensor product (2 <= nz <= 8)
-// and nzs = n2 otherwise.  Only rows that can hold non-zero data are touched: the forward x pass
+// Spectrum layout: planes [comp][kz] of n1 hxs complex values, kx in [0, hx), hx = n0/2 + 1 (R2C
+// half spectrum along x), hxs >= hx padded to 16 columns; ky in [0, n1); kz in [0, nzs) where
+// nzs = nz when the z transform is fused into the tensor product (2 <= nz <= 8) and nzs = n2
+// otherwise.  Inside a plane the columns are grouped in tiles of 4 (tcol, fft_device.hpp):
+// (ky, kx) sits at tcol(kx, 4 n1) + 4 ky.  The tensor spectra use the same planes.  Only rows that can hold non-zero data are touched: the forward x pass
 // writes rows y < ny, z < nz; the forward y pass reads y < ny and writes every ky for z < nz; the
 // fused z kernel reads and writes z < nz; the inverse passes stop at the cropped extent.
 //
@@ -91,8 +92,9 @@ constexpr int kMinWaves = 4;
 // Forward real-to-complex transform along x of rows (c, z, y): padded length L = 2 M, input
 // x[t] = in[((c Zi + z) Yi + y) Xi + t] * scale * (ms ? ms[cell] : 1) for t < Xi, 0 beyond.
 // Z = FFT_M(x[2n] + i x[2n+1]);  X[k] = E + w_L^k O,  E = (Z[k] + conj Z[M-k]) / 2,
-// O = -i (Z[k] - conj Z[M-k]) / 2, k = 0..M, written to out[((c n2o + z) n1 + y) hxs + k].
-// Lanes walk along rows (consecutive lanes, consecutive addresses) for the global accesses.  The
+// O = -i (Z[k] - conj Z[M-k]) / 2, k = 0..M, written to out[(c n2o + z) n1 hxs + 4 y + tcol(k)]
+// (column tiles).  The input lanes walk along rows; the output lanes take the 4 columns of a
+// tile for each of the block's rows in turn, so consecutive rows y give contiguous blocks.  The
 // (c, z, y) decomposition of each of the block's W <= 64 rows is done once, into LDS.
 template <class C, bool POW2, int ELEMS = lds_elems<C>(), int MINW = kMinWaves>
 __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
@@ -115,7 +117,7 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
             const uint32_t c = fdiv(zc, Zr), z = zc - c * Zr.d;
             cell0 = (z * Yi + y) * Xi;
             src = c * Zi * Yi * Xi + cell0;
-            dst = ((c * n2o + z) * n1 + y) * hxs;
+            dst = (c * n2o + z) * n1 * hxs + kTile * static_cast<int64_t>(y);
         }
         s_src[threadIdx.x] = src;
         s_cell[threadIdx.x] = cell0;
@@ -150,20 +152,26 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
     __syncthreads();
     fft_lds<C, ELEMS, POW2, kAxisBlock>(buf, dh, Wd, tws, false);
     const T half = static_cast<T>(0.5);
-    for (int e = threadIdx.x; e < (M + 1) * W; e += kAxisBlock) {
-        const int w = static_cast<int>(fdiv(e, M1d)), k = e - w * (M + 1);
+    const int64_t n1T = kTile * n1;
+    // e = (kt W + w) 4 + kk: column k = 4 kt + kk of row w
+    const int ntile = (M + kTile) / kTile;
+    for (int e = threadIdx.x; e < ntile * W * kTile; e += kAxisBlock) {
+        const int kk = e & (kTile - 1), r = e >> 2;
+        const int kt = static_cast<int>(fdiv(r, Wd)), w = r - kt * W;
+        const int k = kt * kTile + kk;
         const int64_t dst = s_dst[w];
-        if (dst < 0) continue;
+        if (dst < 0 || k > M) continue;
         const C Zk = buf[lds_pad((k == M ? 0 : k) * W + w)];
         const C Zm = buf[lds_pad((k == 0 ? 0 : M - k) * W + w)];
         const C E{half * (Zk.x + Zm.x), half * (Zk.y - Zm.y)};
         const C O{half * (Zk.y + Zm.y), -half * (Zk.x - Zm.x)};
-        out[dst + k] = cadd(E, cmul(twL[k], O));
+        out[dst + tcol(k, n1T)] = cadd(E, cmul(twL[k], O));
     }
 }
 
 // Inverse real transform along x of spectrum rows (c, z, y), c < 3, z < Zr, y < Yr, read at
-// spec[((c n2s + z) n1 + y) hxs + k]: C2R through the half-length packing (c2r_pack), an inverse
+// spec[(c n2s + z) n1 hxs + 4 y + tcol(k)] (column tiles, the input lanes taking the 4 columns of
+// a tile for each of the block's rows in turn): C2R through the half-length packing, an inverse
 // FFT of length M = n0 / 2, the crop to x < Xo and the 1 / P scale,
 // out[c No + (z Yr + y) Xo + x] = T(hscale * x_real[x]).
 template <class C, class T, bool POW2, int ELEMS = lds_elems<C>(), int MINW = kMinWaves>
@@ -183,7 +191,7 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
         if (r < rows) {
             const uint32_t zc = fdiv(r, Yr), y = r - zc * Yr.d;
             const uint32_t c = fdiv(zc, Zr), z = zc - c * Zr.d;
-            src = ((c * n2s + z) * n1 + y) * hxs;
+            src = (c * n2s + z) * n1 * hxs + kTile * static_cast<int64_t>(y);
             dst = c * No + (static_cast<int64_t>(z) * Yr.d + y) * Xo;
         }
         s_src[threadIdx.x] = src;
@@ -192,14 +200,21 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
     C* tws = buf + lds_padded(W * M); // stage twiddles, staged next to the data
     for (int t = threadIdx.x; t < M; t += kAxisBlock) tws[t] = twh[t];
     __syncthreads();
-    constexpr int IT = (ELEMS + kAxisBlock - 1) / kAxisBlock;
+    // e = (kt W + w) 4 + kk: column k = 4 kt + kk of row w; rounding the M columns up to whole
+    // tiles adds at most 3 W <= 192 elements to the M W <= ELEMS of the block
+    constexpr int IT = (ELEMS + 3 * 64 + kAxisBlock - 1) / kAxisBlock;
+    const int64_t n1T = kTile * n1;
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
         const int e = static_cast<int>(threadIdx.x) + i * kAxisBlock;
-        if (e >= M * W) break;
-        const int w = static_cast<int>(fdiv(e, Md)), k = e - w * M;
+        if (e >= ((M + kTile - 1) / kTile) * W * kTile) break;
+        const int kk = e & (kTile - 1), r = e >> 2;
+        const int kt = static_cast<int>(fdiv(r, Wd)), w = r - kt * W;
+        const int k = kt * kTile + kk;
         const int64_t so = s_src[w];
-        buf[lds_pad(k * W + w)] = so >= 0 ? c2r_pack(spec + so, k, M, twL) : C{0, 0};
+        if (k < M)
+            buf[lds_pad(k * W + w)] =
+                so >= 0 ? c2r_combine(spec[so + tcol(k, n1T)], spec[so + tcol(M - k, n1T)], twL[k]) : C{0, 0};
     }
     __syncthreads();
     fft_lds<C, ELEMS, POW2, kAxisBlock>(buf, dh, Wd, tws, true);
@@ -216,31 +231,32 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
 }
 
 // Complex transform of strided lines: line (tile, b, a) covers columns kx0 .. kx0+W-1 and holds
-// data[a sa + b sb + kx + pos * stride].  Inputs pos >= n_in are taken as zero; outputs pos >= n_out
-// are not written.
+// data[a sa + b sb + tcol(kx, n1T) + pos * stride] (column tiles: for lines along y, b is the
+// plane and stride = 4; for lines along z, b sb = 4 ky and stride is the plane).  Inputs
+// pos >= n_in are taken as zero; outputs pos >= n_out are not written.
 template <class C, bool POW2>
 __global__ void __launch_bounds__(kAxisBlock, POW2 ? kMinWaves : 1)
     k_fft_strided(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, int n_in, int n_out,
-                  FastDiv Wd, FftDesc d, const C* __restrict__ tw, int inv) {
+                  FastDiv Wd, FftDesc d, const C* __restrict__ tw, int inv, int64_t n1T) {
     extern __shared__ unsigned char smem_raw[];
     C* buf = reinterpret_cast<C*>(smem_raw);
     const int L = d.L;
     const int W = static_cast<int>(Wd.d);
     const uint3 blk = swizzled_block();
     const int64_t kx0 = static_cast<int64_t>(blk.x) * W;
-    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + kx0;
+    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb;
     const int wv = static_cast<int>(hx - kx0 < W ? hx - kx0 : W);
     for (int e = threadIdx.x; e < L * W; e += kAxisBlock) {
         const int pos = static_cast<int>(fdiv(e, Wd)), w = e - pos * W;
         C v{0, 0};
-        if (w < wv && pos < n_in) v = base[pos * stride + w];
+        if (w < wv && pos < n_in) v = base[pos * stride + tcol(kx0 + w, n1T)];
         buf[lds_pad(pos * W + w)] = v;
     }
     __syncthreads();
     fft_lds<C, lds_elems<C>(), POW2, kAxisBlock>(buf, d, Wd, tw, inv != 0);
     for (int e = threadIdx.x; e < n_out * W; e += kAxisBlock) {
         const int pos = static_cast<int>(fdiv(e, Wd)), w = e - pos * W;
-        if (w < wv) base[pos * stride + w] = buf[lds_pad(pos * W + w)];
+        if (w < wv) base[pos * stride + tcol(kx0 + w, n1T)] = buf[lds_pad(pos * W + w)];
     }
 }
 
@@ -252,7 +268,7 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? kMinWaves : 1)
 template <class C, bool POW2, int BS, int ELEMS, int MINW, bool TWL>
 __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
     k_fft_y_fwd2(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, FastDiv Wd, FastDiv W2d,
-                 FftDesc dh, const C* __restrict__ twh, const C* __restrict__ twL) {
+                 FftDesc dh, const C* __restrict__ twh, const C* __restrict__ twL, int64_t n1T) {
     extern __shared__ unsigned char smem_raw[];
     C* buf = reinterpret_cast<C*>(smem_raw);
     const int M = dh.L;
@@ -263,7 +279,7 @@ __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
     const C* tw = TWL ? tws : twh;
     const uint3 blk = swizzled_block();
     const int64_t kx0 = static_cast<int64_t>(blk.x) * W;
-    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + kx0;
+    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb;
     const int wv = static_cast<int>(hx - kx0 < W ? hx - kx0 : W);
     // all of a lane's loads are issued before the first use (IT is a compile-time bound)
     constexpr int IT = (ELEMS / 2 + BS - 1) / BS;
@@ -273,7 +289,7 @@ __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
         const int e = threadIdx.x + i * BS;
         if (e < M * W) {
             const int n = static_cast<int>(fdiv(e, Wd)), w = e - n * W;
-            v[i] = w < wv ? base[n * stride + w] : C{0, 0};
+            v[i] = w < wv ? base[n * stride + tcol(kx0 + w, n1T)] : C{0, 0};
             t[i] = twL[n];
         }
     }
@@ -291,7 +307,7 @@ __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
     for (int e = threadIdx.x; e < M * W2; e += BS) {
         const int k = static_cast<int>(fdiv(e, W2d)), l = e - k * W2;
         const int odd = l >= W ? 1 : 0, w = l - odd * W;
-        if (w < wv) base[(2 * k + odd) * stride + w] = buf[lds_pad(e)];
+        if (w < wv) base[(2 * k + odd) * stride + tcol(kx0 + w, n1T)] = buf[lds_pad(e)];
     }
 }
 
@@ -300,7 +316,7 @@ __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
 template <class C, bool POW2, int BS, int ELEMS, int MINW, bool TWL>
 __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
     k_fft_y_inv2(C* __restrict__ data, int64_t sa, int64_t sb, int64_t stride, int64_t hx, FastDiv Wd, FastDiv W2d,
-                 FftDesc dh, const C* __restrict__ twh, const C* __restrict__ twL) {
+                 FftDesc dh, const C* __restrict__ twh, const C* __restrict__ twL, int64_t n1T) {
     extern __shared__ unsigned char smem_raw[];
     C* buf = reinterpret_cast<C*>(smem_raw);
     const int M = dh.L;
@@ -311,7 +327,7 @@ __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
     const C* tw = TWL ? tws : twh;
     const uint3 blk = swizzled_block();
     const int64_t kx0 = static_cast<int64_t>(blk.x) * W;
-    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + kx0;
+    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb;
     const int wv = static_cast<int>(hx - kx0 < W ? hx - kx0 : W);
     constexpr int IT = (ELEMS + BS - 1) / BS;
     C v[IT];
@@ -321,7 +337,7 @@ __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
         if (e < M * W2) {
             const int k = static_cast<int>(fdiv(e, W2d)), l = e - k * W2;
             const int odd = l >= W ? 1 : 0, w = l - odd * W;
-            v[i] = w < wv ? base[(2 * k + odd) * stride + w] : C{0, 0};
+            v[i] = w < wv ? base[(2 * k + odd) * stride + tcol(kx0 + w, n1T)] : C{0, 0};
         }
     }
 #pragma unroll
@@ -334,7 +350,8 @@ __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
     for (int e = threadIdx.x; e < M * W; e += BS) {
         const int n = static_cast<int>(fdiv(e, Wd)), w = e - n * W;
         if (w < wv)
-            base[n * stride + w] = cadd(buf[lds_pad(n * W2 + w)], cmul(conj_if(twL[n], true), buf[lds_pad(n * W2 + W + w)]));
+            base[n * stride + tcol(kx0 + w, n1T)] =
+                cadd(buf[lds_pad(n * W2 + w)], cmul(conj_if(twL[n], true), buf[lds_pad(n * W2 + W + w)]));
     }
 }
 
@@ -425,60 +442,71 @@ __global__ void k_zero_unpaired(double* __restrict__ N, int64_t n0, int64_t n1, 
     }
 }
 
+// Ts holds, per component and kz <= n2/2, a plane of n1h = n1/2 + 1 rows ky in column tiles:
+// column kx of row kys at tcol(kx, 4 n1h) + 4 kys.  T is the transformed tensor in the spectrum
+// layout (planes of n1 rows).
 __global__ void k_tensor_sym_f32(const double2* __restrict__ T, float* __restrict__ Ts, int64_t n1, int64_t n2,
                                  int64_t hx, int64_t hxs, int64_t n1h, int64_t n2h, int64_t total) {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= total) return;
-    const int64_t kx = e % hxs, r = e / hxs, ky = r % n1h, r2 = r / n1h, kz = r2 % n2h, c = r2 / n2h;
-    Ts[e] = kx < hx ? static_cast<float>(T[((c * n2 + kz) * n1 + ky) * hxs + kx].x) : 0.0f;
+    const int64_t Q = n1h * hxs, p = e % Q, r2 = e / Q, kz = r2 % n2h, c = r2 / n2h;
+    const int64_t kt = p / (kTile * n1h), r = p - kt * (kTile * n1h), ky = r / kTile, kx = kt * kTile + r % kTile;
+    Ts[e] = kx < hx ? static_cast<float>(T[(c * n2 + kz) * n1 * hxs + tcol(kx, kTile * n1) + kTile * ky].x) : 0.0f;
 }
 
-// H^ = N^ M^ with the symmetric tensor, in place on the spectrum [3][n2][n1][hxs]; one thread per
-// k-point (kz, ky, kx < hx).
+// H^ = N^ M^ with the symmetric tensor, in place on the spectrum [3][n2][plane]; one thread per
+// plane position e (column tiles, padding columns kx >= hx skipped).
 __global__ void __launch_bounds__(256)
-    k_kspace_mul_sym(float2* __restrict__ s, const float* __restrict__ t, int n1, int n2, FastDiv hxd, FastDiv n1d,
-                     int hxs, int n1h, int n2h, int total) {
+    k_kspace_mul_sym(float2* __restrict__ s, const float* __restrict__ t, int n1, int n2, int hx, FastDiv pld,
+                     FastDiv n1Td, int hxs, int n1h, int n2h, int total) {
     const int e = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
     if (e >= total) return;
-    const int r = static_cast<int>(fdiv(e, hxd)), kx = e - r * static_cast<int>(hxd.d);
-    const int kz = static_cast<int>(fdiv(r, n1d)), ky = r - kz * n1;
+    const int kz = static_cast<int>(fdiv(e, pld)), p = e - kz * static_cast<int>(pld.d);
+    const int kt = static_cast<int>(fdiv(p, n1Td)), r = p - kt * static_cast<int>(n1Td.d);
+    const int ky = r >> 2, kx = kt * kTile + (r & (kTile - 1));
+    if (kx >= hx) return;
     const bool my = ky >= n1h, mz = kz >= n2h;
     const int kys = my ? n1 - ky : ky, kzs = mz ? n2 - kz : kz;
     const float sy = my ? -1.0f : 1.0f, sz = mz ? -1.0f : 1.0f;
-    const int64_t tc = static_cast<int64_t>(n2h) * n1h * hxs;
-    const int64_t to = (static_cast<int64_t>(kzs) * n1h + kys) * hxs + kx;
+    const int64_t Q = static_cast<int64_t>(n1h) * hxs, tc = static_cast<int64_t>(n2h) * Q;
+    const int64_t to = kzs * Q + tcol(kx, kTile * n1h) + kTile * kys;
     const float nxx = t[to], nxy = sy * t[tc + to], nxz = sz * t[2 * tc + to], nyy = t[3 * tc + to],
                 nyz = sy * sz * t[4 * tc + to], nzz = t[5 * tc + to];
-    const int64_t K = static_cast<int64_t>(n2) * n1 * hxs;
-    const int64_t so = (static_cast<int64_t>(kz) * n1 + ky) * hxs + kx;
-    const float2 m0 = s[so], m1 = s[K + so], m2 = s[2 * K + so];
-    s[so] = float2{nxx * m0.x + nxy * m1.x + nxz * m2.x, nxx * m0.y + nxy * m1.y + nxz * m2.y};
-    s[K + so] = float2{nxy * m0.x + nyy * m1.x + nyz * m2.x, nxy * m0.y + nyy * m1.y + nyz * m2.y};
-    s[2 * K + so] = float2{nxz * m0.x + nyz * m1.x + nzz * m2.x, nxz * m0.y + nyz * m1.y + nzz * m2.y};
+    const int64_t K = static_cast<int64_t>(total);
+    const float2 m0 = s[e], m1 = s[K + e], m2 = s[2 * K + e];
+    s[e] = float2{nxx * m0.x + nxy * m1.x + nxz * m2.x, nxx * m0.y + nxy * m1.y + nxz * m2.y};
+    s[K + e] = float2{nxy * m0.x + nyy * m1.x + nyz * m2.x, nxy * m0.y + nyy * m1.y + nyz * m2.y};
+    s[2 * K + e] = float2{nxz * m0.x + nyz * m1.x + nzz * m2.x, nxz * m0.y + nyz * m1.y + nzz * m2.y};
 }
 
-// Fused z stage with the symmetric tensor.  One thread per (kys <= n1/2, kx < hx): it loads the
-// tensor column once (6 (NZ + 1) reals) and serves both spectrum columns ky = kys and ky = n1 - kys.
+// Fused z stage with the symmetric tensor.  One thread per (kys <= n1/2, kx < hx), taken in
+// (kx mod 4, kys, kx / 4) order so neighbouring lanes touch neighbouring addresses of the column
+// tiles: it loads the tensor column once (6 (NZ + 1) reals) and serves both spectrum columns
+// ky = kys and ky = n1 - kys.
 template <int NZ>
 __global__ void __launch_bounds__(256)
-    k_zfused_sym(float2* __restrict__ s, const float* __restrict__ t, int n1, FastDiv hxd, int hxs, int n1h,
+    k_zfused_sym(float2* __restrict__ s, const float* __restrict__ t, int n1, int hx, int hxs, FastDiv n1hd,
                  ZTw<float2> tw) {
     constexpr int NZ2 = 2 * NZ, NH = NZ + 1;
     const int e = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
-    const int kys = static_cast<int>(fdiv(e, hxd)), kx = e - kys * static_cast<int>(hxd.d);
-    if (kys >= n1h) return;
+    const int r = e >> 2, kt = static_cast<int>(fdiv(r, n1hd)), kys = r - kt * static_cast<int>(n1hd.d);
+    const int kx = kt * kTile + (e & (kTile - 1));
+    if (kx >= hx) return;
+    const int n1h = static_cast<int>(n1hd.d);
     const int64_t plane = static_cast<int64_t>(n1) * hxs;
-    const int64_t tc = static_cast<int64_t>(NH) * n1h * hxs;
+    const int64_t Q = static_cast<int64_t>(n1h) * hxs, tc = static_cast<int64_t>(NH) * Q;
+    const int64_t tcx = tcol(kx, kTile * n1h) + kTile * kys;
     float tv[6][NH];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
 #pragma unroll
-        for (int k = 0; k < NH; ++k) tv[c][k] = t[c * tc + (static_cast<int64_t>(k) * n1h + kys) * hxs + kx];
+        for (int k = 0; k < NH; ++k) tv[c][k] = t[c * tc + k * Q + tcx];
+    const int64_t scx = tcol(kx, kTile * static_cast<int64_t>(n1));
     for (int pass = 0; pass < 2; ++pass) {
         if (pass == 1 && (kys == 0 || kys == n1h - 1)) break;
         const int ky = pass == 0 ? kys : n1 - kys;
         const float sy = pass == 0 ? 1.0f : -1.0f;
-        const int64_t col = static_cast<int64_t>(ky) * hxs + kx;
+        const int64_t col = scx + kTile * ky;
         float2 m[3][NZ], h[3][NZ];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -614,7 +642,7 @@ void fft_strided(mafx_ctx* ctx, const FftPlan& p, C* data, int64_t sa, int na, i
     const size_t lds = lds_padded(W * L) * sizeof(C);
     allow_lds(reinterpret_cast<const void*>(fn), lds);
     hipLaunchKernelGGL(fn, dim3(gx, nb, na), dim3(kAxisBlock), lds, ctx->stream, data, sa, sb, stride, ctx->hx, n_in,
-                       n_out, make_fastdiv(W), d, tw_of<C>(p), inv ? 1 : 0);
+                       n_out, make_fastdiv(W), d, tw_of<C>(p), inv ? 1 : 0, kTile * ctx->n1);
     MAFX_HIP(hipGetLastError());
 }
 
@@ -631,7 +659,7 @@ void fft_y_split(mafx_ctx* ctx, C* data, int64_t sa, int na, int64_t sb, int nb,
     // lines of up to E64/2 run 1024-lane work-groups on a 64 KiB tile under an 8-wave register cap
     // (two groups per CU); FP64 ones 1024-lane groups on a 128 KiB tile; both stage the twiddles in
     // LDS.  Longer or non-power-of-two lines use 512-lane groups, a 128 KiB tile and global twiddles.
-    using KT = void (*)(C*, int64_t, int64_t, int64_t, int64_t, FastDiv, FastDiv, FftDesc, const C*, const C*);
+    using KT = void (*)(C*, int64_t, int64_t, int64_t, int64_t, FastDiv, FastDiv, FftDesc, const C*, const C*, int64_t);
     KT fn = nullptr;
     int bs = 512, elems = 2 * E64;
     bool twl = false;
@@ -659,7 +687,7 @@ void fft_y_split(mafx_ctx* ctx, C* data, int64_t sa, int na, int64_t sb, int nb,
     const size_t lds = lds_of(W2);
     allow_lds(reinterpret_cast<const void*>(fn), lds);
     hipLaunchKernelGGL(fn, dim3(gx, nb, na), dim3(bs), lds, ctx->stream, data, sa, sb, stride, ctx->hx, make_fastdiv(W),
-                       make_fastdiv(W2), d, tw_of<C>(ph), tw_of<C>(ctx->py));
+                       make_fastdiv(W2), d, tw_of<C>(ph), tw_of<C>(ctx->py), kTile * ctx->n1);
     MAFX_HIP(hipGetLastError());
 }
 
@@ -710,15 +738,16 @@ void tensor_sym_f32(mafx_ctx* ctx, const double2* T, float* Ts) {
 }
 
 void kspace_mul_sym(mafx_ctx* ctx, float2* spec, const float* ts, int64_t n2) {
-    const int64_t total = n2 * ctx->n1 * ctx->hx;
+    const int64_t plane = ctx->n1 * ctx->hxs, total = n2 * plane;
     require(total < (int64_t{1} << 31), "spectrum too large for the symmetric tensor product");
-    const double bytes = static_cast<double>(total) * (6.0 * sizeof(float2)) +
+    const double bytes = static_cast<double>(n2 * ctx->n1 * ctx->hx) * (6.0 * sizeof(float2)) +
                          6.0 * sizeof(float) * static_cast<double>(ctx->n2h * ctx->n1h * ctx->hx);
     ProfScope ps(ctx, "kspace_mul_sym_f32", bytes);
     hipLaunchKernelGGL(k_kspace_mul_sym, dim3(grid_for(total, 256)), dim3(256), 0, ctx->stream, spec, ts,
-                       static_cast<int>(ctx->n1), static_cast<int>(n2), make_fastdiv(static_cast<uint32_t>(ctx->hx)),
-                       make_fastdiv(static_cast<uint32_t>(ctx->n1)), static_cast<int>(ctx->hxs),
-                       static_cast<int>(ctx->n1h), static_cast<int>(ctx->n2h), static_cast<int>(total));
+                       static_cast<int>(ctx->n1), static_cast<int>(n2), static_cast<int>(ctx->hx),
+                       make_fastdiv(static_cast<uint32_t>(plane)), make_fastdiv(static_cast<uint32_t>(kTile * ctx->n1)),
+                       static_cast<int>(ctx->hxs), static_cast<int>(ctx->n1h), static_cast<int>(ctx->n2h),
+                       static_cast<int>(total));
     MAFX_HIP(hipGetLastError());
 }
 
@@ -730,22 +759,24 @@ void zfused_sym(mafx_ctx* ctx, float2* spec, const float* ts, int nz) {
         const long double a = two_pi * t / n2;
         tw.w[t] = float2{static_cast<float>(std::cos(a)), static_cast<float>(-std::sin(a))};
     }
-    const int64_t threads = ctx->n1h * ctx->hx;
+    // (kx mod 4, kys, kx / 4) over whole column tiles; columns kx >= hx return at once
+    const int64_t threads = ctx->n1h * kTile * ((ctx->hx + kTile - 1) / kTile);
+    require(threads < (int64_t{1} << 31), "spectrum too large for the fused z stage");
     // spectrum columns read and written once, the tensor quarter read once
     const double bytes = static_cast<double>(ctx->n1 * ctx->hx) * (2.0 * 3 * nz) * sizeof(float2) +
-                         static_cast<double>(threads) * 6.0 * (nz + 1) * sizeof(float);
+                         static_cast<double>(ctx->n1h * ctx->hx) * 6.0 * (nz + 1) * sizeof(float);
     ProfScope ps(ctx, "zfused_mul_f32", bytes);
     const dim3 g(grid_for(threads, 256)), b(256);
-    const int n1 = static_cast<int>(ctx->n1), hxs = static_cast<int>(ctx->hxs), n1h = static_cast<int>(ctx->n1h);
-    const FastDiv hxd = make_fastdiv(static_cast<uint32_t>(ctx->hx));
+    const int n1 = static_cast<int>(ctx->n1), hxs = static_cast<int>(ctx->hxs), hx = static_cast<int>(ctx->hx);
+    const FastDiv n1hd = make_fastdiv(static_cast<uint32_t>(ctx->n1h));
     switch (nz) {
-    case 2: hipLaunchKernelGGL((k_zfused_sym<2>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
-    case 3: hipLaunchKernelGGL((k_zfused_sym<3>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
-    case 4: hipLaunchKernelGGL((k_zfused_sym<4>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
-    case 5: hipLaunchKernelGGL((k_zfused_sym<5>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
-    case 6: hipLaunchKernelGGL((k_zfused_sym<6>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
-    case 7: hipLaunchKernelGGL((k_zfused_sym<7>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
-    case 8: hipLaunchKernelGGL((k_zfused_sym<8>), g, b, 0, ctx->stream, spec, ts, n1, hxd, hxs, n1h, tw); break;
+    case 2: hipLaunchKernelGGL((k_zfused_sym<2>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
+    case 3: hipLaunchKernelGGL((k_zfused_sym<3>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
+    case 4: hipLaunchKernelGGL((k_zfused_sym<4>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
+    case 5: hipLaunchKernelGGL((k_zfused_sym<5>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
+    case 6: hipLaunchKernelGGL((k_zfused_sym<6>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
+    case 7: hipLaunchKernelGGL((k_zfused_sym<7>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
+    case 8: hipLaunchKernelGGL((k_zfused_sym<8>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
     default: throw Error(MAFX_ERR_ARG, "fused z stage supports 2 <= nz <= 8");
     }
     MAFX_HIP(hipGetLastError());

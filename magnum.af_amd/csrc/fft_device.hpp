@@ -40,6 +40,14 @@ __device__ __forceinline__ uint3 swizzled_block() {
     return uint3{l % gx, (l / gx) % gy, l / (gx * gy)};
 }
 
+// Column tiles of spectra and tensors.  A plane (fixed component and z) of n1 rows ky and hxs
+// columns kx stores column kx of row ky at tcol(kx, 4 n1) + 4 ky with
+// tcol(kx, n1T) = (kx / 4) n1T + kx % 4: each group of 4 columns is one contiguous block over all
+// rows.  A y line tile of 4 columns is then a contiguous 32 n1-byte block (FP32) instead of n1
+// separate 32-byte pieces, and 4 consecutive x rows of one column tile form 128 contiguous bytes.
+constexpr int kTile = 4;
+__host__ __device__ inline int64_t tcol(int64_t kx, int64_t n1T) { return (kx >> 2) * n1T + (kx & 3); }
+
 // LDS index padding: one spare element after every 32.  The autosort writes of a stage move
 // groups of W lanes at a stride of R W elements, a multiple of the bank count for the power-of-two
 // shapes here; the padding spreads consecutive groups over different banks.
@@ -101,12 +109,11 @@ __device__ __forceinline__ void dft(C (&v)[R], bool inv, const C* __restrict__ t
 // Packs row k of the half spectrum X[0..M] for the half-length inverse transform of a real
 // sequence of length 2M: Zp[k] = (X[k] + conj X[M-k]) + i w_{2M}^-k (X[k] - conj X[M-k]); the
 // unnormalised inverse FFT_M of Zp then holds x[2n] + i x[2n+1].
-template <class C>
-__device__ __forceinline__ C c2r_pack(const C* __restrict__ row, int k, int M, const C* __restrict__ twL) {
-    const C Xk = row[k], Xm = row[M - k];
+// (Xk = X[k], Xm = X[M-k], twk = w_{2M}^k.)
+template <class C> __device__ __forceinline__ C c2r_combine(C Xk, C Xm, C twk) {
     const C S{Xk.x + Xm.x, Xk.y - Xm.y};
     const C D{Xk.x - Xm.x, Xk.y + Xm.y};
-    const C wD = cmul(conj_if(twL[k], true), D);
+    const C wD = cmul(conj_if(twk, true), D);
     return C{S.x - wD.y, S.y + wD.x};
 }
 

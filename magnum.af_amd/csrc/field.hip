@@ -534,7 +534,8 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
         const double* ms = (!ctx->atom_dipole && ctx->has_ms_field) ? ctx->ms_field.as<double>() : nullptr;
         const double scale = ctx->atom_dipole ? 1.0 : (ctx->has_ms_field ? 1.0 : ctx->Ms);
         fft_x_forward<C>(ctx, a.m, ms, scale, g.nx, g.ny, g.nz, 3, g.ny, g.nz, spec, nzs);
-        fft_y_split<C>(ctx, spec, sa, 3, n1 * hxs, static_cast<int>(g.nz), hxs, false);
+        // column tiles: rows ky are kTile elements apart, planes n1 hxs
+        fft_y_split<C>(ctx, spec, sa, 3, n1 * hxs, static_cast<int>(g.nz), kTile, false);
         // tensor product: complex reference spectrum (FP64) or its real symmetric quarter (FP32)
         auto kmul = [&](int64_t planes) {
             if constexpr (std::is_same<C, float2>::value)
@@ -550,13 +551,13 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
             else
                 zfused<C>(ctx, spec, ctx->tensor.as<C>(), static_cast<int>(g.nz));
         } else {
-            fft_strided<C>(ctx, ctx->pz, spec, sa, 3, hxs, static_cast<int>(n1), n1 * hxs, static_cast<int>(g.nz),
+            fft_strided<C>(ctx, ctx->pz, spec, sa, 3, kTile, static_cast<int>(n1), n1 * hxs, static_cast<int>(g.nz),
                            static_cast<int>(n2), false, sizeof(C) == 8 ? "fft_z_fwd_f32" : "fft_z_fwd_f64");
             kmul(n2);
-            fft_strided<C>(ctx, ctx->pz, spec, sa, 3, hxs, static_cast<int>(n1), n1 * hxs, static_cast<int>(n2),
+            fft_strided<C>(ctx, ctx->pz, spec, sa, 3, kTile, static_cast<int>(n1), n1 * hxs, static_cast<int>(n2),
                            static_cast<int>(g.nz), true, sizeof(C) == 8 ? "fft_z_inv_f32" : "fft_z_inv_f64");
         }
-        fft_y_split<C>(ctx, spec, sa, 3, n1 * hxs, static_cast<int>(g.nz), hxs, true);
+        fft_y_split<C>(ctx, spec, sa, 3, n1 * hxs, static_cast<int>(g.nz), kTile, true);
         const double P_total = static_cast<double>(ctx->n0) * n1 * n2;
         const double hscale = (ctx->atom_dipole ? ctx->Ms : 1.0) / P_total;
         T* hd = ctx->hdem.as<T>();
@@ -766,10 +767,11 @@ void build_tensor(mafx_ctx* ctx) {
     MAFX_HIP(hipMemset(T64.p, 0, T64.bytes)); // padding columns stay defined
     double2* t = T64.as<double2>();
     fft_x_forward<double2>(ctx, Nd.as<double>(), nullptr, 1.0, n0, n1, n2, 6, n1, n2, t, n2);
-    fft_strided<double2>(ctx, ctx->py, t, K, 6, n1 * hxs, static_cast<int>(n2), hxs, static_cast<int>(n1),
+    // column tiles (fft_device.hpp): lines along y step kTile, lines along z one plane
+    fft_strided<double2>(ctx, ctx->py, t, K, 6, n1 * hxs, static_cast<int>(n2), kTile, static_cast<int>(n1),
                          static_cast<int>(n1), false, "tensor_fft_y");
     if (n2 > 1)
-        fft_strided<double2>(ctx, ctx->pz, t, K, 6, hxs, static_cast<int>(n1), n1 * hxs, static_cast<int>(n2),
+        fft_strided<double2>(ctx, ctx->pz, t, K, 6, kTile, static_cast<int>(n1), n1 * hxs, static_cast<int>(n2),
                              static_cast<int>(n2), false, "tensor_fft_z");
     MAFX_HIP(hipStreamSynchronize(ctx->stream));
     Nd.release();

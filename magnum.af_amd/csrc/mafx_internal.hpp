@@ -130,10 +130,11 @@ struct mafx_ctx {
     int64_t n1h{0}, n2h{0};   // n1/2 + 1, n2/2 + 1: ky, kz extent of the symmetric FP32 tensor
     mafx::FftPlan px, py, pz; // lengths n0, n1, n2 (px, py also supply the packing / split twiddles)
     mafx::FftPlan pxh, pyh;   // lengths nx, ny: the half-length x and split y transforms
-    // FP64 path: [6][n2][n1][hxs] complex f64, the transformed reference tensor as it is.
-    // FP32 path: [6][n2h][n1h][hxs] real f32, the mirror-symmetric spectrum (fft.hip).
+    // FP64 path: [6][n2] planes of n1 hxs complex f64, the transformed reference tensor as it is.
+    // FP32 path: [6][n2h] planes of n1h hxs real f32, the mirror-symmetric spectrum (fft.hip).
+    // Planes store columns in tiles of 4 (tcol, fft_device.hpp).
     mafx::DevBuf tensor;
-    mafx::DevBuf spec;        // [3][nzs][n1][hxs] complex workspace
+    mafx::DevBuf spec;        // [3][nzs] planes of n1 hxs complex values, column tiles
     mafx::DevBuf hdem;        // [3][N] FFT-convolution field (f64 or f32) between the x inverse and the cell pass
     mafx::DevBuf hbuf;        // 3N f64 scratch (field output for helpers)
     mafx::DevBuf partials;    // reduction partials (f64)

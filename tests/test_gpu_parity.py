@@ -166,14 +166,18 @@ def test_demag_cube_energy_gpu():
 
 
 # FP32 field path: nz = 1 (k-space product), 2 <= nz <= 8 (fused z stage), nz > 8 (strided z
-# transforms), odd and prime sizes, one cell.  Tolerance: FP32 rounding of the FFTs and stencils.
+# transforms), power-of-two y lines from 1 to 1024 and odd and prime sizes.  Tolerance: FP32
+# rounding of the FFTs and stencils.
 FP32_TOL = 1e-5
 
 
 @pytest.mark.parametrize("mesh", [(16, 8, 4, 2e-9, 2e-9, 2e-9), (64, 32, 1, 3e-9, 3e-9, 3e-9),
-                                  (5, 4, 3, 1e-9, 1.2e-9, 0.8e-9), (7, 3, 1, 2e-9, 1e-9, 3e-9),
-                                  (33, 7, 5, 1e-9, 2e-9, 1.5e-9), (12, 6, 10, 2e-9, 2e-9, 2e-9),
-                                  (20, 9, 8, 1e-9, 1e-9, 1e-9), (1, 1, 1, 2e-9, 2e-9, 2e-9)])
+                                  (5, 4, 3, 1e-9, 1.2e-9, 0.8e-9), (24, 16, 2, 1e-9, 2e-9, 1e-9),
+                                  (16, 8, 5, 2e-9, 1e-9, 1e-9), (128, 256, 4, 2e-9, 2e-9, 2e-9),
+                                  (8, 1024, 4, 2e-9, 2e-9, 2e-9), (10, 64, 6, 1e-9, 1e-9, 2e-9),
+                                  (7, 3, 1, 2e-9, 1e-9, 3e-9), (33, 7, 5, 1e-9, 2e-9, 1.5e-9),
+                                  (12, 6, 10, 2e-9, 2e-9, 2e-9), (20, 9, 8, 1e-9, 1e-9, 1e-9),
+                                  (1, 1, 1, 2e-9, 2e-9, 2e-9)])
 def test_heff_fp32_field_error(mesh):
     print(f"\n[heff fp32] mesh={mesh}", flush=True)
     m = rand_m(mesh[:3], 5)
@@ -204,7 +208,8 @@ def test_fp32_ms_field_airbox():
     assert normwise(llg.H_in_Apm(state), sim.heff()) <= FP32_TOL
 
 
-@pytest.mark.parametrize("mesh", [(16, 12, 1, 2.7e-10, 2.7e-10, 2.7e-10), (8, 6, 3, 2.7e-10, 2.7e-10, 2.7e-10)])
+@pytest.mark.parametrize("mesh", [(16, 12, 1, 2.7e-10, 2.7e-10, 2.7e-10), (8, 6, 3, 2.7e-10, 2.7e-10, 2.7e-10),
+                                  (12, 16, 2, 2.7e-10, 2.7e-10, 2.7e-10)])
 def test_fp32_atomistic_dipole(mesh):
     print(f"\n[fp32 atomistic] mesh={mesh}", flush=True)
     J = 1.6e-21
