@@ -355,6 +355,102 @@ __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
     }
 }
 
+// Compile-time shaped forms of k_fft_y_fwd2 / k_fft_y_inv2 for power-of-two M = 2^LOGM with
+// 2^LOGE complex values per LDS tile: W = 2^(LOGE - LOGM - 1) >= 4 columns, i.e. W / 4 whole
+// column tiles, per work-group.  Row y of column kx0 + w sits at tile w / 4, offset 4 y + w % 4,
+// and a tile spans n1T = 4 n1 = 8 M elements, so a lane's global index g walks the tiles'
+// contiguous storage and every coordinate is a bit field of g.  Columns kx >= hx (the padding of
+// the last tile group) are read as zero and not written.
+template <class C, int LOGM, int LOGE, int BS, int MINW>
+__global__ void __launch_bounds__(BS, MINW)
+    k_fft_y_fwd_ct(C* __restrict__ data, int64_t sa, int64_t sb, int hx, const C* __restrict__ twh,
+                   const C* __restrict__ twL) {
+    constexpr int M = 1 << LOGM, LOGW2 = LOGE - LOGM, W2 = 1 << LOGW2, W = W2 / 2, E = 1 << LOGE;
+    constexpr int n1T = 8 * M; // tile stride; offsets inside a work-group's tiles stay below 2 E
+    static_assert(W >= kTile && E % BS == 0, "whole column tiles, whole lane loops");
+    extern __shared__ unsigned char smem_raw[];
+    C* buf = reinterpret_cast<C*>(smem_raw);
+    C* tws = buf + lds_padded(E);
+    for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
+    const uint3 blk = swizzled_block();
+    const int kx0 = static_cast<int>(blk.x) * W;
+    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + (kx0 / kTile) * n1T;
+    const int tid = static_cast<int>(threadIdx.x);
+    // inputs y < M of the W columns: g = (tile * M + y) * 4 + w % 4
+    constexpr int IT = M * W / BS;
+    C v[IT], t[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int g = tid + i * BS;
+        const int tl = g >> (LOGM + 2), y = (g >> 2) & (M - 1), w = (tl << 2) | (g & 3);
+        v[i] = kx0 + w < hx ? base[tl * n1T + (g & (4 * M - 1))] : C{0, 0};
+        t[i] = twL[y];
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int g = tid + i * BS;
+        const int tl = g >> (LOGM + 2), y = (g >> 2) & (M - 1), w = (tl << 2) | (g & 3);
+        buf[lds_pad((y << LOGW2) + w)] = v[i];
+        buf[lds_pad((y << LOGW2) + W + w)] = cmul(v[i], t[i]);
+    }
+    __syncthreads();
+    ct_fft<C, LOGM, LOGW2, BS, false>(buf, tws);
+    // outputs ky = 2 k + odd of every column: g = (tile * 2M + ky) * 4 + w % 4
+    constexpr int OT = 2 * M * W / BS;
+#pragma unroll
+    for (int i = 0; i < OT; ++i) {
+        const int g = tid + i * BS;
+        const int tl = g >> (LOGM + 3), r = g & (8 * M - 1), k = r >> 3, odd = (r >> 2) & 1;
+        const int w = (tl << 2) | (g & 3);
+        if (kx0 + w < hx) base[tl * n1T + r] = buf[lds_pad((k << LOGW2) + odd * W + w)];
+    }
+}
+
+template <class C, int LOGM, int LOGE, int BS, int MINW>
+__global__ void __launch_bounds__(BS, MINW)
+    k_fft_y_inv_ct(C* __restrict__ data, int64_t sa, int64_t sb, int hx, const C* __restrict__ twh,
+                   const C* __restrict__ twL) {
+    constexpr int M = 1 << LOGM, LOGW2 = LOGE - LOGM, W2 = 1 << LOGW2, W = W2 / 2, E = 1 << LOGE;
+    constexpr int n1T = 8 * M; // tile stride; offsets inside a work-group's tiles stay below 2 E
+    static_assert(W >= kTile && E % BS == 0, "whole column tiles, whole lane loops");
+    extern __shared__ unsigned char smem_raw[];
+    C* buf = reinterpret_cast<C*>(smem_raw);
+    C* tws = buf + lds_padded(E);
+    for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
+    const uint3 blk = swizzled_block();
+    const int kx0 = static_cast<int>(blk.x) * W;
+    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + (kx0 / kTile) * n1T;
+    const int tid = static_cast<int>(threadIdx.x);
+    // inputs: every ky = 2 k + odd; X[2k] goes to line w, X[2k+1] to line W + w
+    constexpr int IT = 2 * M * W / BS;
+    C v[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int g = tid + i * BS;
+        const int tl = g >> (LOGM + 3), w = (tl << 2) | (g & 3);
+        v[i] = kx0 + w < hx ? base[tl * n1T + (g & (8 * M - 1))] : C{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int g = tid + i * BS;
+        const int tl = g >> (LOGM + 3), r = g & (8 * M - 1), k = r >> 3, odd = (r >> 2) & 1;
+        const int w = (tl << 2) | (g & 3);
+        buf[lds_pad((k << LOGW2) + odd * W + w)] = v[i];
+    }
+    __syncthreads();
+    ct_fft<C, LOGM, LOGW2, BS, true>(buf, tws);
+    // x[y] = IFFT_M(X_even)[y] + w_L^-y IFFT_M(X_odd)[y], y < M
+    constexpr int OT = M * W / BS;
+#pragma unroll
+    for (int i = 0; i < OT; ++i) {
+        const int g = tid + i * BS;
+        const int tl = g >> (LOGM + 2), y = (g >> 2) & (M - 1), w = (tl << 2) | (g & 3);
+        if (kx0 + w < hx)
+            base[tl * n1T + (g & (4 * M - 1))] = cadd(buf[lds_pad((y << LOGW2) + w)],
+                                                      cmul(conj_if(twL[y], true), buf[lds_pad((y << LOGW2) + W + w)]));
+    }
+}
+
 // H^ = N^ M^ per k-point with the symmetric tensor (xx, xy, xz, yy, yz, zz) stored as full complex
 // (demag_field.cpp:236-249); in place on the spectrum.
 template <class C>
@@ -655,6 +751,29 @@ void fft_y_split(mafx_ctx* ctx, C* data, int64_t sa, int na, int64_t sb, int nb,
     constexpr int E64 = lds_elems<C>();
     constexpr bool kF32 = sizeof(C) == sizeof(float2);
     constexpr size_t kLdsMax = 160 * 1024;
+    // compile-time shapes: power-of-two M of 256..1024 on 8192-element tiles of whole column tiles
+    // (1024 lanes; FP32 under an 8-wave register cap, two groups per CU; FP64 one group per CU)
+    if (stride == kTile && ctx->n1 == 2 * M && (M == 256 || M == 512 || M == 1024)) {
+        constexpr int LOGE = 13, MW = kF32 ? 8 : 4;
+        using KC = void (*)(C*, int64_t, int64_t, int, const C*, const C*);
+        KC fc = nullptr;
+        if (M == 1024) fc = inv ? k_fft_y_inv_ct<C, 10, LOGE, 1024, MW> : k_fft_y_fwd_ct<C, 10, LOGE, 1024, MW>;
+        if (M == 512) fc = inv ? k_fft_y_inv_ct<C, 9, LOGE, 1024, MW> : k_fft_y_fwd_ct<C, 9, LOGE, 1024, MW>;
+        if (M == 256) fc = inv ? k_fft_y_inv_ct<C, 8, LOGE, 1024, MW> : k_fft_y_fwd_ct<C, 8, LOGE, 1024, MW>;
+        const int W = (1 << LOGE) / (2 * M);
+        const unsigned gx = static_cast<unsigned>((ctx->hx + W - 1) / W);
+        require(static_cast<int64_t>(gx) * W <= ctx->hxs, "y transform: column tiles exceed the padded rows");
+        const double bytes = static_cast<double>(na) * nb * ctx->hx * (3.0 * M) * sizeof(C);
+        ProfScope ps(ctx,
+                     inv ? nm("fft_y_inv_f32", "fft_y_inv_f64", sizeof(C)) : nm("fft_y_fwd_f32", "fft_y_fwd_f64", sizeof(C)),
+                     bytes);
+        const size_t lds = (lds_padded(1 << LOGE) + M) * sizeof(C);
+        allow_lds(reinterpret_cast<const void*>(fc), lds);
+        hipLaunchKernelGGL(fc, dim3(gx, nb, na), dim3(1024), lds, ctx->stream, data, sa, sb, static_cast<int>(ctx->hx),
+                           tw_of<C>(ph), tw_of<C>(ctx->py));
+        MAFX_HIP(hipGetLastError());
+        return;
+    }
     // Tile shapes, measured at 1024 x 1024 x 4 (one column pair = 2 lines): FP32 power-of-two
     // lines of up to E64/2 run 1024-lane work-groups on a 64 KiB tile under an 8-wave register cap
     // (two groups per CU); FP64 ones 1024-lane groups on a 128 KiB tile; both stage the twiddles in

@@ -199,6 +199,60 @@ __device__ void fft_lds(C* buf, const FftDesc& d, const FastDiv& W, const C* __r
     }
 }
 
+// ---- Compile-time shaped transforms ---------------------------------------------------------
+// Power-of-two lines whose length M = 2^LOGM and line count W2 = 2^LOGW2 are template constants.
+// Every index of a stage is then a shift or a mask instead of a multiply-high division, each lane
+// owns an exact number of butterflies, and the twiddle step of every stage is a constant.  Same
+// Stockham recurrence, radix schedule (radix-8 stages, then one radix-2 or radix-4 stage) and
+// twiddle table as fft_lds, so the arithmetic of each butterfly is unchanged.
+constexpr int ct_nstages(int logm) { return logm / 3 + (logm % 3 ? 1 : 0); }
+constexpr int ct_logr(int logm, int s) { return s < logm / 3 ? 3 : logm % 3; }
+constexpr int ct_logns(int logm, int s) { return 3 * (s < logm / 3 ? s : logm / 3); }
+
+template <class C, int LOGM, int LOGW2, int LOGR, int LOGNS, int BS, bool INV>
+__device__ __forceinline__ void ct_stage(C* buf, const C* __restrict__ tw) {
+    constexpr int R = 1 << LOGR, W2 = 1 << LOGW2, NS = 1 << LOGNS;
+    constexpr int NB = (1 << (LOGM - LOGR)) << LOGW2; // butterflies; also the stride between inputs
+    constexpr int NBT = (NB + BS - 1) / BS;
+    constexpr int TWS = LOGM - LOGNS - LOGR; // log2 of the twiddle step M / (Ns R)
+    const int tid = static_cast<int>(threadIdx.x);
+    C v[NBT][R];
+#pragma unroll
+    for (int i = 0; i < NBT; ++i) {
+        const int b = tid + i * BS;
+        if (NB % BS == 0 || b < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = buf[lds_pad(b + r * NB)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NBT; ++i) {
+        const int b = tid + i * BS;
+        if (NB % BS == 0 || b < NB) {
+            const int j = b >> LOGW2, l = b & (W2 - 1), k = j & (NS - 1);
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], conj_if(tw[(r * k) << TWS], INV));
+            }
+            dft<C, R>(v[i], INV, tw, 1 << LOGM);
+            const int o = ((((j - k) << LOGR) + k) << LOGW2) + l;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[lds_pad(o + (r << (LOGNS + LOGW2)))] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+// All stages of W2 lines of length M held in buf[pos * W2 + line]; tw = w_M^t, t < M.
+template <class C, int LOGM, int LOGW2, int BS, bool INV, int S = 0>
+__device__ __forceinline__ void ct_fft(C* buf, const C* __restrict__ tw) {
+    if constexpr (S < ct_nstages(LOGM)) {
+        ct_stage<C, LOGM, LOGW2, ct_logr(LOGM, S), ct_logns(LOGM, S), BS, INV>(buf, tw);
+        ct_fft<C, LOGM, LOGW2, BS, INV, S + 1>(buf, tw);
+    }
+}
+
 inline bool is_pow2_plan(const FftDesc& d) {
     for (int s = 0; s < d.nstages; ++s)
         if (d.radix[s] != 2 && d.radix[s] != 4 && d.radix[s] != 8) return false;

@@ -110,6 +110,18 @@ def test_heff_fp64_matches_oracle(mesh, which):
     assert abs(E_gpu - E_ref) <= FP64_TOL * energy_scale(mesh, 1.1e6, Et_ref)
 
 
+@pytest.mark.parametrize("mesh", [(8, 256, 2, 2e-9, 1e-9, 2e-9), (5, 512, 3, 1e-9, 1e-9, 1e-9),
+                                  (4, 1024, 1, 2e-9, 2e-9, 3e-9), (3, 1024, 4, 2e-9, 2e-9, 2e-9)])
+def test_heff_fp64_pow2_columns(mesh):
+    """Power-of-two ny of 256..1024: the y transforms run the compile-time shaped kernels."""
+    print(f"\n[heff fp64 pow2 y] mesh={mesh}", flush=True)
+    m = rand_m(mesh[:3], 21)
+    state, terms, sim = build(mesh, 1.1e6, m, MICRO_ALL)
+    M = mf()
+    h = M.LLGIntegrator(0.1, terms).H_in_Apm(state)
+    assert normwise(h, sim.heff()) <= FP64_TOL, normwise(h, sim.heff())
+
+
 def test_heff_each_term_energy_and_zeeman():
     mesh = (12, 10, 4, 2e-9, 2e-9, 2e-9)
     nx, ny, nz = mesh[:3]
@@ -177,6 +189,7 @@ FP32_TOL = 1e-5
                                   (8, 1024, 4, 2e-9, 2e-9, 2e-9), (10, 64, 6, 1e-9, 1e-9, 2e-9),
                                   (7, 3, 1, 2e-9, 1e-9, 3e-9), (33, 7, 5, 1e-9, 2e-9, 1.5e-9),
                                   (12, 6, 10, 2e-9, 2e-9, 2e-9), (20, 9, 8, 1e-9, 1e-9, 1e-9),
+                                  (6, 512, 2, 2e-9, 1e-9, 2e-9), (12, 256, 1, 2e-9, 2e-9, 3e-9),
                                   (1, 1, 1, 2e-9, 2e-9, 2e-9)])
 def test_heff_fp32_field_error(mesh):
     print(f"\n[heff fp32] mesh={mesh}", flush=True)
