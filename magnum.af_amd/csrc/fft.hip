@@ -16,6 +16,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "fft_device.hpp"
 #include "fft_kernels.hpp"
@@ -227,6 +228,117 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
         const int x = 2 * n;
         out[dst + x] = static_cast<T>(static_cast<double>(v.x) * hscale);
         if (x + 1 < Xo) out[dst + x + 1] = static_cast<T>(static_cast<double>(v.y) * hscale);
+    }
+}
+
+// Compile-time shaped forms of k_fft_x_fwd / k_fft_x_inv for the demag field: power-of-two
+// M = nx = n0 / 2, rows of exactly M reals, W = 2^(LOGE - LOGM) rows per work-group taken from
+// one (c, z) plane (ny % W == 0), so a group covers rows y0 .. y0 + W - 1 of one plane and every
+// lane coordinate is a bit field of its index.  Group g = (c nz + z) (ny / W) + y0 / W.
+// Packed input pairs (x[2n], x[2n+1]), n < M / 2, are single 16-byte loads; n >= M / 2 is the
+// zero padding.  Spectrum rows are written and read as in the generic kernels: column k of row y
+// at tcol(k, n1T) + 4 y, so the W rows of a column tile form 4 W contiguous elements.
+template <class C, int LOGM, int LOGE, int BS, int MINW>
+__global__ void __launch_bounds__(BS, MINW)
+    k_fft_x_fwd_ct(const double* __restrict__ in, const double* __restrict__ ms, double scale, int ny, FastDiv gyd,
+                   FastDiv nzd, const C* __restrict__ twh, const C* __restrict__ twL, C* __restrict__ out,
+                   int64_t plane, int64_t n2o, int n1T, int64_t N) {
+    using T = typename real_of<C>::type;
+    constexpr int M = 1 << LOGM, LOGW = LOGE - LOGM, W = 1 << LOGW, E = 1 << LOGE, MH = M / 2;
+    static_assert(LOGW >= 0 && E % BS == 0 && (W * MH) % BS == 0, "whole lane loops");
+    extern __shared__ unsigned char smem_raw[];
+    C* buf = reinterpret_cast<C*>(smem_raw);
+    C* tws = buf + lds_padded(E);
+    for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
+    const uint32_t grp = blockIdx.x, zc = fdiv(grp, gyd), yg = grp - zc * gyd.d;
+    const uint32_t c = fdiv(zc, nzd), z = zc - c * nzd.d;
+    const int y0 = static_cast<int>(yg) << LOGW;
+    const int64_t cell0 = (static_cast<int64_t>(z) * ny + y0) << LOGM; // cell of (x = 0, y0, z)
+    const double* src = in + static_cast<int64_t>(c) * N + cell0;
+    const int tid = static_cast<int>(threadIdx.x);
+    constexpr int IT = W * MH / BS;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int g = tid + i * BS, w = g >> (LOGM - 1), n = g & (MH - 1);
+        const int o = (w << LOGM) + 2 * n;
+        const double2 p = *reinterpret_cast<const double2*>(src + o);
+        T re, im;
+        if (ms) {
+            const double2 q = *reinterpret_cast<const double2*>(ms + cell0 + o);
+            re = static_cast<T>(p.x * scale * q.x);
+            im = static_cast<T>(p.y * scale * q.y);
+        } else {
+            re = static_cast<T>(p.x * scale);
+            im = static_cast<T>(p.y * scale);
+        }
+        buf[lds_pad((n << LOGW) + w)] = C{re, im};
+        buf[lds_pad(((n + MH) << LOGW) + w)] = C{0, 0};
+    }
+    __syncthreads();
+    ct_fft<C, LOGM, LOGW, BS, false>(buf, tws);
+    // X[k] = E + w_L^k O, E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2, k <= M
+    const T half = static_cast<T>(0.5);
+    C* dst = out + (static_cast<int64_t>(c) * n2o + z) * plane + kTile * y0;
+    constexpr int NOUT = (M / kTile + 1) * W * kTile, OT = (NOUT + BS - 1) / BS;
+#pragma unroll
+    for (int i = 0; i < OT; ++i) {
+        const int g = tid + i * BS;
+        if (g < NOUT) {
+            const int kk = g & 3, w = (g >> 2) & (W - 1), kt = g >> (LOGW + 2), k = kt * kTile + kk;
+            if (k <= M) {
+                const C Zk = buf[lds_pad(((k & (M - 1)) << LOGW) + w)];
+                const C Zm = buf[lds_pad((((M - k) & (M - 1)) << LOGW) + w)];
+                const C Ev{half * (Zk.x + Zm.x), half * (Zk.y - Zm.y)};
+                const C Ov{half * (Zk.y + Zm.y), -half * (Zk.x - Zm.x)};
+                dst[kt * n1T + kTile * w + kk] = cadd(Ev, cmul(twL[k], Ov));
+            }
+        }
+    }
+}
+
+template <class C, class T, int LOGM, int LOGE, int BS, int MINW>
+__global__ void __launch_bounds__(BS, MINW)
+    k_fft_x_inv_ct(const C* __restrict__ spec, int64_t plane, int64_t n2s, int n1T, int ny, FastDiv gyd, FastDiv nzd,
+                   const C* __restrict__ twh, const C* __restrict__ twL, T* __restrict__ out, int64_t N, double hscale) {
+    constexpr int M = 1 << LOGM, LOGW = LOGE - LOGM, W = 1 << LOGW, E = 1 << LOGE, MH = M / 2;
+    static_assert(LOGW >= 0 && E % BS == 0 && (W * MH) % BS == 0, "whole lane loops");
+    extern __shared__ unsigned char smem_raw[];
+    C* buf = reinterpret_cast<C*>(smem_raw);
+    C* tws = buf + lds_padded(E);
+    for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
+    const uint32_t grp = blockIdx.x, zc = fdiv(grp, gyd), yg = grp - zc * gyd.d;
+    const uint32_t c = fdiv(zc, nzd), z = zc - c * nzd.d;
+    const int y0 = static_cast<int>(yg) << LOGW;
+    const C* srow = spec + (static_cast<int64_t>(c) * n2s + z) * plane + kTile * y0;
+    const int tid = static_cast<int>(threadIdx.x);
+    // Zp[k] = c2r_combine(X[k], X[M - k]), k < M: g = (kt W + w) 4 + kk
+    constexpr int IT = E / BS;
+    C a[IT], b[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int g = tid + i * BS;
+        const int kk = g & 3, w = (g >> 2) & (W - 1), kt = g >> (LOGW + 2), m = M - (kt * kTile + kk);
+        a[i] = srow[kt * n1T + kTile * w + kk];
+        b[i] = srow[(m >> 2) * n1T + kTile * w + (m & 3)];
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int g = tid + i * BS;
+        const int kk = g & 3, w = (g >> 2) & (W - 1), kt = g >> (LOGW + 2), k = kt * kTile + kk;
+        buf[lds_pad((k << LOGW) + w)] = c2r_combine(a[i], b[i], twL[k]);
+    }
+    __syncthreads();
+    ct_fft<C, LOGM, LOGW, BS, true>(buf, tws);
+    // x[2n] + i x[2n+1] = Z[n], n < M / 2 (x < nx = M), scaled by 1 / P
+    T* orow = out + static_cast<int64_t>(c) * N + ((static_cast<int64_t>(z) * ny + y0) << LOGM);
+    using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+    constexpr int OT = W * MH / BS;
+#pragma unroll
+    for (int i = 0; i < OT; ++i) {
+        const int g = tid + i * BS, w = g >> (LOGM - 1), n = g & (MH - 1);
+        const C v = buf[lds_pad((n << LOGW) + w)];
+        *reinterpret_cast<T2*>(orow + (w << LOGM) + 2 * n) =
+            T2{static_cast<T>(static_cast<double>(v.x) * hscale), static_cast<T>(static_cast<double>(v.y) * hscale)};
     }
 }
 
@@ -673,6 +785,32 @@ void fft_x_forward(mafx_ctx* ctx, const double* in, const double* ms, double sca
     const FftPlan& ph = ctx->pxh;
     const int M = ph.L;
     require(M <= lds_elems<C>(), "x transform length exceeds the LDS budget");
+    // compile-time shapes for the demag input (rows of nx = M reals, 256 <= M <= 2048): 32 KiB tiles
+    // of whole row groups of one plane, 512 lanes under a 6-wave register cap
+    constexpr int LOGE = sizeof(C) == sizeof(float2) ? 12 : 11;
+    if (ncomp == 3 && Xi == M && Yi == Yr && Zi == Zr && ctx->n0 == 2 * M && M >= 256 && M <= 2048 &&
+        (M & (M - 1)) == 0 && Yr % std::max(1, (1 << LOGE) / M) == 0) {
+        using KC = void (*)(const double*, const double*, double, int, FastDiv, FastDiv, const C*, const C*, C*, int64_t,
+                            int64_t, int, int64_t);
+        KC fc = M == 256 ? k_fft_x_fwd_ct<C, 8, LOGE, 512, 6>
+                : M == 512 ? k_fft_x_fwd_ct<C, 9, LOGE, 512, 6>
+                : M == 1024 ? k_fft_x_fwd_ct<C, 10, LOGE, 512, 6>
+                            : k_fft_x_fwd_ct<C, 11, LOGE, 512, 6>;
+        const int W = std::max(1, (1 << LOGE) / M);
+        const int64_t groups = 3 * Zr * (Yr / W);
+        require(groups < (int64_t{1} << 31) && kTile * ctx->n1 * (M / kTile + 1) < (int64_t{1} << 31),
+                "x transform: too many rows");
+        const double bytes = static_cast<double>(3 * Zr * Yr) * (Xi * 8.0 + ctx->hx * sizeof(C));
+        ProfScope ps(ctx, nm("fft_x_fwd_f32", "fft_x_fwd_f64", sizeof(C)), bytes);
+        const size_t lds = (lds_padded(1 << LOGE) + M) * sizeof(C);
+        allow_lds(reinterpret_cast<const void*>(fc), lds);
+        hipLaunchKernelGGL(fc, dim3(static_cast<unsigned>(groups)), dim3(512), lds, ctx->stream, in, ms, scale,
+                           static_cast<int>(Yr), make_fastdiv(static_cast<uint32_t>(Yr / W)),
+                           make_fastdiv(static_cast<uint32_t>(Zr)), tw_of<C>(ph), tw_of<C>(ctx->px), out,
+                           ctx->n1 * ctx->hxs, n2o, static_cast<int>(kTile * ctx->n1), Xi * Yi * Zi);
+        MAFX_HIP(hipGetLastError());
+        return;
+    }
     // Power-of-two rows that fit use the 32 KiB tile under a 6-wave register cap (three resident
     // work-groups per CU), measured faster than the 64 KiB tile at 1024 x 1024 x 4 in both precisions.
     constexpr int E64 = lds_elems<C>();
@@ -702,6 +840,31 @@ void fft_x_inverse(mafx_ctx* ctx, const C* spec, int64_t n2s, T* out, double hsc
     const int M = ph.L;
     const mafx_mesh& g = ctx->mesh;
     require(M <= lds_elems<C>(), "x transform length exceeds the LDS budget");
+    // compile-time shapes, as in fft_x_forward
+    constexpr int LOGE = sizeof(C) == sizeof(float2) ? 12 : 11;
+    if (g.nx == M && ctx->n0 == 2 * M && M >= 256 && M <= 2048 && (M & (M - 1)) == 0 &&
+        g.ny % std::max(1, (1 << LOGE) / M) == 0) {
+        using KC = void (*)(const C*, int64_t, int64_t, int, int, FastDiv, FastDiv, const C*, const C*, T*, int64_t,
+                            double);
+        KC fc = M == 256 ? k_fft_x_inv_ct<C, T, 8, LOGE, 512, 6>
+                : M == 512 ? k_fft_x_inv_ct<C, T, 9, LOGE, 512, 6>
+                : M == 1024 ? k_fft_x_inv_ct<C, T, 10, LOGE, 512, 6>
+                            : k_fft_x_inv_ct<C, T, 11, LOGE, 512, 6>;
+        const int W = std::max(1, (1 << LOGE) / M);
+        const int64_t groups = 3 * g.nz * (g.ny / W);
+        require(groups < (int64_t{1} << 31) && kTile * ctx->n1 * (M / kTile + 1) < (int64_t{1} << 31),
+                "x transform: too many rows");
+        const double bytes = static_cast<double>(3 * g.nz * g.ny) * (ctx->hx * sizeof(C) + g.nx * sizeof(T));
+        ProfScope ps(ctx, nm("fft_x_inv_f32", "fft_x_inv_f64", sizeof(C)), bytes);
+        const size_t lds = (lds_padded(1 << LOGE) + M) * sizeof(C);
+        allow_lds(reinterpret_cast<const void*>(fc), lds);
+        hipLaunchKernelGGL(fc, dim3(static_cast<unsigned>(groups)), dim3(512), lds, ctx->stream, spec, ctx->n1 * ctx->hxs,
+                           n2s, static_cast<int>(kTile * ctx->n1), static_cast<int>(g.ny),
+                           make_fastdiv(static_cast<uint32_t>(g.ny / W)), make_fastdiv(static_cast<uint32_t>(g.nz)),
+                           tw_of<C>(ph), tw_of<C>(ctx->px), out, ctx->N, hscale);
+        MAFX_HIP(hipGetLastError());
+        return;
+    }
     constexpr int E64 = lds_elems<C>();
     const bool small = is_pow2_plan(desc_of(ph)) && M <= E64 / 2; // as in fft_x_forward
     const int W = lines_per_block(M, small ? E64 / 2 : E64);

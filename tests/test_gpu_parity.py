@@ -111,15 +111,37 @@ def test_heff_fp64_matches_oracle(mesh, which):
 
 
 @pytest.mark.parametrize("mesh", [(8, 256, 2, 2e-9, 1e-9, 2e-9), (5, 512, 3, 1e-9, 1e-9, 1e-9),
-                                  (4, 1024, 1, 2e-9, 2e-9, 3e-9), (3, 1024, 4, 2e-9, 2e-9, 2e-9)])
-def test_heff_fp64_pow2_columns(mesh):
-    """Power-of-two ny of 256..1024: the y transforms run the compile-time shaped kernels."""
-    print(f"\n[heff fp64 pow2 y] mesh={mesh}", flush=True)
+                                  (4, 1024, 1, 2e-9, 2e-9, 3e-9), (3, 1024, 4, 2e-9, 2e-9, 2e-9),
+                                  (256, 8, 2, 1e-9, 2e-9, 2e-9), (512, 4, 1, 2e-9, 2e-9, 3e-9),
+                                  (1024, 2, 3, 2e-9, 1e-9, 2e-9), (2048, 1, 1, 2e-9, 2e-9, 2e-9),
+                                  (256, 256, 1, 3e-9, 3e-9, 3e-9)])
+def test_heff_fp64_pow2_axes(mesh):
+    """Power-of-two nx of 256..2048 or ny of 256..1024: the x or y transforms run the
+    compile-time shaped kernels (row groups of one plane, whole column tiles)."""
+    print(f"\n[heff fp64 pow2 axes] mesh={mesh}", flush=True)
     m = rand_m(mesh[:3], 21)
     state, terms, sim = build(mesh, 1.1e6, m, MICRO_ALL)
     M = mf()
     h = M.LLGIntegrator(0.1, terms).H_in_Apm(state)
     assert normwise(h, sim.heff()) <= FP64_TOL, normwise(h, sim.heff())
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", FP64_TOL), ("f32", 1e-5)])
+def test_ms_field_airbox_pow2_x(precision, tol):
+    """Ms_field scaling of the packed x input on the compile-time shaped x kernels (nx = 256)."""
+    nx, ny, nz = 256, 16, 2
+    m = np.zeros((nx, ny, nz, 3))
+    m[64:192, 4:12, :, 0] = 1
+    m[64:192, 4:12, :, 1] = 0.5
+    mesh = (nx, ny, nz, 2e-9, 2e-9, 2e-9)
+    print(f"\n[Ms_field pow2 x] mesh={mesh} {precision}", flush=True)
+    M = mf()
+    g = M.Mesh(*mesh)
+    state = M.State(g, 1e5, m)
+    assert state.Ms_field is not None
+    sim = O.Sim(mesh, 1e5, m, [O.term(O.DEMAG), O.term(O.EXCHANGE, 1e-11)])
+    llg = M.LLGIntegrator(0.1, [M.DemagField(g), M.ExchangeField(1e-11)], precision=precision)
+    assert normwise(llg.H_in_Apm(state), sim.heff()) <= tol
 
 
 def test_heff_each_term_energy_and_zeeman():
@@ -181,6 +203,7 @@ def test_demag_cube_energy_gpu():
 # transforms), power-of-two y lines from 1 to 1024 and odd and prime sizes.  Tolerance: FP32
 # rounding of the FFTs and stencils.
 FP32_TOL = 1e-5
+FP32_TOL_LONG_ROWS = 3e-5
 
 
 @pytest.mark.parametrize("mesh", [(16, 8, 4, 2e-9, 2e-9, 2e-9), (64, 32, 1, 3e-9, 3e-9, 3e-9),
@@ -190,6 +213,8 @@ FP32_TOL = 1e-5
                                   (7, 3, 1, 2e-9, 1e-9, 3e-9), (33, 7, 5, 1e-9, 2e-9, 1.5e-9),
                                   (12, 6, 10, 2e-9, 2e-9, 2e-9), (20, 9, 8, 1e-9, 1e-9, 1e-9),
                                   (6, 512, 2, 2e-9, 1e-9, 2e-9), (12, 256, 1, 2e-9, 2e-9, 3e-9),
+                                  (256, 16, 2, 1e-9, 2e-9, 2e-9), (512, 8, 1, 2e-9, 2e-9, 3e-9),
+                                  (1024, 4, 2, 2e-9, 1e-9, 2e-9), (2048, 2, 1, 2e-9, 2e-9, 2e-9),
                                   (1, 1, 1, 2e-9, 2e-9, 2e-9)])
 def test_heff_fp32_field_error(mesh):
     print(f"\n[heff fp32] mesh={mesh}", flush=True)
@@ -200,10 +225,13 @@ def test_heff_fp32_field_error(mesh):
     err = normwise(h32, sim.heff())
     print(f"[fp32 field] mesh={mesh} normwise error={err:.3e}", flush=True)
     assert err <= FP32_TOL
-    # the demag term alone, where the symmetric tensor spectrum acts
+    # the demag term alone, where the symmetric tensor spectrum acts.  Rows of 2048 cells (4096
+    # padded points) strips carry more FP32 rounding relative to their small demag field: 1.4e-5
+    # at 2048x2x1 and 2.5e-5 at 2048x4x1, the same to the last bit on the generic (run-time shaped)
+    # and the compile-time shaped x kernels
     hd = M.LLGIntegrator(0.1, [terms[0]], precision="f32").H_in_Apm(state)
     err_d = normwise(hd, sim.heff(mask=1))
-    assert err_d <= FP32_TOL, err_d
+    assert err_d <= (FP32_TOL if mesh[0] < 2048 else FP32_TOL_LONG_ROWS), err_d
 
 
 def test_fp32_ms_field_airbox():
