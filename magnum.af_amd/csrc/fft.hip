@@ -470,25 +470,35 @@ __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
 // Compile-time shaped forms of k_fft_y_fwd2 / k_fft_y_inv2 for power-of-two M = 2^LOGM with
 // 2^LOGE complex values per LDS tile: W = 2^(LOGE - LOGM - 1) >= 4 columns, i.e. W / 4 whole
 // column tiles, per work-group.  Row y of column kx0 + w sits at tile w / 4, offset 4 y + w % 4,
-// and a tile spans n1T = 4 n1 = 8 M elements, so a lane's global index g walks the tiles'
-// contiguous storage and every coordinate is a bit field of g.  Columns kx >= hx (the padding of
-// the last tile group) are read as zero and not written.
+// and a tile spans n1T = 4 n1 = 8 M elements.  Line l = odd W + w of the LDS tile is column w
+// (odd = 0: x[n], outputs X[2k]; odd = 1: x[n] w_L^n, outputs X[2k+1]).
+// The last stage writes its outputs straight from registers: butterfly b = j W2 + l holds
+// positions j + r Ns, and ky = 2 pos + odd sits at offset 8 pos + 4 odd + w % 4 of its tile, so
+// consecutive lanes write consecutive addresses.  In the inverse the first stage likewise reads
+// its inputs (rows j + r M / R of line l) straight from global memory.  (The forward reads its
+// input through LDS: both lines of a column take the same x[n], so register loads per butterfly
+// would fetch every element twice.)  Columns kx >= hx (the padding of the last tile group) are
+// read as zero and not written.
 template <class C, int LOGM, int LOGE, int BS, int MINW>
 __global__ void __launch_bounds__(BS, MINW)
     k_fft_y_fwd_ct(C* __restrict__ data, int64_t sa, int64_t sb, int hx, const C* __restrict__ twh,
                    const C* __restrict__ twL) {
-    constexpr int M = 1 << LOGM, LOGW2 = LOGE - LOGM, W2 = 1 << LOGW2, W = W2 / 2, E = 1 << LOGE;
-    constexpr int n1T = 8 * M; // tile stride; offsets inside a work-group's tiles stay below 2 E
-    static_assert(W >= kTile && E % BS == 0, "whole column tiles, whole lane loops");
+    constexpr int M = 1 << LOGM, LOGW2 = LOGE - LOGM, W2 = 1 << LOGW2, LOGW = LOGW2 - 1, W = W2 / 2;
+    constexpr int n1T = 8 * M; // tile stride; offsets inside a work-group's tiles stay below 2^LOGE
+    constexpr int NST = ct_nstages(LOGM);
+    using S0 = CtStage<LOGM, LOGW2, 0>;
+    using SL = CtStage<LOGM, LOGW2, NST - 1>;
+    static_assert(W >= kTile && NST >= 2 && S0::NB % BS == 0 && SL::NB % BS == 0, "tile shape");
     extern __shared__ unsigned char smem_raw[];
     C* buf = reinterpret_cast<C*>(smem_raw);
-    C* tws = buf + lds_padded(E);
+    C* tws = buf + lds_padded(1 << LOGE);
     for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
     const uint3 blk = swizzled_block();
     const int kx0 = static_cast<int>(blk.x) * W;
     C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + (kx0 / kTile) * n1T;
     const int tid = static_cast<int>(threadIdx.x);
-    // inputs y < M of the W columns: g = (tile * M + y) * 4 + w % 4
+    // inputs y < M of the W columns, contiguous: g = (tile * M + y) * 4 + w % 4; both lines of
+    // column w are filled (x and x w_L^y) so the even and odd outputs share one pass
     constexpr int IT = M * W / BS;
     C v[IT], t[IT];
 #pragma unroll
@@ -506,60 +516,87 @@ __global__ void __launch_bounds__(BS, MINW)
         buf[lds_pad((y << LOGW2) + W + w)] = cmul(v[i], t[i]);
     }
     __syncthreads();
-    ct_fft<C, LOGM, LOGW2, BS, false>(buf, tws);
-    // outputs ky = 2 k + odd of every column: g = (tile * 2M + ky) * 4 + w % 4
-    constexpr int OT = 2 * M * W / BS;
+    ct_fft<C, LOGM, LOGW2, BS, false, 0, NST - 1>(buf, tws);
+    constexpr int NBL = SL::NB / BS;
 #pragma unroll
-    for (int i = 0; i < OT; ++i) {
-        const int g = tid + i * BS;
-        const int tl = g >> (LOGM + 3), r = g & (8 * M - 1), k = r >> 3, odd = (r >> 2) & 1;
-        const int w = (tl << 2) | (g & 3);
-        if (kx0 + w < hx) base[tl * n1T + r] = buf[lds_pad((k << LOGW2) + odd * W + w)];
+    for (int i = 0; i < NBL; ++i) {
+        const int b = tid + i * BS, j = b >> LOGW2, l = b & (W2 - 1), w = l & (W - 1), odd = l >> LOGW;
+        C u[SL::R];
+#pragma unroll
+        for (int r = 0; r < SL::R; ++r) u[r] = buf[lds_pad(b + r * SL::NB)];
+        SL::template bfly<C, false>(u, b, tws);
+        if (kx0 + w < hx) {
+            C* dst = base + (w >> 2) * n1T + 4 * odd + (w & 3);
+#pragma unroll
+            for (int r = 0; r < SL::R; ++r) dst[8 * (j + r * SL::NS)] = u[r];
+        }
     }
 }
 
+template <class C> __device__ __forceinline__ C shfl_xor_c(C v, int mask) {
+    return C{__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64)};
+}
+
+// Inverse counterpart.  The last stage leaves IFFT_M(X_even)[n] and IFFT_M(X_odd)[n] of column w
+// in lanes b and b ^ W of one wave (same j, lines w and W + w); the pair trades half of its
+// outputs through a lane shuffle and each lane writes x[n] = even[n] + w_L^-n odd[n] for half of
+// the positions.
 template <class C, int LOGM, int LOGE, int BS, int MINW>
 __global__ void __launch_bounds__(BS, MINW)
     k_fft_y_inv_ct(C* __restrict__ data, int64_t sa, int64_t sb, int hx, const C* __restrict__ twh,
                    const C* __restrict__ twL) {
-    constexpr int M = 1 << LOGM, LOGW2 = LOGE - LOGM, W2 = 1 << LOGW2, W = W2 / 2, E = 1 << LOGE;
-    constexpr int n1T = 8 * M; // tile stride; offsets inside a work-group's tiles stay below 2 E
-    static_assert(W >= kTile && E % BS == 0, "whole column tiles, whole lane loops");
+    constexpr int M = 1 << LOGM, LOGW2 = LOGE - LOGM, W2 = 1 << LOGW2, LOGW = LOGW2 - 1, W = W2 / 2;
+    constexpr int n1T = 8 * M;
+    constexpr int NST = ct_nstages(LOGM);
+    using S0 = CtStage<LOGM, LOGW2, 0>;
+    using SL = CtStage<LOGM, LOGW2, NST - 1>;
+    static_assert(W >= kTile && W2 <= 64 && NST >= 2 && S0::NB % BS == 0 && SL::NB % BS == 0 && SL::R >= 2,
+                  "tile shape");
     extern __shared__ unsigned char smem_raw[];
     C* buf = reinterpret_cast<C*>(smem_raw);
-    C* tws = buf + lds_padded(E);
+    C* tws = buf + lds_padded(1 << LOGE);
     for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
     const uint3 blk = swizzled_block();
     const int kx0 = static_cast<int>(blk.x) * W;
     C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + (kx0 / kTile) * n1T;
     const int tid = static_cast<int>(threadIdx.x);
-    // inputs: every ky = 2 k + odd; X[2k] goes to line w, X[2k+1] to line W + w
-    constexpr int IT = 2 * M * W / BS;
-    C v[IT];
+    constexpr int NB0 = S0::NB / BS;
+    C v0[NB0][S0::R];
 #pragma unroll
-    for (int i = 0; i < IT; ++i) {
-        const int g = tid + i * BS;
-        const int tl = g >> (LOGM + 3), w = (tl << 2) | (g & 3);
-        v[i] = kx0 + w < hx ? base[tl * n1T + (g & (8 * M - 1))] : C{0, 0};
+    for (int i = 0; i < NB0; ++i) {
+        const int b = tid + i * BS, j = b >> LOGW2, l = b & (W2 - 1), w = l & (W - 1), odd = l >> LOGW;
+        const C* src = base + (w >> 2) * n1T + 4 * odd + (w & 3);
+        const bool ok = kx0 + w < hx;
+#pragma unroll
+        for (int r = 0; r < S0::R; ++r) v0[i][r] = ok ? src[8 * (j + r * (M / S0::R))] : C{0, 0};
     }
 #pragma unroll
-    for (int i = 0; i < IT; ++i) {
-        const int g = tid + i * BS;
-        const int tl = g >> (LOGM + 3), r = g & (8 * M - 1), k = r >> 3, odd = (r >> 2) & 1;
-        const int w = (tl << 2) | (g & 3);
-        buf[lds_pad((k << LOGW2) + odd * W + w)] = v[i];
+    for (int i = 0; i < NB0; ++i) {
+        const int b = tid + i * BS;
+        S0::template bfly<C, true>(v0[i], b, tws);
+#pragma unroll
+        for (int r = 0; r < S0::R; ++r) buf[lds_pad((S0::opos(b, r) << LOGW2) + (b & (W2 - 1)))] = v0[i][r];
     }
     __syncthreads();
-    ct_fft<C, LOGM, LOGW2, BS, true>(buf, tws);
-    // x[y] = IFFT_M(X_even)[y] + w_L^-y IFFT_M(X_odd)[y], y < M
-    constexpr int OT = M * W / BS;
+    ct_fft<C, LOGM, LOGW2, BS, true, 1, NST - 1>(buf, tws);
+    constexpr int NBL = SL::NB / BS, H = SL::R / 2;
 #pragma unroll
-    for (int i = 0; i < OT; ++i) {
-        const int g = tid + i * BS;
-        const int tl = g >> (LOGM + 2), y = (g >> 2) & (M - 1), w = (tl << 2) | (g & 3);
-        if (kx0 + w < hx)
-            base[tl * n1T + (g & (4 * M - 1))] = cadd(buf[lds_pad((y << LOGW2) + w)],
-                                                      cmul(conj_if(twL[y], true), buf[lds_pad((y << LOGW2) + W + w)]));
+    for (int i = 0; i < NBL; ++i) {
+        const int b = tid + i * BS, j = b >> LOGW2, l = b & (W2 - 1), w = l & (W - 1), odd = l >> LOGW;
+        C v[SL::R];
+#pragma unroll
+        for (int r = 0; r < SL::R; ++r) v[r] = buf[lds_pad(b + r * SL::NB)];
+        SL::template bfly<C, true>(v, b, tws);
+        const bool ok = kx0 + w < hx;
+        C* dst = base + (w >> 2) * n1T + (w & 3);
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+            // the even lane keeps positions q, the odd lane q + H
+            const C recv = shfl_xor_c(odd ? v[q] : v[q + H], W);
+            const int n = j + (odd ? q + H : q) * SL::NS;
+            const C e = odd ? recv : v[q], o = odd ? v[q + H] : recv;
+            if (ok) dst[4 * n] = cadd(e, cmul(conj_if(twL[n], true), o));
+        }
     }
 }
 

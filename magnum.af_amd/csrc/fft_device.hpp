@@ -245,13 +245,35 @@ __device__ __forceinline__ void ct_stage(C* buf, const C* __restrict__ tw) {
 }
 
 // All stages of W2 lines of length M held in buf[pos * W2 + line]; tw = w_M^t, t < M.
-template <class C, int LOGM, int LOGW2, int BS, bool INV, int S = 0>
+template <class C, int LOGM, int LOGW2, int BS, bool INV, int S = 0, int SEND = ct_nstages(LOGM)>
 __device__ __forceinline__ void ct_fft(C* buf, const C* __restrict__ tw) {
-    if constexpr (S < ct_nstages(LOGM)) {
+    if constexpr (S < SEND) {
         ct_stage<C, LOGM, LOGW2, ct_logr(LOGM, S), ct_logns(LOGM, S), BS, INV>(buf, tw);
-        ct_fft<C, LOGM, LOGW2, BS, INV, S + 1>(buf, tw);
+        ct_fft<C, LOGM, LOGW2, BS, INV, S + 1, SEND>(buf, tw);
     }
 }
+
+// Pieces of stage S for kernels that keep its inputs or outputs in registers.  Butterfly b of
+// stage S covers line b % W2 and index j = b / W2 < M / R; its inputs are positions j + r M / R,
+// and its outputs go to positions ct_opos(b, r) of the same line.
+template <int LOGM, int LOGW2, int S> struct CtStage {
+    static constexpr int LOGR = ct_logr(LOGM, S), LOGNS = ct_logns(LOGM, S);
+    static constexpr int R = 1 << LOGR, NS = 1 << LOGNS;
+    static constexpr int NB = (1 << (LOGM - LOGR)) << LOGW2;
+    __device__ static __forceinline__ int opos(int b, int r) {
+        const int j = b >> LOGW2, k = j & (NS - 1);
+        return ((j - k) << LOGR) + k + (r << LOGNS);
+    }
+    template <class C, bool INV>
+    __device__ static __forceinline__ void bfly(C (&v)[R], int b, const C* __restrict__ tw) {
+        if constexpr (NS > 1) {
+            const int k = (b >> LOGW2) & (NS - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], conj_if(tw[(r * k) << (LOGM - LOGNS - LOGR)], INV));
+        }
+        dft<C, R>(v, INV, tw, 1 << LOGM);
+    }
+};
 
 inline bool is_pow2_plan(const FftDesc& d) {
     for (int s = 0; s < d.nstages; ++s)

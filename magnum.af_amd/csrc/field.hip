@@ -14,6 +14,7 @@
 //   include/magnumaf/equations.hpp:19-33, solvers/lbfgs_minimizer.cpp:33-42.
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -106,8 +107,28 @@ __device__ void block_reduce_store(double* vals, int nv, const int* is_max, doub
 // acc[] collects the energy partials (slots 0..nterms-1), acc[nterms] = max |m x H|,
 // acc[nterms+1] = max |H| (gradient mode); without RED acc is not touched, which keeps the
 // accumulators out of the register budget of the plain field and LLG evaluations.
-template <class T, bool RED>
-__device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t z, const T hd[3], double* acc) {
+// PROG = 0 reads the term kinds from P at run time; otherwise it lists them at compile time
+// (prog_of: count in bits 0-3, kind of term t in bits 4 + 4t ..), so the term loop unrolls and
+// each term compiles to its own branch-free code.
+constexpr uint64_t prog_of(std::initializer_list<int> kinds) {
+    uint64_t p = kinds.size();
+    int t = 0;
+    for (int k : kinds) p |= static_cast<uint64_t>(k) << (4 + 4 * t++);
+    return p;
+}
+constexpr int prog_count(uint64_t prog) { return static_cast<int>(prog & 15u); }
+constexpr int prog_kind(uint64_t prog, int t) { return static_cast<int>((prog >> (4 + 4 * t)) & 15u); }
+
+template <int I, int N, class F> __device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+template <class T, bool RED, uint64_t PROG = 0>
+__device__ __forceinline__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t z, const T hd[3],
+                                              double* acc) {
     const int64_t N = P.N;
     const int64_t i = x + P.nx * (y + P.ny * z);
     const double* mp[3] = {P.m, P.m + N, P.m + 2 * N};
@@ -140,7 +161,7 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
     }
     const double msi = P.ms ? P.ms[i] : P.Ms;
     T H0 = 0, H1 = 0, H2 = 0;
-    for (int t = 0; t < P.nterms; ++t) {
+    auto term = [&](int t, int kind) __attribute__((always_inline)) {
         const EpiTerm& tm = P.t[t];
         T h0 = 0, h1 = 0, h2 = 0;
         T e_extra = 0; // exchange energy boundary correction (h_withedges - h) . m
@@ -154,7 +175,7 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
             c = static_cast<T>(tm.num / (kMu0 * msi));
             return true;
         };
-        switch (tm.kind) {
+        switch (kind) {
         case MAFX_DEMAG:
         case MAFX_ATOM_DIPOLE:
             h0 = hd[0];
@@ -192,7 +213,7 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
             T c;
             bool ok;
             T cx, cy, cz;
-            if (tm.kind == MAFX_DMI) {
+            if (kind == MAFX_DMI) {
                 ok = micro_coef(c);
                 cx = static_cast<T>(P.dmi_c[0]);
                 cy = static_cast<T>(P.dmi_c[1]);
@@ -315,10 +336,15 @@ __device__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t 
             H2 = H2 + h2;
         }
         if (RED && P.want_energy) {
-            const double w = (P.ms && tm.kind < MAFX_ATOM_DIPOLE) ? msi : 1.0;
+            const double w = (P.ms && kind < MAFX_ATOM_DIPOLE) ? msi : 1.0;
             acc[t] += w * (static_cast<double>(h0) * mc0 + static_cast<double>(h1) * mc1 +
                            static_cast<double>(h2) * mc2 + static_cast<double>(e_extra));
         }
+    };
+    if constexpr (prog_count(PROG) > 0) {
+        static_for<0, prog_count(PROG)>([&](auto tc) { term(tc.value, prog_kind(PROG, tc.value)); });
+    } else {
+        for (int t = 0; t < P.nterms; ++t) term(t, P.t[t].kind);
     }
     const double Hx = static_cast<double>(H0), Hy = static_cast<double>(H1), Hz = static_cast<double>(H2);
     switch (P.mode) {
@@ -386,7 +412,7 @@ template <bool RED> __device__ void finish_block(const EpiParams& P, double* acc
 // tile are read behind the same L2; otherwise cells are taken in memory order.
 // MINW is the waves per SIMD the register budget must allow: 5 (96 VGPRs, no spills) for the
 // FP32 field and LLG evaluations, which run 7 times per RK attempt; 1 (uncapped) otherwise.
-template <class T, bool RED, bool ROWS, int MINW>
+template <class T, bool RED, bool ROWS, int MINW, uint64_t PROG = 0>
 __global__ void __launch_bounds__(kBlock, MINW) k_epi_cells(EpiParams P, FastDiv nxd, FastDiv nyd, FastDiv nzd,
                                                             FastDiv bprd) {
     double acc[kMaxTerms + 2];
@@ -418,7 +444,7 @@ __global__ void __launch_bounds__(kBlock, MINW) k_epi_cells(EpiParams P, FastDiv
             hd[1] = ldb(h + P.N, ob);
             hd[2] = ldb(h + 2 * P.N, ob);
         }
-        cell_epilogue<T, RED>(P, x, y, z, hd, acc);
+        cell_epilogue<T, RED, PROG>(P, x, y, z, hd, acc);
     }
     finish_block<RED>(P, acc);
 }
@@ -574,6 +600,20 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
         constexpr int MW = sizeof(T) == 4 ? 5 : 1;
         auto kern = red ? (rowmap ? k_epi_cells<T, true, true, 1> : k_epi_cells<T, true, false, 1>)
                         : (rowmap ? k_epi_cells<T, false, true, MW> : k_epi_cells<T, false, false, MW>);
+        // compile-time term programs of the common micromagnetic term lists (field and LLG
+        // evaluations); any other list runs the run-time term loop
+        if (!red) {
+            uint64_t prog = static_cast<uint64_t>(P.nterms);
+            for (int t = 0; t < P.nterms; ++t) prog |= static_cast<uint64_t>(P.t[t].kind) << (4 + 4 * t);
+            constexpr uint64_t kP1 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_DMI, MAFX_UNIAXIAL});
+            constexpr uint64_t kP2 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_ZEEMAN});
+            constexpr uint64_t kP3 = prog_of({MAFX_DEMAG, MAFX_RKKY, MAFX_DMI, MAFX_UNIAXIAL, MAFX_ZEEMAN});
+            constexpr uint64_t kP4 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_DMI, MAFX_UNIAXIAL, MAFX_ZEEMAN});
+            if (prog == kP1) kern = rowmap ? k_epi_cells<T, false, true, MW, kP1> : k_epi_cells<T, false, false, MW, kP1>;
+            if (prog == kP2) kern = rowmap ? k_epi_cells<T, false, true, MW, kP2> : k_epi_cells<T, false, false, MW, kP2>;
+            if (prog == kP3) kern = rowmap ? k_epi_cells<T, false, true, MW, kP3> : k_epi_cells<T, false, false, MW, kP3>;
+            if (prog == kP4) kern = rowmap ? k_epi_cells<T, false, true, MW, kP4> : k_epi_cells<T, false, false, MW, kP4>;
+        }
         hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nblocks)), dim3(kBlock), 0, ctx->stream, P,
                            make_fastdiv(static_cast<uint32_t>(g.nx)), make_fastdiv(static_cast<uint32_t>(g.ny)),
                            make_fastdiv(static_cast<uint32_t>(g.nz)), make_fastdiv(static_cast<uint32_t>(bpr)));
