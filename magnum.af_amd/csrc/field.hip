@@ -126,6 +126,10 @@ template <int I, int N, class F> __device__ __forceinline__ void static_for(F&& 
     }
 }
 
+template <class T, bool RED, uint64_t PROG>
+__device__ __forceinline__ void cell_terms(const EpiParams& P, int64_t x, int64_t y, int64_t z, double mc0,
+                                           double mc1, double mc2, const V3<T> (&nb)[6], const T hd[3], double* acc);
+
 template <class T, bool RED, uint64_t PROG = 0>
 __device__ __forceinline__ void cell_epilogue(const EpiParams& P, int64_t x, int64_t y, int64_t z, const T hd[3],
                                               double* acc) {
@@ -134,7 +138,6 @@ __device__ __forceinline__ void cell_epilogue(const EpiParams& P, int64_t x, int
     const double* mp[3] = {P.m, P.m + N, P.m + 2 * N};
     const uint32_t o8 = static_cast<uint32_t>(i) * 8u;
     const double mc0 = ldb(mp[0], o8), mc1 = ldb(mp[1], o8), mc2 = ldb(mp[2], o8);
-    const T m0 = static_cast<T>(mc0), m1 = static_cast<T>(mc1), m2 = static_cast<T>(mc2);
 
     // Neighbours with zero padding (af::convolve AF_CONV_DEFAULT semantics), x-, x+, y-, y+,
     // z-, z+.  A missing neighbour reads the centre cell and is then replaced by 0, so all 18
@@ -159,6 +162,18 @@ __device__ __forceinline__ void cell_epilogue(const EpiParams& P, int64_t x, int
 #pragma unroll
         for (int k = 0; k < 6; ++k) nb[k] = V3<T>{0, 0, 0};
     }
+    cell_terms<T, RED, PROG>(P, x, y, z, mc0, mc1, mc2, nb, hd, acc);
+}
+
+// The local terms and the output stage of cell (x, y, z) from its magnetisation mc (FP64), its six
+// neighbours nb (x-, x+, y-, y+, z-, z+, zero where the mesh ends) and the convolution field hd.
+template <class T, bool RED, uint64_t PROG>
+__device__ __forceinline__ void cell_terms(const EpiParams& P, int64_t x, int64_t y, int64_t z, double mc0,
+                                           double mc1, double mc2, const V3<T> (&nb)[6], const T hd[3], double* acc) {
+    const int64_t N = P.N;
+    const int64_t i = x + P.nx * (y + P.ny * z);
+    const uint32_t o8 = static_cast<uint32_t>(i) * 8u;
+    const T m0 = static_cast<T>(mc0), m1 = static_cast<T>(mc1), m2 = static_cast<T>(mc2);
     const double msi = P.ms ? P.ms[i] : P.Ms;
     T H0 = 0, H1 = 0, H2 = 0;
     auto term = [&](int t, int kind) __attribute__((always_inline)) {
@@ -449,6 +464,68 @@ __global__ void __launch_bounds__(kBlock, MINW) k_epi_cells(EpiParams P, FastDiv
     finish_block<RED>(P, acc);
 }
 
+// Per-cell stage with z-column lanes, for rows of at least one work-group (field and LLG
+// outputs, no energy sums).  Work-group g (after the XCD renumbering) takes cells
+// x0 .. x0 + kBlock - 1 of row y at every z, and lane t walks its column z = 0 .. nz - 1: the z
+// neighbours are the lane's previous and next cells, the x neighbours come from an LDS row of the
+// group's cells at the same z (one halo cell each side, loaded by the edge lanes), so only the
+// centre, the next z and the two y neighbours are read from memory, 9 loads per cell instead
+// of 21.  Zero padding as in cell_epilogue.
+template <class T, uint64_t PROG, int MINW>
+__global__ void __launch_bounds__(kBlock, MINW) k_epi_zcol(EpiParams P, FastDiv bprd) {
+    __shared__ V3<T> row[kBlock + 2];
+    const int nx = static_cast<int>(P.nx), ny = static_cast<int>(P.ny), nz = static_cast<int>(P.nz);
+    const int lb = static_cast<int>(xcd_swizzle(blockIdx.x, gridDim.x));
+    const int y = static_cast<int>(fdiv(lb, bprd)), xc = lb - y * static_cast<int>(bprd.d);
+    const int t = static_cast<int>(threadIdx.x), x0 = xc * kBlock, x = x0 + t;
+    const bool valid = x < nx;
+    const double* mp[3] = {P.m, P.m + P.N, P.m + 2 * P.N};
+    const uint32_t sy = static_cast<uint32_t>(nx) * 8u, sz = static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) * 8u;
+    const bool oky0 = y > 0, oky1 = y + 1 < ny;                  // uniform over the group
+    const bool hl = t == 0 && x0 > 0, hr = t == kBlock - 1 && x0 + kBlock < nx;
+    // byte offset of (x, y, z = 0); lanes past the row end read the row's last cell (unused)
+    uint32_t o = (static_cast<uint32_t>(y) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(valid ? x : nx - 1)) * 8u;
+    auto ld3 = [&](uint32_t off, double (&v)[3]) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = ldb(mp[c], off);
+    };
+    auto to_t = [](const double (&v)[3]) { return V3<T>{static_cast<T>(v[0]), static_cast<T>(v[1]), static_cast<T>(v[2])}; };
+    double cur[3];
+    ld3(o, cur);
+    V3<T> prev{0, 0, 0};
+    double acc[1] = {0};
+    for (int z = 0; z < nz; ++z) {
+        double nxt[3] = {0, 0, 0}, ym[3] = {0, 0, 0}, yp[3] = {0, 0, 0}, hv[3] = {0, 0, 0};
+        if (z + 1 < nz) ld3(o + sz, nxt);
+        if (oky0) ld3(o - sy, ym);
+        if (oky1) ld3(o + sy, yp);
+        if (hl) ld3(o - 8u, hv);
+        if (hr) ld3(o + 8u, hv);
+        T hd[3] = {0, 0, 0};
+        if (P.hd) {
+            const T* h = static_cast<const T*>(P.hd);
+            const uint32_t ob = (o / 8u) * static_cast<uint32_t>(sizeof(T));
+            hd[0] = ldb(h, ob);
+            hd[1] = ldb(h + P.N, ob);
+            hd[2] = ldb(h + 2 * P.N, ob);
+        }
+        __syncthreads(); // the previous z's row readers are done
+        row[t + 1] = valid ? to_t(cur) : V3<T>{0, 0, 0};
+        if (hl) row[0] = to_t(hv);
+        if (t == 0 && x0 == 0) row[0] = V3<T>{0, 0, 0};
+        if (hr) row[kBlock + 1] = to_t(hv);
+        __syncthreads();
+        if (valid) {
+            const V3<T> nb[6] = {row[t], (x + 1 < nx) ? row[t + 2] : V3<T>{0, 0, 0}, to_t(ym), to_t(yp), prev, to_t(nxt)};
+            cell_terms<T, false, PROG>(P, x, y, z, cur[0], cur[1], cur[2], nb, hd, acc);
+        }
+        prev = to_t(cur);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cur[c] = nxt[c];
+        o += sz;
+    }
+}
+
 // Fixed-order reduction of partials[s][0..nb) into out[s].
 __global__ void k_reduce_slots(const double* __restrict__ partials, int64_t nb, int nslots, int nsum,
                                double* __restrict__ out) {
@@ -613,6 +690,24 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
             if (prog == kP2) kern = rowmap ? k_epi_cells<T, false, true, MW, kP2> : k_epi_cells<T, false, false, MW, kP2>;
             if (prog == kP3) kern = rowmap ? k_epi_cells<T, false, true, MW, kP3> : k_epi_cells<T, false, false, MW, kP3>;
             if (prog == kP4) kern = rowmap ? k_epi_cells<T, false, true, MW, kP4> : k_epi_cells<T, false, false, MW, kP4>;
+            // z-column lanes for rows of whole work-groups when a term reads the neighbours
+            if (rowmap && P.needs_nb && g.nx % kBlock == 0) {
+                using KZ = void (*)(EpiParams, FastDiv);
+                // the run-time term loop and the RKKY list need up to 128 VGPRs without spills
+                constexpr int MWW = sizeof(T) == 4 ? 4 : 1;
+                KZ kz = k_epi_zcol<T, 0, MWW>;
+                if (prog == kP1) kz = k_epi_zcol<T, kP1, MW>;
+                if (prog == kP2) kz = k_epi_zcol<T, kP2, MW>;
+                if (prog == kP3) kz = k_epi_zcol<T, kP3, MWW>;
+                if (prog == kP4) kz = k_epi_zcol<T, kP4, MW>;
+                const int64_t ng = g.ny * bpr;
+                hipLaunchKernelGGL(kz, dim3(static_cast<unsigned>(ng)), dim3(kBlock), 0, ctx->stream, P,
+                                   make_fastdiv(static_cast<uint32_t>(bpr)));
+                MAFX_HIP(hipGetLastError());
+                ctx->scal_nblocks = nblocks;
+                ctx->scal_nterms = P.nterms;
+                return;
+            }
         }
         hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nblocks)), dim3(kBlock), 0, ctx->stream, P,
                            make_fastdiv(static_cast<uint32_t>(g.nx)), make_fastdiv(static_cast<uint32_t>(g.ny)),

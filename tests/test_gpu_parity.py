@@ -304,8 +304,33 @@ def test_rkky_layers_with_indices():
     assert normwise(terms[0].H_in_Apm(state), sim.heff()) <= FP64_TOL
 
 
+@pytest.mark.parametrize("precision,tol", [("f64", FP64_TOL), ("f32", 1e-5)])
+def test_rkky_multilayer_rows(precision, tol):
+    """The C5 term list (demag, RKKY between layers, DMI, uniaxial, Zeeman) and RKKY alone on
+    rows of 256 cells: the z-column per-cell kernel with a compile-time and a run-time term list."""
+    nx, ny, nz = 256, 3, 4
+    mesh = (nx, ny, nz, 2e-9, 2e-9, 2e-9)
+    print(f"\n[rkky rows] mesh={mesh} {precision}", flush=True)
+    rng = np.random.default_rng(8)
+    rk = np.full((nx, ny, nz), -1e-3 * 2e-9 / 2)
+    A = np.full((nx, ny, nz), 1.3e-11)
+    m = rand_m((nx, ny, nz), 14)
+    specs = [("demag", {}), ("rkky", {"rk": rk, "A": A}), ("dmi", {"D": 2.5e-3}),
+             ("uniaxial", {"K": 7e5, "axis": (0.0, 0.0, 1.0)}), ("zeeman_h", {"h": (50e-3 / MU0, 0, 0)})]
+    state, terms, sim = build(mesh, 1e6, m, specs)
+    M = mf()
+    h = M.LLGIntegrator(0.05, terms, precision=precision).H_in_Apm(state)
+    assert normwise(h, sim.heff()) <= tol, normwise(h, sim.heff())
+    A2 = 1e-11 * (1 + 0.5 * rng.random((nx, ny, nz)))
+    idx = np.zeros((nx, ny, nz), dtype=np.uint32)
+    idx[:, :, 2:] = 1
+    state, terms, sim = build(mesh, 1e6, m, [("rkky", {"rk": rk, "A": A2, "idx": idx})])
+    h = M.LLGIntegrator(0.05, terms, precision=precision).H_in_Apm(state)
+    assert normwise(h, sim.heff()) <= tol, normwise(h, sim.heff())
+
+
 @pytest.mark.parametrize("mesh", [(2, 1, 1, 2.7e-10, 2.7e-10, 2.7e-10), (16, 12, 1, 2.7e-10, 2.7e-10, 2.7e-10),
-                                  (8, 6, 3, 2.7e-10, 2.7e-10, 2.7e-10)])
+                                  (8, 6, 3, 2.7e-10, 2.7e-10, 2.7e-10), (256, 3, 2, 2.7e-10, 2.7e-10, 2.7e-10)])
 def test_atomistic_terms(mesh):
     J = 1.6e-21
     specs = [("atom_dipole", {}), ("atom_exchange", {"J": J}), ("atom_dmi", {"D": 0.2 * J}),
@@ -353,10 +378,13 @@ def test_vector_helpers():
     assert np.allclose(mean, mm.mean(axis=1), rtol=1e-13, atol=1e-15)
 
 
-@pytest.mark.parametrize("scheme", ["DP45", "RKF45"])
-def test_llg_step_replay(scheme):
-    # identical controller decisions and states, step by step, on a C1-like film
-    mesh = (32, 8, 1, 4e-9, 4e-9, 3e-9)
+@pytest.mark.parametrize("scheme,mesh", [("DP45", (32, 8, 1, 4e-9, 4e-9, 3e-9)),
+                                         ("RKF45", (32, 8, 1, 4e-9, 4e-9, 3e-9)),
+                                         ("DP45", (512, 2, 1, 4e-9, 4e-9, 3e-9))])
+def test_llg_step_replay(scheme, mesh):
+    # identical controller decisions and states, step by step, on a C1-like film (the 512-cell
+    # rows run the fused inverse-x / per-cell kernel)
+    print(f"\n[llg replay] {scheme} mesh={mesh}", flush=True)
     nx, ny, nz = mesh[:3]
     rng = np.random.default_rng(42)
     m = np.zeros((nx, ny, nz, 3))
