@@ -70,10 +70,16 @@ enum mafx_term_kind {
  *            RKKY: host scalar field of RKKY constants [N] (J/m, the reference's rkky_values)
  *   array1:  RKKY: host scalar field of exchange constants A [N] (J/m)
  *   index0:  RKKY: host scalar field of uint32 layer indices [N], or NULL for all zero
+ *   flags:   DEMAG / ATOM_DIPOLE: MAFX_TERM_CACHING reads the transformed tensor from the on-disk
+ *            cache and writes it there when absent, MAFX_TERM_VERBOSE reports cache activity on
+ *            stdout (the caching and verbose arguments of DemagField(mesh, verbose, caching),
+ *            demag_field.cpp:185-204, cache_manager.cpp:49-114)
  */
+#define MAFX_TERM_CACHING 1
+#define MAFX_TERM_VERBOSE 2
 typedef struct mafx_term_desc {
     int32_t kind;
-    int32_t reserved;
+    int32_t flags;
     double value;
     double axis[3];
     double field[3];
@@ -110,6 +116,19 @@ int mafx_ctx_device_bytes(const mafx_ctx* ctx, int64_t* bytes);
  * in the reference's operation order.  Replaces DemagField::calc_N (demag_field.cpp:104-142,207-209).
  * nthreads <= 0 uses every hardware thread.  Needs no GPU. */
 int mafx_demag_calc_N(const mafx_mesh* mesh, double* N_host, int nthreads);
+
+/* On-disk cache of transformed demag / dipole tensors (DemagField caching, demag_field.cpp:185-204;
+ * CacheManager, cache_manager.cpp:49-114).  Folder: $MAFX_CACHE_DIR, else
+ * <temp_directory_path>/magnumaf.cache as in the reference.  File name: the reference's mesh string
+ * to_string(mesh) (demag_field.cpp:185-190) followed by the layout (precision, dipole) and the exact
+ * bits of dx, dy, dz.  When the folder exceeds $MAFX_CACHE_MAX_BYTES (default 2e9, the reference's
+ * max_size_in_byte) the oldest files are removed until it is below a tenth of that (2e8 by
+ * default, the reference's shrink_size_in_byte).  A file that fails its header or checksum check
+ * is ignored and the tensor computed instead, as the reference does when af::readArray fails.
+ * mafx_demag_cache_name writes the file name for a mesh (needs no GPU); mafx_ctx_tensor_source
+ * tells where a context's tensor came from: 0 no tensor, 1 computed, 2 read from the cache. */
+int mafx_demag_cache_name(const mafx_mesh* mesh, int precision, int dipole, char* buf, int64_t buf_len);
+int mafx_ctx_tensor_source(const mafx_ctx* ctx, int* source);
 
 /* ------------------------------------------------------------------ field and energy */
 /* H_eff = sum of the selected terms, in term order (field_term.hpp:116-123), A/m.

@@ -22,7 +22,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", 
 GXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra", "-pthread"]
 
 HIP_SOURCES = ["fft.hip", "field.hip", "vec.hip", "net.hip", "llg.cpp", "lbfgs.cpp", "capi.cpp"]
-GXX_SOURCES = ["newell_host.cpp"]
+GXX_SOURCES = ["newell_host.cpp", "tensor_cache.cpp"]
 
 
 def _run(cmd):

@@ -8,6 +8,7 @@
 
 #include "fft_kernels.hpp"
 #include "mafx_internal.hpp"
+#include "tensor_cache.hpp"
 #include "mafx_llg.hpp"
 #include "net.hpp"
 #include "vec_kernels.hpp"
@@ -131,6 +132,25 @@ int mafx_device_count(int* count) {
     });
 }
 
+int mafx_demag_cache_name(const mafx_mesh* mesh, int precision, int dipole, char* buf, int64_t buf_len) {
+    return guarded([&] {
+        require(mesh && buf && buf_len > 0, "mafx_demag_cache_name: null argument");
+        require(precision == MAFX_F64 || precision == MAFX_F32_FIELD, "unknown precision");
+        TensorCacheKey k{2 * mesh->nx, 2 * mesh->ny, mesh->nz == 1 ? 1 : 2 * mesh->nz, (mesh->nx + 1 + 15) / 16 * 16,
+                         precision, dipole ? 1 : 0, mesh->dx, mesh->dy, mesh->dz};
+        const std::string s = tensor_cache_name(k);
+        require(static_cast<int64_t>(s.size()) < buf_len, "mafx_demag_cache_name: buffer too small");
+        std::memcpy(buf, s.c_str(), s.size() + 1);
+    });
+}
+
+int mafx_ctx_tensor_source(const mafx_ctx* ctx, int* source) {
+    return guarded([&] {
+        require(ctx && source, "mafx_ctx_tensor_source: null argument");
+        *source = ctx->tensor_source;
+    });
+}
+
 int mafx_demag_calc_N(const mafx_mesh* mesh, double* N_host, int nthreads) {
     return guarded([&] {
         require(mesh && N_host, "null argument");
@@ -172,6 +192,7 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
                 require(ctx->demag_term < 0, "at most one demag / dipole term per context");
                 ctx->demag_term = i;
                 ctx->atom_dipole = d.kind == MAFX_ATOM_DIPOLE;
+                ctx->demag_flags = d.flags;
                 break;
             case MAFX_EXCHANGE:
             case MAFX_ATOM_EXCHANGE: break;

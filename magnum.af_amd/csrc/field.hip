@@ -23,6 +23,7 @@
 #include "fft_device.hpp"
 #include "fft_kernels.hpp"
 #include "mafx_internal.hpp"
+#include "tensor_cache.hpp"
 
 namespace mafx {
 
@@ -881,7 +882,43 @@ __global__ void __launch_bounds__(256)
 // Demag / dipole tensor: the real-space tensor (FP64 path: on the host in the reference's
 // arithmetic; FP32 path: on the GPU), then a full FP64 R2C transform on the GPU with the same
 // kernels, stored as the complex half spectrum (FP64) or its real symmetric quarter (FP32).
+void build_tensor_uncached(mafx_ctx* ctx);
+
+// With MAFX_TERM_CACHING the stored tensor is first looked up in the on-disk cache
+// (tensor_cache.hpp) and written there after a computation (demag_field.cpp:185-204).
 void build_tensor(mafx_ctx* ctx) {
+    const int64_t n0 = ctx->n0, n1 = ctx->n1, n2 = ctx->n2, hxs = ctx->hxs;
+    const int64_t K = n2 * n1 * hxs;
+    ctx->n1h = n1 / 2 + 1;
+    ctx->n2h = n2 / 2 + 1;
+    const bool caching = (ctx->demag_flags & MAFX_TERM_CACHING) != 0;
+    const bool verbose = (ctx->demag_flags & MAFX_TERM_VERBOSE) != 0;
+    const uint64_t stored = ctx->precision == MAFX_F64
+                                ? sizeof(double2) * 6 * static_cast<uint64_t>(K)
+                                : sizeof(float) * 6 * static_cast<uint64_t>(ctx->n2h * ctx->n1h * hxs);
+    const TensorCacheKey key{n0, n1, n2, hxs, ctx->precision, ctx->atom_dipole ? 1 : 0,
+                             ctx->mesh.dx, ctx->mesh.dy, ctx->mesh.dz};
+    if (caching) {
+        std::vector<char> host;
+        if (tensor_cache_read(key, stored, host, verbose)) {
+            ctx->tensor.alloc(stored);
+            MAFX_HIP(hipMemcpy(ctx->tensor.p, host.data(), stored, hipMemcpyHostToDevice));
+            ctx->tensor_source = 2;
+            tensor_cache_shrink(verbose);
+            return;
+        }
+    }
+    build_tensor_uncached(ctx);
+    ctx->tensor_source = 1;
+    if (caching) {
+        std::vector<char> host(stored);
+        MAFX_HIP(hipMemcpy(host.data(), ctx->tensor.p, stored, hipMemcpyDeviceToHost));
+        tensor_cache_write(key, host.data(), stored, verbose);
+        tensor_cache_shrink(verbose);
+    }
+}
+
+void build_tensor_uncached(mafx_ctx* ctx) {
     const int64_t n0 = ctx->n0, n1 = ctx->n1, n2 = ctx->n2, hxs = ctx->hxs;
     const int64_t P = n0 * n1 * n2, K = n2 * n1 * hxs;
     DevBuf Nd, T64;
@@ -895,8 +932,6 @@ void build_tensor(mafx_ctx* ctx) {
                            n2, g.dx, g.dy, g.dz, ctx->atom_dipole ? 1 : 0);
         MAFX_HIP(hipGetLastError());
     }
-    ctx->n1h = n1 / 2 + 1;
-    ctx->n2h = n2 / 2 + 1;
     if (ctx->precision != MAFX_F64) zero_unpaired_planes(ctx, Nd.as<double>(), 6);
     T64.alloc(sizeof(double2) * 6 * K);
     MAFX_HIP(hipMemset(T64.p, 0, T64.bytes)); // padding columns stay defined

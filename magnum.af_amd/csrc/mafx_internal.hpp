@@ -123,6 +123,8 @@ struct mafx_ctx {
     std::vector<mafx::TermDev> terms;
     int demag_term{-1};       // index of the (micro or atomistic) FFT term, -1 if none
     bool atom_dipole{false};
+    int demag_flags{0};       // MAFX_TERM_CACHING / MAFX_TERM_VERBOSE of the FFT term
+    int tensor_source{0};     // 0 no tensor, 1 computed, 2 read from the on-disk cache
     // FFT geometry: expanded lengths (demag_field.cpp:14-16) and half-spectrum width
     int64_t n0{0}, n1{0}, n2{0}, hx{0};
     int64_t nzs{0};           // z planes of the spectrum workspace: nz when the z stage is fused, else n2

@@ -125,6 +125,12 @@ class Context:
         check(lib().mafx_energy(self.h, C.c_void_p(m_dev.data_ptr()), mask, _dp(e), C.byref(tot)))
         return tot.value, e
 
+    def tensor_source(self):
+        """0 no demag tensor, 1 computed, 2 read from the on-disk cache."""
+        v = C.c_int()
+        check(lib().mafx_ctx_tensor_source(self.h, C.byref(v)))
+        return v.value
+
     def device_bytes(self):
         b = C.c_int64()
         check(lib().mafx_ctx_device_bytes(self.h, C.byref(b)))
@@ -270,13 +276,20 @@ def _td(kind, value=0.0, axis=(0.0, 0.0, 1.0), field=(0.0, 0.0, 0.0)):
 
 
 class DemagField(FieldTerm):
-    """DemagField(mesh, verbose=False, caching=False, nthreads=4) (demag_field.hpp:14-30)."""
+    """DemagField(mesh, verbose=False, caching=False, nthreads=4) (demag_field.hpp:14-30,
+    magnumaf.pyx:1086-1121).  caching reads the transformed tensor from the on-disk cache
+    (MAFX_CACHE_DIR, else <tmp>/magnumaf.cache) and writes it there when absent; verbose reports
+    the cache activity.  nthreads is accepted for the reference's signature (the host tensor uses
+    every hardware thread)."""
 
     def __init__(self, mesh, verbose=False, caching=False, nthreads=4):
         self.mesh = mesh
+        self.verbose, self.caching, self.nthreads = bool(verbose), bool(caching), nthreads
 
     def _desc(self, mesh):
-        return _td(_abi.DEMAG), []
+        d = _td(_abi.DEMAG)
+        d.flags = (_abi.TERM_CACHING if self.caching else 0) | (_abi.TERM_VERBOSE if self.verbose else 0)
+        return d, []
 
 
 class ExchangeField(FieldTerm):

@@ -8,11 +8,13 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libmafx.so")
 
 MAFX_F64, MAFX_F32_FIELD = 0, 1
 DEMAG, EXCHANGE, DMI, UNIAXIAL, ZEEMAN, RKKY = 1, 2, 3, 4, 5, 6
+TERM_CACHING, TERM_VERBOSE = 1, 2  # mafx_term_desc.flags (include/mafx.h)
 ATOM_DIPOLE, ATOM_EXCHANGE, ATOM_DMI, ATOM_UNIAXIAL, ATOM_ZEEMAN = 11, 12, 13, 14, 15
 
 # every entry point include/mafx.h declares
 EXPORTS = [
-    "mafx_last_error", "mafx_device_count", "mafx_demag_calc_N", "mafx_ctx_create", "mafx_ctx_destroy",
+    "mafx_last_error", "mafx_device_count", "mafx_demag_calc_N", "mafx_demag_cache_name", "mafx_ctx_tensor_source",
+    "mafx_ctx_create", "mafx_ctx_destroy",
     "mafx_ctx_set_stream", "mafx_ctx_sync", "mafx_ctx_device_bytes", "mafx_heff", "mafx_energy", "mafx_llg_rhs",
     "mafx_dot", "mafx_max_abs", "mafx_normalize", "mafx_mean_m", "mafx_llg_create", "mafx_llg_destroy",
     "mafx_llg_set_alpha", "mafx_llg_step", "mafx_llg_run_until", "mafx_llg_integrate_dense", "mafx_llg_get_stats",
@@ -29,7 +31,7 @@ class Mesh(C.Structure):
 
 
 class TermDesc(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("value", C.c_double),
+    _fields_ = [("kind", C.c_int32), ("flags", C.c_int32), ("value", C.c_double),
                 ("axis", C.c_double * 3), ("field", C.c_double * 3),
                 ("array0", C.POINTER(C.c_double)), ("array1", C.POINTER(C.c_double)),
                 ("index0", C.POINTER(C.c_uint32))]
@@ -61,6 +63,8 @@ def lib():
     L.mafx_last_error.restype = C.c_char_p
     L.mafx_device_count.argtypes = [C.POINTER(C.c_int)]
     L.mafx_demag_calc_N.argtypes = [C.POINTER(Mesh), dp, C.c_int]
+    L.mafx_demag_cache_name.argtypes = [C.POINTER(Mesh), C.c_int, C.c_int, C.c_char_p, i64]
+    L.mafx_ctx_tensor_source.argtypes = [vp, C.POINTER(C.c_int)]
     L.mafx_ctx_create.argtypes = [C.POINTER(Mesh), C.c_double, dp, C.POINTER(TermDesc), C.c_int, C.c_int,
                                   C.POINTER(vp)]
     L.mafx_ctx_destroy.argtypes = [vp]

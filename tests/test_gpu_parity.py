@@ -183,6 +183,48 @@ def test_ms_field_airbox():
     assert abs(llg.Eeff_in_J(state) - sim.energy()[0]) <= FP64_TOL * abs(sim.energy()[1]).max()
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_demag_tensor_cache(precision, tmp_path, monkeypatch):
+    """DemagField(mesh, caching=True): the first context computes the tensor and writes it to the
+    cache folder, the second reads it and gives the same field to the last bit; a damaged file is
+    ignored (the tensor is computed again, demag_field.cpp:185-204 / cache_manager.cpp:76-95); a
+    folder above its size limit is shrunk, oldest files first (cache_manager.cpp:29-48,104-110)."""
+    import os
+    monkeypatch.setenv("MAFX_CACHE_DIR", str(tmp_path))
+    monkeypatch.delenv("MAFX_CACHE_MAX_BYTES", raising=False)
+    M = mf()
+    mesh = (24, 16, 2, 2e-9, 2e-9, 2e-9)
+    g = M.Mesh(*mesh)
+    state = M.State(g, 1e6, rand_m(mesh[:3], 31))
+    sim = O.Sim(mesh, 1e6, rand_m(mesh[:3], 31), [O.term(O.DEMAG)])
+    tol = FP64_TOL if precision == "f64" else 1e-5
+
+    def field():
+        d = M.DemagField(g, caching=True)
+        ctx = state._context([d], precision)
+        return ctx.tensor_source(), ctx.heff(state.m_dev).cpu().numpy().copy()
+
+    src1, h1 = field()
+    files = sorted(os.listdir(tmp_path))
+    assert src1 == 1 and len(files) == 1
+    src2, h2 = field()
+    assert src2 == 2 and np.array_equal(h1, h2)
+    assert normwise(h2, sim.heff_flat()) <= tol
+    # flip one payload byte: the checksum rejects the file and the tensor is computed again
+    path = tmp_path / files[0]
+    raw = bytearray(path.read_bytes())
+    raw[-5] ^= 0xFF
+    path.write_bytes(bytes(raw))
+    src3, h3 = field()
+    assert src3 == 1 and np.array_equal(h1, h3)
+    # a limit below one file: the write is followed by a shrink that removes it
+    for f in os.listdir(tmp_path):
+        os.remove(tmp_path / f)
+    monkeypatch.setenv("MAFX_CACHE_MAX_BYTES", "1000")
+    src4, _ = field()
+    assert src4 == 1 and os.listdir(tmp_path) == []
+
+
 def test_demag_cube_energy_gpu():
     # magnumaf/test/micro_demag.cpp:6-29
     x = y = z = 1e-9
