@@ -181,7 +181,7 @@ int oracle_llg_init(oracle_sim* s, double alpha, const char* scheme, double hmin
     s->llg.controller.rtol = rtol;
     s->llg.dissipation_term_only = dissipation_term_only != 0;
     s->llg.normalize = normalize != 0;
-    s->llg.analytic_rhs = analytic_rhs != 0;
+    s->llg.analytic_rhs = analytic_rhs;
     s->llg_ready = true;
     return 0;
 }
@@ -225,6 +225,12 @@ int oracle_llg_attempt(oracle_sim* s, double dt, double* dm, double* err) {
     Vec r;
     if (s->llg.scheme == "DP45")
         r = s->llg.DP45(s->state, dt, e);
+    else if (s->llg.scheme == "BS23")
+        r = s->llg.BS23(s->state, dt, e);
+    else if (s->llg.scheme == "BS45")
+        r = s->llg.BS45(s->state, dt, e);
+    else if (s->llg.scheme == "DP78")
+        r = s->llg.DP78(s->state, dt, e);
     else
         r = s->llg.RKF45(s->state, dt, e);
     std::memcpy(dm, r.data(), sizeof(double) * r.size());

@@ -32,7 +32,8 @@ int oracle_energy(oracle_sim* s, uint32_t mask, double* E_terms, double* E_total
 int oracle_mean_m(const oracle_sim* s, double out[3]);
 
 /* LLGIntegrator(alpha, terms, scheme, Controller(hmin, hmax, atol, rtol), dissipation_term_only).
- * normalize / analytic_rhs exist for the AdaptiveRungeKutta analytic-ODE tests. */
+ * normalize / analytic_rhs exist for the AdaptiveRungeKutta analytic-ODE tests (analytic_rhs 1:
+ * y' = t sqrt(y) as in the reference's test; 2: y' = cos(t) y). */
 int oracle_llg_init(oracle_sim* s, double alpha, const char* scheme, double hmin, double hmax, double atol,
                     double rtol, int dissipation_term_only, int normalize, int analytic_rhs);
 int oracle_llg_set_alpha(oracle_sim* s, double alpha);

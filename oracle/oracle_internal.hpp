@@ -126,14 +126,21 @@ struct LLG { // llg_integrator.hpp / adaptive_runge_kutta.hpp
     Vec k_fsal;
     int64_t heff_evals{0};
     double h_last{0}, err_last{0};
-    // optional analytic right-hand side for the integrator unit tests (y' = t sqrt(y))
-    bool analytic_rhs{false};
+    // optional analytic right-hand side for the integrator unit tests: 1 is the reference's
+    // y' = t sqrt(y) (test/adaptive_runge_kutta.cpp:9-17), 2 is y' = cos(t) y, y = y0 exp(sin t)
+    int analytic_rhs{0};
 
     void f(const State& s, Vec& out) const; // llg_integrator.cpp:18-38
     void step(State& s);                    // adaptive_runge_kutta.cpp:39-71
     Vec RKF45(const State& s, double dt, double& err) const;
     Vec DP45(const State& s, double dt, double& err);
     Vec BS23(const State& s, double dt, double& err);
+    Vec BS45(const State& s, double dt, double& err);
+    Vec DP78(const State& s, double dt, double& err) const;
+    // explicit embedded scheme in the k = dt f form: stage inputs m + a[s][1] k1 + ..., sumbk =
+    // sum bs_j k_j, rk_error = sumbk - sum be_j k_j (all sums left to right)
+    Vec tableau(const State& s, double dt, double& err, int ns, const double (*a)[14], const double* c,
+                const double* bs, const double* be) const;
     // llg_integrator.cpp:68-104: returns rows (t, <mx>, <my>, <mz>)
     std::vector<std::array<double, 4>> integrate_dense(State& s, double time, double dt_write);
     mutable int64_t f_calls{0};

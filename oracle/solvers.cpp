@@ -61,9 +61,14 @@ bool Controller::success(const double err, double& h) {
 void LLG::f(const State& s, Vec& out) const {
     ++f_calls;
     const int64_t N = static_cast<int64_t>(s.m.size() / 3);
-    if (analytic_rhs) { // adaptive_runge_kutta test callback: t * sqrt(m)
+    if (analytic_rhs == 1) { // adaptive_runge_kutta test callback: t * sqrt(m)
         out.resize(s.m.size());
         for (size_t i = 0; i < s.m.size(); ++i) out[i] = s.t * std::sqrt(s.m[i]);
+        return;
+    }
+    if (analytic_rhs == 2) { // cos(t) m
+        out.resize(s.m.size());
+        for (size_t i = 0; i < s.m.size(); ++i) out[i] = std::cos(s.t) * s.m[i];
         return;
     }
     Vec h;
@@ -222,6 +227,108 @@ Vec LLG::BS23(const State& state, const double dt, double& err) {
     return sumbk;
 }
 
+Vec LLG::tableau(const State& state, const double dt, double& err, const int ns, const double (*a)[14],
+                 const double* c, const double* bs, const double* be) const {
+    const size_t n = state.m.size();
+    State ts = state;
+    std::vector<Vec> k(ns + 1);
+    Vec tmp;
+    for (int s = 1; s <= ns; ++s) {
+        ts.t = state.t + c[s] * dt;
+        ts.m = state.m;
+        for (int j = 1; j < s; ++j)
+            for (size_t i = 0; i < n; ++i) ts.m[i] = ts.m[i] + a[s][j] * k[j][i];
+        f(ts, tmp);
+        k[s] = scaled(dt, tmp);
+    }
+    Vec sumbk(n), rk_error(n);
+    for (size_t i = 0; i < n; ++i) {
+        double sb = bs[1] * k[1][i], se = be[1] * k[1][i];
+        for (int j = 2; j <= ns; ++j) {
+            sb = sb + bs[j] * k[j][i]; // zero weights add +0 and leave the sum unchanged
+            se = se + be[j] * k[j][i];
+        }
+        sumbk[i] = sb;
+        rk_error[i] = sb - se;
+    }
+    err = err_norm(controller, state.m, sumbk, rk_error);
+    return sumbk;
+}
+
+// ------------------------------------------------------------------ adaptive_runge_kutta.cpp:176-282
+// Bogacki-Shampine 5(4), eight stages, the last one at m + sumbk.  k1 is always recomputed (the
+// reference reuses k_FSAL only when its step_calls_ counter is non-zero, and it is never advanced).
+Vec LLG::BS45(const State& state, const double dt, double& err) {
+    double a[14][14] = {};
+    double b[14] = {}, c[14] = {}, bs[14] = {};
+    a[2][1] = 1.0 / 6.0;
+    a[3][1] = 2.0 / 27.0, a[3][2] = 4.0 / 27.0;
+    a[4][1] = 183.0 / 1372.0, a[4][2] = -162.0 / 343.0, a[4][3] = 1053.0 / 1372.0;
+    a[5][1] = 68.0 / 297.0, a[5][2] = -4.0 / 11.0, a[5][3] = 42.0 / 143.0, a[5][4] = 1960.0 / 3861.0;
+    a[6][1] = 597.0 / 22528.0, a[6][2] = 81.0 / 352.0, a[6][3] = 63099.0 / 585728.0, a[6][4] = 58653.0 / 366080.0,
+    a[6][5] = 4617.0 / 20480.0;
+    a[7][1] = 174197.0 / 959244.0, a[7][2] = -30942.0 / 79937.0, a[7][3] = 8152137.0 / 19744439.0,
+    a[7][4] = 666106.0 / 1039181.0, a[7][5] = -29421.0 / 29068.0, a[7][6] = 482048.0 / 414219.0;
+    a[8][1] = 587.0 / 8064.0, a[8][2] = 0.0, a[8][3] = 4440339.0 / 15491840.0, a[8][4] = 24353.0 / 124800.0,
+    a[8][5] = 387.0 / 44800.0, a[8][6] = 2152.0 / 5985.0, a[8][7] = 7267.0 / 94080.0;
+    c[2] = 1.0 / 6.0, c[3] = 2.0 / 9.0, c[4] = 3.0 / 7.0, c[5] = 2.0 / 3.0, c[6] = 3.0 / 4.0, c[7] = 1.0, c[8] = 1.0;
+    b[1] = 2479.0 / 34992.0, b[2] = 0.0, b[3] = 123.0 / 416.0, b[4] = 612941.0 / 3411720.0, b[5] = 43.0 / 1440.0,
+    b[6] = 2272.0 / 6561.0, b[7] = 79937.0 / 1113912.0, b[8] = 3293.0 / 556956.0;
+    for (int j = 1; j <= 7; ++j) bs[j] = a[8][j];
+    // the reference's sumbk runs over k1..k7; bs[8] = 0 adds 0 * k8, which leaves it unchanged
+    return tableau(state, dt, err, 8, a, c, bs, b);
+}
+
+// ------------------------------------------------------------------ adaptive_runge_kutta.cpp:285-493
+// Dormand-Prince 8(7), thirteen stages; advances with the order-8 weights bhat.
+Vec LLG::DP78(const State& state, const double dt, double& err) const {
+    double a[14][14] = {};
+    double b[14] = {}, bh[14] = {}, c[14] = {};
+    a[2][1] = 5.55555555555555555555555555556e-2;
+    a[3][1] = 2.08333333333333333333333333333e-2, a[3][2] = 6.25e-2;
+    a[4][1] = 3.125e-2, a[4][3] = 9.375e-2;
+    a[5][1] = 3.125e-1, a[5][3] = -1.171875e0, a[5][4] = 1.171875e0;
+    a[6][1] = 3.75e-2, a[6][4] = 1.875e-1, a[6][5] = 1.5e-1;
+    a[7][1] = 4.79101371111111111111111111111e-2, a[7][4] = 1.12248712777777777777777777778e-1,
+    a[7][5] = -2.55056737777777777777777777778e-2, a[7][6] = 1.28468238888888888888888888889e-2;
+    a[8][1] = 1.6917989787292281181431107136e-2, a[8][4] = 3.87848278486043169526545744159e-1,
+    a[8][5] = 3.59773698515003278967008896348e-2, a[8][6] = 1.96970214215666060156715256072e-1,
+    a[8][7] = -1.72713852340501838761392997002e-1;
+    a[9][1] = 6.90957533591923006485645489846e-2, a[9][4] = -6.34247976728854151882807874972e-1,
+    a[9][5] = -1.61197575224604080366876923982e-1, a[9][6] = 1.38650309458825255419866950133e-1,
+    a[9][7] = 9.4092861403575626972423968413e-1, a[9][8] = 2.11636326481943981855372117132e-1;
+    a[10][1] = 1.83556996839045385489806023537e-1, a[10][4] = -2.46876808431559245274431575997e0,
+    a[10][5] = -2.91286887816300456388002572804e-1, a[10][6] = -2.6473020233117375688439799466e-2,
+    a[10][7] = 2.84783876419280044916451825422e0, a[10][8] = 2.81387331469849792539403641827e-1,
+    a[10][9] = 1.23744899863314657627030212664e-1;
+    a[11][1] = -1.21542481739588805916051052503e0, a[11][4] = 1.66726086659457724322804132886e1,
+    a[11][5] = 9.15741828416817960595718650451e-1, a[11][6] = -6.05660580435747094755450554309e0,
+    a[11][7] = -1.60035735941561781118417064101e1, a[11][8] = 1.4849303086297662557545391898e1,
+    a[11][9] = -1.33715757352898493182930413962e1, a[11][10] = 5.13418264817963793317325361166e0;
+    a[12][1] = 2.58860916438264283815730932232e-1, a[12][4] = -4.77448578548920511231011750971e0,
+    a[12][5] = -4.3509301377703250944070041181e-1, a[12][6] = -3.04948333207224150956051286631e0,
+    a[12][7] = 5.57792003993609911742367663447e0, a[12][8] = 6.15583158986104009733868912669e0,
+    a[12][9] = -5.06210458673693837007740643391e0, a[12][10] = 2.19392617318067906127491429047e0,
+    a[12][11] = 1.34627998659334941535726237887e-1;
+    a[13][1] = 8.22427599626507477963168204773e-1, a[13][4] = -1.16586732572776642839765530355e1,
+    a[13][5] = -7.57622116690936195881116154088e-1, a[13][6] = 7.13973588159581527978269282765e-1,
+    a[13][7] = 1.20757749868900567395661704486e1, a[13][8] = -2.12765911392040265639082085897e0,
+    a[13][9] = 1.99016620704895541832807169835e0, a[13][10] = -2.34286471544040292660294691857e-1,
+    a[13][11] = 1.7589857770794226507310510589e-1;
+    bh[1] = 4.17474911415302462220859284685e-2, bh[6] = -5.54523286112393089615218946547e-2,
+    bh[7] = 2.39312807201180097046747354249e-1, bh[8] = 7.0351066940344302305804641089e-1,
+    bh[9] = -7.59759613814460929884487677085e-1, bh[10] = 6.60563030922286341461378594838e-1,
+    bh[11] = 1.58187482510123335529614838601e-1, bh[12] = -2.38109538752862804471863555306e-1, bh[13] = 2.5e-1;
+    b[1] = 2.9553213676353496981964883112e-2, b[6] = -8.28606276487797039766805612689e-1,
+    b[7] = 3.11240900051118327929913751627e-1, b[8] = 2.46734519059988698196468570407e0,
+    b[9] = -2.54694165184190873912738007542e0, b[10] = 1.44354858367677524030187495069e0,
+    b[11] = 7.94155958811272872713019541622e-2, b[12] = 4.44444444444444444444444444445e-2;
+    c[2] = 5.55555555555555555555555555556e-2, c[3] = 8.33333333333333333333333333334e-2, c[4] = 1.25e-1,
+    c[5] = 3.125e-1, c[6] = 3.75e-1, c[7] = 1.475e-1, c[8] = 4.65e-1, c[9] = 5.64865451382259575398358501426e-1,
+    c[10] = 6.5e-1, c[11] = 9.24656277640504446745013574318e-1, c[12] = 1.0, c[13] = 1.0;
+    return tableau(state, dt, err, 13, a, c, bh, b);
+}
+
 // ------------------------------------------------------------------ adaptive_runge_kutta.cpp:39-71
 void LLG::step(State& s) {
     Vec mtemp;
@@ -233,6 +340,10 @@ void LLG::step(State& s) {
             mtemp = DP45(s, h, err);
         else if (scheme == "BS23")
             mtemp = BS23(s, h, err);
+        else if (scheme == "BS45")
+            mtemp = BS45(s, h, err);
+        else if (scheme == "DP78")
+            mtemp = DP78(s, h, err);
         else
             throw std::runtime_error("oracle: unsupported scheme " + scheme);
     } while (!controller.success(err, h));

@@ -346,7 +346,8 @@ def test_rkky_mumax3_energy_sweep():
 
 # ---------------------------------------------------------------- RK tableaux (magnumaf/test/adaptive_runge_kutta.cpp:9-62)
 @pytest.mark.parametrize("scheme,steps,tol,ctol", [("BS23", 200, 1e-12, 1e-10), ("DP45", 100, 1e-8, 1e-10),
-                                                   ("RKF45", 100, 1e-8, 1e-10)])
+                                                   ("RKF45", 100, 1e-8, 1e-10), ("BS45", 100, 1e-8, 1e-10),
+                                                   ("DP78", 100, 1e-8, 1e-14)])
 def test_adaptive_rk_analytic(scheme, steps, tol, ctol):
     m = np.zeros((1, 1, 1, 3))
     m[0, 0, 0, 0] = 1
@@ -356,6 +357,46 @@ def test_adaptive_rk_analytic(scheme, steps, tol, ctol):
         s.step()
         t = s.t
         assert abs(s.m_flat()[0] - (t * t + 4) ** 2 / 16.0) < tol
+
+
+@pytest.mark.parametrize("scheme,order,h", [("BS23", 3, 0.2), ("RKF45", 5, 0.2), ("BS45", 5, 0.1),
+                                             ("DP78", 8, 0.8), ("DP45", 1, 0.2)])
+def test_adaptive_rk_convergence_order(scheme, order, h):
+    """Fixed steps (hmin = hmax = h: every step is accepted at h, controller.cpp:77-88) on
+    y' = cos(t) y, y(0) = 1, y = exp(sin t), to t ~ 6: halving h divides the error by 2^order.  A
+    wrong tableau entry breaks an order condition and drops the observed order, which the adaptive
+    known answers above can miss (the controller shrinks h until the error is small anyway).
+    DP45 shows first order here, and that pins the reference's FSAL reuse: without
+    normalisation k1 is the previous step's k_FSAL = dt_prev f (adaptive_runge_kutta.cpp:132-136,
+    166), so the second step, the first at h after the 1.01e-15 start, takes k1 scaled by
+    1.01e-15 / h.  The LLG integrator normalises and never takes that branch."""
+    def run(hh):
+        m = np.zeros((1, 1, 1, 3))
+        m[0, 0, 0, 0] = 1
+        s = O.Sim((1, 1, 1, 0, 0, 0), 1.0, m, [O.term(O.ZEEMAN)])
+        s.llg_init(0.0, scheme, hmin=hh, hmax=hh, atol=1e-10, rtol=1e-10, normalize=False, analytic_rhs=2)
+        while s.t < 6.0 - 1e-9:
+            s.step()
+        return abs(s.m_flat()[0] - math.exp(math.sin(s.t)))
+    e1, e2 = run(h), run(h / 2)
+    p = math.log2(e1 / e2)
+    assert abs(p - order) < 0.3, (scheme, e1, e2, p)
+
+
+@pytest.mark.parametrize("scheme,order", [("BS23", 3), ("RKF45", 5), ("DP45", 5), ("BS45", 5), ("DP78", 8)])
+def test_adaptive_rk_error_estimate_order(scheme, order):
+    """The embedded error estimate of one attempt from t = 0 on y' = cos(t) y is the local error of
+    the lower-order member, O(h^order): halving h divides it by at least 2^order (BS23 shows 2^4
+    from this starting point).  The error weights enter only this estimate, so a wrong one, which
+    leaves sum e_j != 0 or breaks an order condition, shows as O(h) here."""
+    def err(hh):
+        m = np.zeros((1, 1, 1, 3))
+        m[0, 0, 0, 0] = 1
+        s = O.Sim((1, 1, 1, 0, 0, 0), 1.0, m, [O.term(O.ZEEMAN)])
+        s.llg_init(0.0, scheme, hmin=1e-15, hmax=1e15, atol=1.0, rtol=1.0, normalize=False, analytic_rhs=2)
+        return s.attempt(hh)[1]
+    p = math.log2(err(0.2) / err(0.1))
+    assert p > order - 0.3, (scheme, p)
 
 
 # ---------------------------------------------------------------- SP4 (python/test/sp4.py:21-62)
