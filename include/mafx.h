@@ -162,8 +162,8 @@ int mafx_mean_m(mafx_ctx* ctx, const double* m_dev, double out_host[3]);
 
 /* ------------------------------------------------------------------ adaptive RK / LLG */
 /* LLGIntegrator(alpha, terms, scheme, Controller(hmin, hmax, atol, rtol), dissipation_term_only)
- * (llg_integrator.hpp:26-49, controller.hpp:11-17).  scheme: "DP45" or "RKF45"
- * (adaptive_runge_kutta.cpp:75-173).  The integrator evaluates all terms of ctx. */
+ * (llg_integrator.hpp:26-49, controller.hpp:11-17).  scheme: "RKF45", "DP45", "BS45", "DP78" or
+ * "BS23" (adaptive_runge_kutta.cpp:8-37,75-529).  The integrator evaluates all terms of ctx. */
 int mafx_llg_create(mafx_ctx* ctx, double alpha, const char* scheme, double hmin, double hmax,
                     double atol, double rtol, int dissipation_term_only, mafx_llg** out);
 int mafx_llg_destroy(mafx_llg* llg);
@@ -172,6 +172,12 @@ int mafx_llg_set_alpha(mafx_llg* llg, double alpha);
  * attempts until the controller accepts, then t += h, m += dm, renormalise.  m_dev is
  * updated in place, *t_inout advanced. */
 int mafx_llg_step(mafx_llg* llg, double* m_dev, double* t_inout);
+/* LLGIntegrator::relax (llg_integrator.cpp:43-66): while the relative energy change
+ * |(E_prev - E) / E_prev| between two checks exceeds precision (E_prev starts at 1e20), takes
+ * eval_E steps; with verbose, prints the relative change and <m> whenever the integrator's step
+ * count is a multiple of iwritecout (> 0).  E is the total energy of all terms of ctx. */
+int mafx_llg_relax(mafx_llg* llg, double* m_dev, double* t_inout, double precision, int64_t eval_E,
+                   int64_t iwritecout, int verbose, int64_t* nsteps_done);
 /* Runs mafx_llg_step until *t_inout >= t_end or nsteps_max steps were taken. */
 int mafx_llg_run_until(mafx_llg* llg, double* m_dev, double* t_inout, double t_end,
                        int64_t nsteps_max, int64_t* nsteps_done);

@@ -1,6 +1,7 @@
 // extern "C" entry points of libmafx (declared in include/mafx.h): context construction, term
 // upload, field / energy evaluation, vector helpers, the LLG integrator, LBFGS and profiling.
 // No exception crosses the boundary: each entry point returns a status and records the message.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -331,7 +332,8 @@ int mafx_llg_create(mafx_ctx* ctx, double alpha, const char* scheme, double hmin
     return guarded([&] {
         require(ctx && scheme && out, "null argument");
         const std::string s(scheme);
-        require(s == "DP45" || s == "RKF45", "scheme must be \"DP45\" or \"RKF45\"");
+        require(s == "DP45" || s == "RKF45" || s == "BS23" || s == "BS45" || s == "DP78",
+                "scheme must be one of \"RKF45\", \"DP45\", \"BS45\", \"DP78\", \"BS23\"");
         auto L = std::make_unique<mafx_llg>();
         L->ctx = ctx;
         L->alpha = alpha;
@@ -342,7 +344,9 @@ int mafx_llg_create(mafx_ctx* ctx, double alpha, const char* scheme, double hmin
         L->ctrl.atol = atol;
         L->ctrl.rtol = rtol;
         const size_t bytes = sizeof(double) * 3 * ctx->N;
-        for (int j = 1; j <= 7; ++j) L->k[j].alloc(bytes);
+        // the scheme's stages; at least RKF45's six for the dense-output back step
+        const int ns = std::max(scheme_stages(s.c_str()), 6);
+        for (int j = 1; j <= ns; ++j) L->k[j].alloc(bytes);
         L->mtmp.alloc(bytes);
         L->mtmp2.alloc(bytes);
         L->sumbk.alloc(bytes);
@@ -362,6 +366,45 @@ int mafx_llg_step(mafx_llg* llg, double* m_dev, double* t_inout) {
     return guarded([&] {
         require(llg && m_dev && t_inout, "null argument");
         llg_step(llg, m_dev, t_inout);
+    });
+}
+
+int mafx_llg_relax(mafx_llg* llg, double* m_dev, double* t_inout, double precision, int64_t eval_E,
+                   int64_t iwritecout, int verbose, int64_t* nsteps_done) {
+    return guarded([&] {
+        require(llg && m_dev && t_inout, "null argument");
+        require(eval_E > 0, "eval_E must be positive");
+        mafx_ctx* ctx = llg->ctx;
+        auto energy = [&] {
+            EvalArgs a;
+            a.m = m_dev;
+            a.mode = OutMode::ENERGY;
+            a.want_energy = true;
+            evaluate(ctx, a);
+            double E = 0;
+            finish_energy(ctx, ~0u, nullptr, &E);
+            return E;
+        };
+        const double t0 = *t_inout;
+        int64_t n = 0;
+        double E_prev = 1e20;
+        double E = energy();
+        while (std::fabs((E_prev - E) / E_prev) > precision) {
+            E_prev = E;
+            for (int64_t i = 0; i < eval_E; ++i) {
+                llg_step(llg, m_dev, t_inout);
+                ++n;
+            }
+            E = energy();
+            if (verbose && iwritecout > 0 && llg->steps % iwritecout == 0) {
+                double mm[3];
+                mean_m(ctx, m_dev, mm);
+                std::printf("relaxing: step=%lld, rel_diff= %e, <mx>=%f, <my>=%f, <mz>=%f\n",
+                            static_cast<long long>(llg->steps), std::fabs((E_prev - E) / E_prev), mm[0], mm[1], mm[2]);
+            }
+        }
+        if (verbose) std::printf("relaxed %e [s]. Current state.t= %e\n", *t_inout - t0, *t_inout);
+        if (nsteps_done) *nsteps_done = n;
     });
 }
 

@@ -66,11 +66,11 @@ namespace {
 
 // Butcher tableaux exactly as the reference writes its coefficients.
 struct Tableau {
-    int stages;          // number of f evaluations
-    double a[8][8];      // a[s][j], s = 2..stages
-    double b[8];         // weights of the propagated solution (sumbk)
-    double e[8];         // error weights
-    bool rkf_form;       // rk_error = sumbk - sum e_j k_j
+    int stages;                             // number of f evaluations
+    double a[kMaxStages + 1][kMaxStages + 1]; // a[s][j], s = 2..stages
+    double b[kMaxStages + 1];               // weights of the propagated solution (sumbk)
+    double e[kMaxStages + 1];               // error weights
+    bool rkf_form;                          // rk_error = sumbk - sum e_j k_j
 };
 
 Tableau dp45() { // adaptive_runge_kutta.cpp:117-128
@@ -114,12 +114,104 @@ Tableau rkf45() { // adaptive_runge_kutta.cpp:75-110 (subtractions written as ne
 // with '-' for the negative entries; x - (c k) == x + ((-c) k) exactly, so the negative weights
 // reproduce its rounding.
 
+Tableau bs23() { // adaptive_runge_kutta.cpp:496-529
+    // The reference keeps f (not dt f) in its stages and forms m + dt (1/2 k1), m + dt 3/4 k2,
+    // sumbk = dt (2/9 k1 + 1/3 k2 + 4/9 k3) and rk_error = sumbk - dt (7/24 k1 + ... + 1/8 k4).
+    // Here every stage is k = dt f like the other schemes: the same values up to rounding.
+    Tableau t{};
+    t.stages = 4;
+    t.rkf_form = true;
+    t.a[2][1] = 1. / 2.;
+    t.a[3][2] = 3. / 4.;
+    t.a[4][1] = 2. / 9., t.a[4][2] = 1. / 3., t.a[4][3] = 4. / 9.;
+    t.b[1] = 2. / 9., t.b[2] = 1. / 3., t.b[3] = 4. / 9.;
+    t.e[1] = 7. / 24., t.e[2] = 1. / 4., t.e[3] = 1. / 3., t.e[4] = 1. / 8.;
+    return t;
+}
+
+Tableau bs45() { // adaptive_runge_kutta.cpp:176-282 (eight stages, the last one at m + sumbk)
+    Tableau t{};
+    t.stages = 8;
+    t.rkf_form = true;
+    t.a[2][1] = 1.0 / 6.0;
+    t.a[3][1] = 2.0 / 27.0, t.a[3][2] = 4.0 / 27.0;
+    t.a[4][1] = 183.0 / 1372.0, t.a[4][2] = -162.0 / 343.0, t.a[4][3] = 1053.0 / 1372.0;
+    t.a[5][1] = 68.0 / 297.0, t.a[5][2] = -4.0 / 11.0, t.a[5][3] = 42.0 / 143.0, t.a[5][4] = 1960.0 / 3861.0;
+    t.a[6][1] = 597.0 / 22528.0, t.a[6][2] = 81.0 / 352.0, t.a[6][3] = 63099.0 / 585728.0,
+    t.a[6][4] = 58653.0 / 366080.0, t.a[6][5] = 4617.0 / 20480.0;
+    t.a[7][1] = 174197.0 / 959244.0, t.a[7][2] = -30942.0 / 79937.0, t.a[7][3] = 8152137.0 / 19744439.0,
+    t.a[7][4] = 666106.0 / 1039181.0, t.a[7][5] = -29421.0 / 29068.0, t.a[7][6] = 482048.0 / 414219.0;
+    t.a[8][1] = 587.0 / 8064.0, t.a[8][3] = 4440339.0 / 15491840.0, t.a[8][4] = 24353.0 / 124800.0,
+    t.a[8][5] = 387.0 / 44800.0, t.a[8][6] = 2152.0 / 5985.0, t.a[8][7] = 7267.0 / 94080.0;
+    for (int j = 1; j <= 7; ++j) t.b[j] = t.a[8][j];
+    t.e[1] = 2479.0 / 34992.0, t.e[3] = 123.0 / 416.0, t.e[4] = 612941.0 / 3411720.0, t.e[5] = 43.0 / 1440.0,
+    t.e[6] = 2272.0 / 6561.0, t.e[7] = 79937.0 / 1113912.0, t.e[8] = 3293.0 / 556956.0;
+    return t;
+}
+
+Tableau dp78() { // adaptive_runge_kutta.cpp:285-493 (advances with the order-8 weights bhat)
+    Tableau t{};
+    t.stages = 13;
+    t.rkf_form = true;
+    auto& a = t.a;
+    a[2][1] = 5.55555555555555555555555555556e-2;
+    a[3][1] = 2.08333333333333333333333333333e-2, a[3][2] = 6.25e-2;
+    a[4][1] = 3.125e-2, a[4][3] = 9.375e-2;
+    a[5][1] = 3.125e-1, a[5][3] = -1.171875e0, a[5][4] = 1.171875e0;
+    a[6][1] = 3.75e-2, a[6][4] = 1.875e-1, a[6][5] = 1.5e-1;
+    a[7][1] = 4.79101371111111111111111111111e-2, a[7][4] = 1.12248712777777777777777777778e-1,
+    a[7][5] = -2.55056737777777777777777777778e-2, a[7][6] = 1.28468238888888888888888888889e-2;
+    a[8][1] = 1.6917989787292281181431107136e-2, a[8][4] = 3.87848278486043169526545744159e-1,
+    a[8][5] = 3.59773698515003278967008896348e-2, a[8][6] = 1.96970214215666060156715256072e-1,
+    a[8][7] = -1.72713852340501838761392997002e-1;
+    a[9][1] = 6.90957533591923006485645489846e-2, a[9][4] = -6.34247976728854151882807874972e-1,
+    a[9][5] = -1.61197575224604080366876923982e-1, a[9][6] = 1.38650309458825255419866950133e-1,
+    a[9][7] = 9.4092861403575626972423968413e-1, a[9][8] = 2.11636326481943981855372117132e-1;
+    a[10][1] = 1.83556996839045385489806023537e-1, a[10][4] = -2.46876808431559245274431575997e0,
+    a[10][5] = -2.91286887816300456388002572804e-1, a[10][6] = -2.6473020233117375688439799466e-2,
+    a[10][7] = 2.84783876419280044916451825422e0, a[10][8] = 2.81387331469849792539403641827e-1,
+    a[10][9] = 1.23744899863314657627030212664e-1;
+    a[11][1] = -1.21542481739588805916051052503e0, a[11][4] = 1.66726086659457724322804132886e1,
+    a[11][5] = 9.15741828416817960595718650451e-1, a[11][6] = -6.05660580435747094755450554309e0,
+    a[11][7] = -1.60035735941561781118417064101e1, a[11][8] = 1.4849303086297662557545391898e1,
+    a[11][9] = -1.33715757352898493182930413962e1, a[11][10] = 5.13418264817963793317325361166e0;
+    a[12][1] = 2.58860916438264283815730932232e-1, a[12][4] = -4.77448578548920511231011750971e0,
+    a[12][5] = -4.3509301377703250944070041181e-1, a[12][6] = -3.04948333207224150956051286631e0,
+    a[12][7] = 5.57792003993609911742367663447e0, a[12][8] = 6.15583158986104009733868912669e0,
+    a[12][9] = -5.06210458673693837007740643391e0, a[12][10] = 2.19392617318067906127491429047e0,
+    a[12][11] = 1.34627998659334941535726237887e-1;
+    a[13][1] = 8.22427599626507477963168204773e-1, a[13][4] = -1.16586732572776642839765530355e1,
+    a[13][5] = -7.57622116690936195881116154088e-1, a[13][6] = 7.13973588159581527978269282765e-1,
+    a[13][7] = 1.20757749868900567395661704486e1, a[13][8] = -2.12765911392040265639082085897e0,
+    a[13][9] = 1.99016620704895541832807169835e0, a[13][10] = -2.34286471544040292660294691857e-1,
+    a[13][11] = 1.7589857770794226507310510589e-1;
+    t.b[1] = 4.17474911415302462220859284685e-2, t.b[6] = -5.54523286112393089615218946547e-2,
+    t.b[7] = 2.39312807201180097046747354249e-1, t.b[8] = 7.0351066940344302305804641089e-1,
+    t.b[9] = -7.59759613814460929884487677085e-1, t.b[10] = 6.60563030922286341461378594838e-1,
+    t.b[11] = 1.58187482510123335529614838601e-1, t.b[12] = -2.38109538752862804471863555306e-1, t.b[13] = 2.5e-1;
+    t.e[1] = 2.9553213676353496981964883112e-2, t.e[6] = -8.28606276487797039766805612689e-1,
+    t.e[7] = 3.11240900051118327929913751627e-1, t.e[8] = 2.46734519059988698196468570407e0,
+    t.e[9] = -2.54694165184190873912738007542e0, t.e[10] = 1.44354858367677524030187495069e0,
+    t.e[11] = 7.94155958811272872713019541622e-2, t.e[12] = 4.44444444444444444444444444445e-2;
+    return t;
+}
+
+Tableau tableau_of(const std::string& s) {
+    if (s == "DP45") return dp45();
+    if (s == "BS23") return bs23();
+    if (s == "BS45") return bs45();
+    if (s == "DP78") return dp78();
+    return rkf45();
+}
+
 } // namespace
+
+int scheme_stages(const char* scheme) { return tableau_of(scheme).stages; }
 
 // One attempt of the scheme at step dt from m; returns the error norm, sumbk in llg->sumbk.
 double attempt(mafx_llg* L, const double* m, double dt, const char* scheme) {
     mafx_ctx* ctx = L->ctx;
-    const Tableau T = std::string(scheme) == "DP45" ? dp45() : rkf45();
+    const Tableau T = tableau_of(scheme);
     const int64_t len = 3 * ctx->N;
     EvalArgs ea;
     ea.mode = OutMode::LLG;
@@ -132,8 +224,8 @@ double attempt(mafx_llg* L, const double* m, double dt, const char* scheme) {
     evaluate(ctx, ea);
     L->heff_evals++;
     for (int s = 2; s <= T.stages; ++s) {
-        const double* v[8];
-        double c[8];
+        const double* v[kMaxStages];
+        double c[kMaxStages];
         for (int j = 1; j < s; ++j) {
             v[j - 1] = L->k[j].as<double>();
             c[j - 1] = T.a[s][j];
@@ -144,7 +236,7 @@ double attempt(mafx_llg* L, const double* m, double dt, const char* scheme) {
         evaluate(ctx, ea);
         L->heff_evals++;
     }
-    const double* kp[8] = {nullptr};
+    const double* kp[kMaxStages + 1] = {nullptr};
     for (int j = 1; j <= T.stages; ++j) kp[j] = L->k[j].as<double>();
     return rk_error(ctx, m, kp, T.b, T.e, T.stages, T.rkf_form, L->ctrl.atol, L->ctrl.rtol, L->sumbk.as<double>());
 }

@@ -31,12 +31,14 @@ struct mafx_llg {
     double h{1.01e-15}; // adaptive_runge_kutta.hpp:39
     double h_last{0}, err_last{0};
     int64_t heff_evals{0}, steps{0};
-    mafx::DevBuf k[8]; // k[1..7]
+    mafx::DevBuf k[14]; // k[1..stages], stages <= 13 (DP78)
     mafx::DevBuf mtmp, mtmp2, sumbk;
 };
 
 namespace mafx {
 double attempt(mafx_llg* L, const double* m, double dt, const char* scheme);
+// f evaluations per attempt of a scheme ("RKF45", "DP45", "BS23", "BS45", "DP78")
+int scheme_stages(const char* scheme);
 void llg_step(mafx_llg* L, double* m, double* t);
 int64_t llg_integrate_dense(mafx_llg* L, double* m, double* t, double time_in_s, double dt_write, double* out,
                             int64_t max_rows);

@@ -127,8 +127,8 @@ __global__ void __launch_bounds__(kB) k_llg_rhs(const double* __restrict__ m, co
 }
 
 struct LinArgs {
-    const double* v[8];
-    double c[8];
+    const double* v[kMaxStages];
+    double c[kMaxStages];
     int n;
 };
 
@@ -163,8 +163,8 @@ __global__ void __launch_bounds__(kB) k_line_point(double* __restrict__ x, const
 }
 
 struct ErrArgs {
-    const double* k[8]; // k[1..ns]
-    double b[8], e[8];
+    const double* k[kMaxStages + 1]; // k[1..ns]
+    double b[kMaxStages + 1], e[kMaxStages + 1];
     int ns;
     double atol, rtol;
 };
@@ -301,7 +301,7 @@ void llg_rhs(mafx_ctx* ctx, double alpha, double dt, const double* m, const doub
 
 void lincomb(mafx_ctx* ctx, double* out, const double* base, int n, const double* const* v, const double* c,
              int64_t len) {
-    require(n >= 0 && n <= 8, "lincomb: at most 8 vectors");
+    require(n >= 0 && n <= kMaxStages - 1, "lincomb: at most 12 vectors");
     LinArgs a{};
     a.n = n;
     for (int j = 0; j < n; ++j) {
@@ -321,7 +321,11 @@ void lincomb(mafx_ctx* ctx, double* out, const double* base, int n, const double
     case 5: hipLaunchKernelGGL(k_lincomb<5>, g, b, 0, ctx->stream, out, base, a, len); break;
     case 6: hipLaunchKernelGGL(k_lincomb<6>, g, b, 0, ctx->stream, out, base, a, len); break;
     case 7: hipLaunchKernelGGL(k_lincomb<7>, g, b, 0, ctx->stream, out, base, a, len); break;
-    default: hipLaunchKernelGGL(k_lincomb<8>, g, b, 0, ctx->stream, out, base, a, len); break;
+    case 8: hipLaunchKernelGGL(k_lincomb<8>, g, b, 0, ctx->stream, out, base, a, len); break;
+    case 9: hipLaunchKernelGGL(k_lincomb<9>, g, b, 0, ctx->stream, out, base, a, len); break;
+    case 10: hipLaunchKernelGGL(k_lincomb<10>, g, b, 0, ctx->stream, out, base, a, len); break;
+    case 11: hipLaunchKernelGGL(k_lincomb<11>, g, b, 0, ctx->stream, out, base, a, len); break;
+    default: hipLaunchKernelGGL(k_lincomb<12>, g, b, 0, ctx->stream, out, base, a, len); break;
     }
     MAFX_HIP(hipGetLastError());
 }
@@ -345,7 +349,8 @@ double rk_error(mafx_ctx* ctx, const double* m, const double* const* k, const do
     a.rtol = rtol;
     const int64_t len = 3 * ctx->N;
     const unsigned nb = red_blocks(len);
-    require(ns >= 1 && ns <= 7, "rk_error: 1..7 stages");
+    require((ns >= 1 && ns <= 8) || ns == 13, "rk_error: 1..8 or 13 stages");
+    require(ns <= 7 || rkf_form, "rk_error: more than 7 stages only in the sumbk - sum e k form");
     int nread = 0;
     for (int j = 1; j <= ns; ++j) nread += (b[j] != 0.0 || e[j] != 0.0);
     {
@@ -361,7 +366,9 @@ double rk_error(mafx_ctx* ctx, const double* m, const double* const* k, const do
             case 4: go(k_rk_error<4, true>); break;
             case 5: go(k_rk_error<5, true>); break;
             case 6: go(k_rk_error<6, true>); break;
-            default: go(k_rk_error<7, true>); break;
+            case 7: go(k_rk_error<7, true>); break;
+            case 8: go(k_rk_error<8, true>); break;
+            default: go(k_rk_error<13, true>); break;
             }
         } else {
             switch (ns) {

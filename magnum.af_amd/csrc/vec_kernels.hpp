@@ -4,6 +4,9 @@
 
 namespace mafx {
 
+// Largest stage count of the adaptive schemes (DP78); stage vectors are indexed 1..kMaxStages.
+constexpr int kMaxStages = 13;
+
 // sumbk = sum_j b_j k_j and the max-norm error of the embedded pair; k, b, e are indexed 1..ns.
 // rkf_form selects rk_error = sumbk - sum_j e_j k_j (adaptive_runge_kutta.cpp:105-106) instead of
 // rk_error = sum_j e_j k_j (:169).

@@ -416,7 +416,8 @@ class AtomisticExternalField(ExternalField):
 # ---------------------------------------------------------------- integrator
 class LLGIntegrator:
     """LLGIntegrator(alpha, terms, mode="RKF45", hmin=1e-15, hmax=3.5e-10, atol=1e-6, rtol=1e-6,
-    dissipation_term_only=False) (magnumaf.pyx:912-1033, llg_integrator.hpp:26-82).
+    dissipation_term_only=False) (magnumaf.pyx:912-1033, llg_integrator.hpp:26-82).  mode is one of
+    "RKF45", "DP45", "BS45", "DP78", "BS23".
 
     precision="f32" selects the FP32-field path (demag FFT and stencils in FP32, state and RK in FP64)."""
 
@@ -489,6 +490,20 @@ class LLGIntegrator:
         n = C.c_int64()
         check(lib().mafx_llg_run_until(self._llg, C.c_void_p(state.m_dev.data_ptr()), C.byref(t), t_end, max_steps,
                                        C.byref(n)))
+        state.t = t.value
+        state.steps += n.value
+        self.accumulated_steps += n.value
+        return n.value
+
+    def relax(self, state, precision=1e-10, ncalcE=100, nprint=1000, verbose=True):
+        """relax(state, precision=1e-10, ncalcE=100, nprint=1000) (magnumaf.pyx:993-998,
+        llg_integrator.cpp:43-66): steps until the relative energy change over ncalcE steps is
+        below precision.  Returns the number of steps taken."""
+        self._bind(state)
+        t = C.c_double(state.t)
+        n = C.c_int64()
+        check(lib().mafx_llg_relax(self._llg, C.c_void_p(state.m_dev.data_ptr()), C.byref(t), float(precision),
+                                   int(ncalcE), int(nprint), int(bool(verbose)), C.byref(n)))
         state.t = t.value
         state.steps += n.value
         self.accumulated_steps += n.value

@@ -17,7 +17,7 @@ EXPORTS = [
     "mafx_ctx_create", "mafx_ctx_destroy",
     "mafx_ctx_set_stream", "mafx_ctx_sync", "mafx_ctx_device_bytes", "mafx_heff", "mafx_energy", "mafx_llg_rhs",
     "mafx_dot", "mafx_max_abs", "mafx_normalize", "mafx_mean_m", "mafx_llg_create", "mafx_llg_destroy",
-    "mafx_llg_set_alpha", "mafx_llg_step", "mafx_llg_run_until", "mafx_llg_integrate_dense", "mafx_llg_get_stats",
+    "mafx_llg_set_alpha", "mafx_llg_step", "mafx_llg_run_until", "mafx_llg_relax", "mafx_llg_integrate_dense", "mafx_llg_get_stats",
     "mafx_llg_set_controller_state", "mafx_llg_get_controller_state", "mafx_lbfgs_minimize", "mafx_prof_enable",
     "mafx_prof_reset", "mafx_prof_report", "mafx_net_create", "mafx_net_destroy", "mafx_net_param_count",
     "mafx_net_set_params", "mafx_net_get_params", "mafx_net_forward", "mafx_net_gradients", "mafx_net_train_step",
@@ -84,6 +84,7 @@ def lib():
     L.mafx_llg_set_alpha.argtypes = [vp, C.c_double]
     L.mafx_llg_step.argtypes = [vp, vp, dp]
     L.mafx_llg_run_until.argtypes = [vp, vp, dp, C.c_double, i64, C.POINTER(i64)]
+    L.mafx_llg_relax.argtypes = [vp, vp, dp, C.c_double, i64, i64, C.c_int, C.POINTER(i64)]
     L.mafx_llg_integrate_dense.argtypes = [vp, vp, dp, C.c_double, C.c_double, dp, i64, C.POINTER(i64)]
     L.mafx_llg_get_stats.argtypes = [vp, C.POINTER(LlgStats)]
     L.mafx_llg_set_controller_state.argtypes = [vp, C.c_double, C.c_double, C.c_int]

@@ -422,6 +422,9 @@ def test_vector_helpers():
 
 @pytest.mark.parametrize("scheme,mesh", [("DP45", (32, 8, 1, 4e-9, 4e-9, 3e-9)),
                                          ("RKF45", (32, 8, 1, 4e-9, 4e-9, 3e-9)),
+                                         ("BS45", (32, 8, 1, 4e-9, 4e-9, 3e-9)),
+                                         ("DP78", (32, 8, 1, 4e-9, 4e-9, 3e-9)),
+                                         ("BS23", (32, 8, 1, 4e-9, 4e-9, 3e-9)),
                                          ("DP45", (512, 2, 1, 4e-9, 4e-9, 3e-9))])
 def test_llg_step_replay(scheme, mesh):
     # identical controller decisions and states, step by step, on a C1-like film (the 512-cell
@@ -442,14 +445,47 @@ def test_llg_step_replay(scheme, mesh):
     # From h = 1.01e-15 the embedded error estimate of the first steps is rounding noise of the
     # stage sums, and the PI controller turns its relative differences into err^-0.14 changes of
     # h (~1e-8 between two FP64 summation orders).  The replay therefore requires the same
-    # accept/reject decisions, h within 1e-6 and the state within 1e-8.
+    # accept/reject decisions, h within 1e-6 and the state within 1e-8.  DP78's error sum has
+    # weights up to +-2.5 over 13 stages and amplifies that noise: h within 1e-5 there (2e-6 is
+    # seen at h ~ 3e-13), and after 40 steps the two runs sit ~1e-16 s apart in time, which at
+    # |dm/dt| ~ 1e10 / s moves the state by ~3e-7, so the state is compared within 1e-6.
+    htol = 1e-5 if scheme == "DP78" else 1e-6
+    mtol = 1e-6 if scheme == "DP78" else 1e-8
     for i in range(40):
         r = sim.step()
         llg.step(state)
         st = llg.stats()
-        assert abs(st["h_last"] - r["h"]) <= 1e-6 * r["h"], (i, st["h_last"], r["h"])
-        assert abs(state.t - sim.t) <= 1e-6 * sim.t
+        assert abs(st["h_last"] - r["h"]) <= htol * r["h"], (i, st["h_last"], r["h"])
+        assert abs(state.t - sim.t) <= htol * sim.t
         assert st["rejected"] == r["rejected"] and st["accepted"] == r["accepted"]
+    assert np.max(np.abs(state.m - sim.m())) <= mtol
+
+
+def test_relax_matches_reference_loop():
+    """LLGIntegrator::relax (llg_integrator.cpp:43-66) on the GPU against the same loop driven
+    through the oracle: E_prev = 1e20; while |(E_prev - E) / E_prev| > precision: E_prev = E,
+    ncalcE steps.  Same number of steps, same final state."""
+    mesh = (32, 8, 1, 4e-9, 4e-9, 3e-9)
+    rng = np.random.default_rng(5)
+    m = np.zeros(mesh[:3] + (3,))
+    m[..., 0] = 1
+    m += 0.3 * rng.standard_normal(m.shape)
+    specs = [("demag", {}), ("exchange", {"A": 1.3e-11}), ("zeeman_h", {"h": (0.0, 2e4, 0.0)})]
+    state, terms, sim = build(mesh, 8e5, m, specs)
+    M = mf()
+    llg = M.LLGIntegrator(1.0, terms, "RKF45")
+    n_gpu = llg.relax(state, precision=1e-6, ncalcE=10, nprint=0, verbose=False)
+    sim.llg_init(1.0, "RKF45")
+    E_prev, E = 1e20, sim.energy()[0]
+    n_ref = 0
+    while abs((E_prev - E) / E_prev) > 1e-6:
+        E_prev = E
+        for _ in range(10):
+            sim.step()
+            n_ref += 1
+        E = sim.energy()[0]
+    assert n_gpu == n_ref and n_gpu > 10
+    assert abs(state.t - sim.t) <= 1e-6 * sim.t
     assert np.max(np.abs(state.m - sim.m())) <= 1e-8
 
 
