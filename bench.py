@@ -51,12 +51,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--heff-only", choices=["f32", "f64"], default=None,
                     help="run --steps H_eff evaluations of one path and exit (for profiler runs)")
-    ap.add_argument("--config", choices=["C1", "C2traj", "C3", "C4", "C5"], default=None,
+    ap.add_argument("--config", choices=["C1", "C2traj", "C3", "C4", "C5", "C5full"], default=None,
                     help="run one secondary configuration and print its JSON line instead of the main one")
     ap.add_argument("--c4-samples", type=int, default=4096, help="C4: training films (64x64x1)")
     ap.add_argument("--c4-epochs", type=int, default=20, help="C4: training epochs")
     ap.add_argument("--c4-eval", type=int, default=4, help="C4: 256x256x1 evaluation films")
-    ap.add_argument("--c4-weights", default=None, help="C4: write the trained parameters here (.npy)")
+    ap.add_argument("--c4-weights", default=None, help="C4: write the trained parameters here (.npy); "
+                    "C5full: read them from here when the file exists")
+    ap.add_argument("--c5-samples", type=int, default=256, help="C5full: training films when no weights are given")
+    ap.add_argument("--c5-lbfgs-iters", type=int, default=4000, help="C5full: LBFGS iteration bound")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the library has no multi-GPU path; run with --gpus 1", file=sys.stderr)
@@ -397,6 +400,66 @@ def config_c5(M, torch, args):
     return {"config": "C5 2048x1024x8 RKKY multilayer, demag+RKKY+DMI+uniaxial+50 mT", **out}
 
 
+def config_c5full(M, torch, args):
+    """C5 in its full form: a learned warm start of the 2048x1024x8 RKKY multilayer (network trained
+    as in C4 on a small set of relaxed 64x64x1 films, applied plane by plane), LBFGS on the FP32-field
+    path to max|m x H| / max|H| <= 1e-6 (bounded iterations), then 50 ps of DP45 (FP32 field, FP64
+    state, alpha = 0.05) under the 50 mT in-plane field."""
+    import magnumaf.warmstart as WS
+    nx, ny, nz, d = 2048, 1024, 8, 2e-9
+    out = {}
+    t0 = time.perf_counter()
+    if args.c4_weights and os.path.exists(args.c4_weights):
+        net = WS.WarmStartNet.load(args.c4_weights)
+        out["network"] = f"loaded from {args.c4_weights}"
+    else:
+        data = WS.generate_dataset(args.c5_samples, seed=1, size=64, torque_tol=1e-4, log=log)
+        net = WS.WarmStartNet()
+        WS.train(net, data, epochs=args.c4_epochs, batch=32, lr=1e-3, log=log)
+        out["network"] = f"trained on {args.c5_samples} films of 64x64x1, {args.c4_epochs} epochs"
+    out["network_s"] = time.perf_counter() - t0
+    log(f"[C5full] network ready in {out['network_s']:.1f} s ({out['network']})")
+    mesh = M.Mesh(nx, ny, nz, d, d, d)
+    m0 = workload(nx, ny, nz, seed=5)
+    rk = np.full((nx, ny, nz), -1e-3 * d / 2.0)
+    ex = np.full((nx, ny, nz), 1.3e-11)
+    terms = [M.DemagField(mesh), M.RKKYExchangeField(rk, ex, mesh), M.DmiField(2.5e-3, [0, 0, 1]),
+             M.UniaxialAnisotropyField(7e5, [0, 0, 1]), M.ExternalField(homogeneous=(50e-3 / MU0, 0, 0))]
+    state = M.State(mesh, 1.0e6, m0)
+    t0 = time.perf_counter()
+    state._context(terms, "f32")
+    out["context_s"] = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    WS.warm_start_layers(net, state, {"Ms": 1.0e6, "A": 1.3e-11, "D": 2.5e-3, "Ku": 7e5, "Bz": 0.0})
+    torch.cuda.synchronize()
+    out["warm_start_s"] = time.perf_counter() - t0
+    mini = M.LBFGS_Minimizer(terms, tol=1e-20, maxiter=args.c5_lbfgs_iters, verbose=2, torque_tol=1e-6,
+                             precision="f32")
+    t0 = time.perf_counter()
+    E = mini.minimize(state)
+    torch.cuda.synchronize()
+    out["lbfgs_s"] = time.perf_counter() - t0
+    st = mini.last_stats
+    out.update(lbfgs_iterations=st["iterations"], lbfgs_evals=st["energy_evals"], lbfgs_stop_reason=st["stop_reason"],
+               lbfgs_final_torque_rel=st["final_torque_rel"], lbfgs_energy=E)
+    log(f"[C5full] LBFGS: {st['iterations']} iterations in {out['lbfgs_s']:.1f} s, torque {st['final_torque_rel']:.3e}, "
+        f"stop reason {st['stop_reason']}")
+    llg = M.LLGIntegrator(0.05, terms, "DP45", precision="f32")
+    t_start = state.t
+    t0 = time.perf_counter()
+    steps = 0
+    for k in range(1, 11):
+        steps += llg.run_until(state, t_start + k * 5e-12)
+        log(f"[C5full] DP45 t = {state.t - t_start:.3e} s, {steps} steps, <m> = {state.mean_m()}")
+    torch.cuda.synchronize()
+    out["dp45_s"] = time.perf_counter() - t0
+    out.update(dp45_steps=steps, dp45_simulated_s=state.t - t_start, dp45_steps_per_s=steps / out["dp45_s"],
+               mean_m_final=[float(v) for v in state.mean_m()], rejected=llg.stats()["rejected"])
+    out["end_to_end_s"] = out["warm_start_s"] + out["lbfgs_s"] + out["dp45_s"]
+    return {"config": "C5 full form: warm start + LBFGS + 50 ps DP45, 2048x1024x8 RKKY multilayer", **out}
+
+
 def config_c4(M, torch, args):
     """Learned warm start: FP64 LBFGS relaxations of seeded 64x64x1 films as targets, training of
     the encoder-decoder on the GPU, then warm start + LBFGS vs plain LBFGS at 256x256x1."""
@@ -422,7 +485,8 @@ def config_c4(M, torch, args):
             "equal_final_energy_count": int(sum(same)), "lbfgs_iterations_median_dataset": float(np.median(its))}
 
 
-CONFIGS = {"C1": config_c1, "C2traj": config_c2traj, "C3": config_c3, "C4": config_c4, "C5": config_c5}
+CONFIGS = {"C1": config_c1, "C2traj": config_c2traj, "C3": config_c3, "C4": config_c4, "C5": config_c5,
+           "C5full": config_c5full}
 
 
 if __name__ == "__main__":

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <limits>
 
 #include "mafx_internal.hpp"
@@ -230,7 +231,6 @@ int cvsrch(Lbfgs& L, double* m, const double* wa, double& f, double* g, double& 
 // lbfgs_minimizer.cpp:55-198
 int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter, int verbose, double torque_tol,
                    double* E_out, mafx_lbfgs_stats* stats) {
-    (void)verbose;
     Lbfgs L{ctx, 3 * ctx->N};
     const int64_t n3 = L.n3;
     const size_t bytes = sizeof(double) * n3;
@@ -262,6 +262,12 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         }
         return MAFX_OK;
     };
+    // lbfgs_minimizer.cpp:67-69,142-146: verbose > 0 prints the start energy, verbose > 1 one
+    // line per iteration (iteration, energy, max |gradient|, evaluations so far, step length)
+    if (verbose > 0) {
+        std::printf("f= %.17g\n", f);
+        std::fflush(stdout);
+    }
     if (gradNorm < (epsr * (1 + std::fabs(f)))) return finish(1);
     std::array<double, hist> alpha{};
     double f_old = 0.0;
@@ -296,6 +302,11 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         cvsrch(L, m, x_old.as<double>(), f, grad.as<double>(), rate, dir.as<double>(), tolerance);
         const double f1 = 1 + std::fabs(f);
         gradNorm = L.maxabs(grad.as<double>());
+        if (verbose > 1) {
+            std::printf("bb> %zu %.17g  %.17g %lld %.17g\n", globIter, f, gradNorm, static_cast<long long>(L.evals),
+                        rate);
+            std::fflush(stdout);
+        }
         if (gradNorm < (epsr * f1)) {
             globIter++;
             reason = 1;

@@ -133,6 +133,22 @@ class WarmStartNet:
         state.m_dev.copy_(out)
 
 
+def warm_start_layers(net, state, p):
+    """The network's prediction, z-plane by z-plane, for a state of nz >= 1 planes: each plane is
+    fed to the network as a film with the parameters p (the network has no input for couplings
+    between planes or for in-plane fields), and the predicted unit vectors overwrite state.m."""
+    torch = _torch()
+    mesh = state.mesh
+    nx, ny, nz = mesh.nx, mesh.ny, mesh.nz
+    m = state.m_dev.view(3, nz, ny, nx)
+    pl = torch.from_numpy(planes(p)).to(m.device)
+    x = torch.empty((1, 8, ny, nx), dtype=torch.float32, device=m.device)
+    for z in range(nz):
+        x[0, :3] = m[:, z].float()
+        x[0, 3:] = pl[:, None, None]
+        m[:, z] = net.forward(x)[0].double()
+
+
 def input_planes(m0, p):
     """x [8, ny, nx] float32 from m0 (nx, ny, 1, 3) and the parameter dict."""
     nx, ny = m0.shape[0], m0.shape[1]
