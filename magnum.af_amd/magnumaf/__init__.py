@@ -239,8 +239,24 @@ class State:
         return self.mean_m(2)
 
     def __str__(self):
-        mx, my, mz = self.mean_m()
-        return f"{self.t} {mx} {my} {mz}"
+        """'t<TAB><mx><TAB><my><TAB><mz>' as the reference's operator<< prints it (state.cpp:26-30)."""
+        from . import io
+        return io.stream_line(self.t, self.mean_m())
+
+    def write_vti(self, outputname):
+        """State::write_vti (state.cpp:139): m as cell data of a .vti image grid."""
+        from . import io
+        return io.write_vti(outputname, self.m, (self.mesh.dx, self.mesh.dy, self.mesh.dz))
+
+    def read_vti(self, inputname):
+        """State::_vti_reader (state.cpp:141): m (not renormalised) from a .vti file of this mesh."""
+        from . import io
+        m, _ = io.read_vti(inputname)
+        if m.shape[:3] != (self.mesh.nx, self.mesh.ny, self.mesh.nz):
+            raise ValueError(f"read_vti: file holds {m.shape[:3]} cells, the mesh {self.mesh.nx, self.mesh.ny, self.mesh.nz}")
+        self.set_m_raw(m)
+
+    _vti_reader = read_vti
 
 
 # ---------------------------------------------------------------- field terms
@@ -527,10 +543,42 @@ class LLGIntegrator:
         state.t = t.value
         rows = out[: 4 * nr.value].reshape(nr.value, 4)
         if filename:
+            from . import io
             with open(filename, "a" if append else "w") as f:
                 for r in rows:
-                    f.write(f"{r[0]!r}\t{r[1]!r}\t{r[2]!r}\t{r[3]!r}\n")
+                    f.write(io.stream_line(r[0], r[1:]) + "\n")
         return rows
+
+    def save_state(self, path, state):
+        """Writes a resume file (.npz): m bit for bit, t, steps, this integrator's scheme and
+        parameters and its controller state, so that load_state + step continues exactly as the
+        uninterrupted run would.  The step counters of stats() are not part of it."""
+        self._bind(state)
+        h, errold, reject = self.controller_state()
+        g = state.mesh
+        np.savez(path, m=state.m_dev.cpu().numpy(), t=state.t, steps=state.steps,
+                 mesh=np.array([g.nx, g.ny, g.nz], dtype=np.int64), spacing=np.array([g.dx, g.dy, g.dz]),
+                 scheme=np.array(self.mode), alpha=self._alpha,
+                 params=np.array([self.hmin, self.hmax, self.atol, self.rtol]),
+                 dissipation_term_only=int(self.dissipation_term_only), h=h, errold=errold, reject=reject,
+                 accumulated_steps=self.accumulated_steps)
+
+    def load_state(self, path, state):
+        """Restores state (m, t, steps) and this integrator's controller state from save_state's
+        file.  The mesh, scheme and controller parameters must match."""
+        with np.load(path, allow_pickle=False) as f:
+            g = state.mesh
+            if tuple(f["mesh"]) != (g.nx, g.ny, g.nz) or not np.array_equal(f["spacing"], [g.dx, g.dy, g.dz]):
+                raise ValueError("load_state: the file was written for another mesh")
+            if str(f["scheme"]) != self.mode or not np.array_equal(f["params"], [self.hmin, self.hmax, self.atol,
+                                                                                 self.rtol]):
+                raise ValueError("load_state: the file was written by an integrator with another scheme or controller")
+            state.m_dev.copy_(_torch().from_numpy(np.ascontiguousarray(f["m"])))
+            state.t = float(f["t"])
+            state.steps = int(f["steps"])
+            self._bind(state)
+            check(lib().mafx_llg_set_controller_state(self._llg, float(f["h"]), float(f["errold"]), int(f["reject"])))
+            self.accumulated_steps = int(f["accumulated_steps"])
 
     def H_in_Apm(self, state):
         ctx = state._context(self.terms, self.precision)

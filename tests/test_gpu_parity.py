@@ -513,6 +513,47 @@ def test_relax_matches_reference_loop():
     assert np.max(np.abs(state.m - sim.m())) <= 1e-8
 
 
+@pytest.mark.parametrize("scheme", ["DP45", "RKF45"])
+def test_resume_is_step_exact(scheme, tmp_path):
+    """save_state after 12 steps, then a fresh State and LLGIntegrator load it and take 12 more:
+    the same m to the last bit, the same t and step size as 24 uninterrupted steps."""
+    mesh = (32, 8, 1, 4e-9, 4e-9, 3e-9)
+    rng = np.random.default_rng(11)
+    m = np.zeros(mesh[:3] + (3,))
+    m[..., 0] = 1
+    m += 0.2 * rng.standard_normal(m.shape)
+    M = mf()
+    g = M.Mesh(*mesh)
+
+    def terms():
+        return [M.DemagField(g), M.ExchangeField(1.3e-11), M.ExternalField(homogeneous=(-2e4, 3e3, 0.0))]
+
+    a = M.State(g, 8e5, m)
+    la = M.LLGIntegrator(0.02, terms(), scheme)
+    for _ in range(24):
+        la.step(a)
+    b = M.State(g, 8e5, m)
+    lb = M.LLGIntegrator(0.02, terms(), scheme)
+    for _ in range(12):
+        lb.step(b)
+    path = tmp_path / "resume.npz"
+    lb.save_state(path, b)
+    c = M.State(g, 8e5, np.ones(mesh[:3] + (3,)))
+    lc = M.LLGIntegrator(0.02, terms(), scheme)
+    lc.load_state(path, c)
+    assert c.t == b.t and c.steps == 12
+    for _ in range(12):
+        lc.step(c)
+    assert np.array_equal(c.m.view(np.uint64), a.m.view(np.uint64))
+    assert c.t == a.t and lc.controller_state() == la.controller_state()
+    # the .vti form of the same state reads back to the last bit
+    fn = c.write_vti(tmp_path / "c")
+    d = M.State(g, 8e5, np.ones(mesh[:3] + (3,)))
+    d.read_vti(fn)
+    assert np.array_equal(d.m, c.m)
+    assert str(c).count("\t") == 3
+
+
 def test_sp4_relaxation_and_switch_gpu():
     # python/test/sp4.py:21-62 on the GPU FP64 path (RKF45, the LLGIntegrator default)
     M = mf()
