@@ -18,7 +18,7 @@ ARCH = os.environ.get("MAFX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-             "-packed-fp32-ops", "-Wall", "-Wno-unused-result", "-ffp-contract=fast"]
+             "-packed-fp32-ops", "-Wall", "-Wno-unused-result", "-ffp-contract=fast-honor-pragmas"]
 GXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra", "-pthread"]
 
 HIP_SOURCES = ["fft.hip", "field.hip", "vec.hip", "net.hip", "llg.cpp", "lbfgs.cpp", "capi.cpp"]

@@ -731,14 +731,24 @@ void evaluate(mafx_ctx* ctx, const EvalArgs& a) {
 
 // Reduces the partials of the last evaluate() (energy slots and, in gradient mode, the two
 // maxima) to host values.  E_terms is indexed by the context's term order.
-void finish_energy(mafx_ctx* ctx, uint32_t mask, double* E_terms, double* E_total) {
+int finish_energy_launch(mafx_ctx* ctx) {
     const int nt = ctx->scal_nterms;
     const int nslots = nt + 2;
     hipLaunchKernelGGL(k_reduce_slots, dim3(nslots), dim3(kBlock), 0, ctx->stream, ctx->partials.as<double>(),
                        ctx->scal_nblocks, nslots, nt, ctx->scal.as<double>());
     MAFX_HIP(hipGetLastError());
+    return nslots;
+}
+
+void finish_energy(mafx_ctx* ctx, uint32_t mask, double* E_terms, double* E_total) {
+    const int nslots = finish_energy_launch(ctx);
     MAFX_HIP(hipMemcpyAsync(ctx->host_scal, ctx->scal.p, sizeof(double) * nslots, hipMemcpyDeviceToHost, ctx->stream));
     MAFX_HIP(hipStreamSynchronize(ctx->stream));
+    finish_energy_combine(ctx, mask, E_terms, E_total);
+}
+
+void finish_energy_combine(mafx_ctx* ctx, uint32_t mask, double* E_terms, double* E_total) {
+    const int nt = ctx->scal_nterms;
     const mafx_mesh& g = ctx->mesh;
     double total = 0;
     int slot = 0;

@@ -38,6 +38,30 @@ struct Lbfgs {
         ++evals;
         return E;
     }
+    // The line search's trial: energy and gradient, plus g . s and max |g| reduced on the device and
+    // read back with the energy partials in one transfer (one host synchronisation per trial).
+    double energy_gradient_dg(const double* m, double* g, const double* s, double& dg, double& gmax) {
+        EvalArgs ea;
+        ea.m = m;
+        ea.mode = OutMode::GRAD;
+        ea.out = g;
+        ea.want_energy = true;
+        evaluate(ctx, ea);
+        const int ns = finish_energy_launch(ctx);
+        double* sc = ctx->scal.as<double>();
+        dot_dev(ctx, g, s, n3, sc + ns);
+        max_abs_dev(ctx, g, n3, sc + ns + 1);
+        MAFX_HIP(hipMemcpyAsync(ctx->host_scal, sc, sizeof(double) * (ns + 2), hipMemcpyDeviceToHost, ctx->stream));
+        MAFX_HIP(hipStreamSynchronize(ctx->stream));
+        double E = 0;
+        finish_energy_combine(ctx, ~0u, nullptr, &E);
+        dg = ctx->host_scal[ns];
+        gmax = ctx->host_scal[ns + 1];
+        last_torque = ctx->last_max_torque;
+        last_hmax = ctx->last_max_h;
+        ++evals;
+        return E;
+    }
     double mydot(const double* a, const double* b) { return dot(ctx, a, b, n3); }
     double mynorm(const double* a) { return std::sqrt(mydot(a, a)); }
     double maxabs(const double* a) { return max_abs(ctx, a, n3); }
@@ -148,8 +172,10 @@ int cstep(double& stx, double& fx, double& dx, double& sty, double& fy, double& 
 }
 
 // lbfgs_minimizer.cpp:213-360; m is overwritten with normalize(wa + stp s) at every trial.
+// dginit = g . s is handed in by the caller (the two-loop's phiPrime0, the same dot product with
+// the sign flipped, exactly); gmax returns max |g| of the last trial's gradient.
 int cvsrch(Lbfgs& L, double* m, const double* wa, double& f, double* g, double& stp, const double* s,
-           const double tolf) {
+           const double tolf, const double dginit, double& gmax) {
     int info = 0;
     int infoc = 1;
     const double xtol = 1e-15;
@@ -161,7 +187,6 @@ int cvsrch(Lbfgs& L, double* m, const double* wa, double& f, double* g, double& 
     const double xtrapf = 4;
     const int maxfev = 20;
     int nfev = 0;
-    const double dginit = L.mydot(g, s);
     if (dginit >= 0.0) return -1;
     bool brackt = false;
     bool stage1 = true;
@@ -187,9 +212,9 @@ int cvsrch(Lbfgs& L, double* m, const double* wa, double& f, double* g, double& 
             (brackt & (stmax - stmin <= xtol * stmax)))
             stp = stx;
         line_point(L.ctx, m, wa, stp, s);
-        f = L.energy_and_gradient(m, g);
+        double dg = 0;
+        f = L.energy_gradient_dg(m, g, s, dg, gmax);
         nfev++;
-        const double dg = L.mydot(g, s);
         const double ftest1 = finit + stp * dgtest;
         const double ftest2 = finit + eps * std::fabs(finit);
         const double ft = 2 * ftol - 1;
@@ -242,6 +267,17 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         sV[i].alloc(bytes); // read only after being written (k <= iter)
         yV[i].alloc(bytes);
     }
+    // rho_i = 1 / (s_i . y_i) of the stored pairs: the reference recomputes the dot product at every
+    // use (lbfgs_minimizer.cpp:88,98); it is the y . s computed when the pair was stored (the dot
+    // is commutative and deterministic), kept here.  Device scalars of the two-loop recursion:
+    // [0, hist) alpha_i, [hist] the current dot, [hist + 1] the axpy coefficient, [hist + 2, +3)
+    // y . s, y . y, s . s of a new pair.
+    std::array<double, hist> rho{};
+    DevBuf dsc;
+    dsc.alloc(sizeof(double) * 16);
+    double* ds = dsc.as<double>();
+    double* d_dot = ds + hist;
+    double* d_coef = ds + hist + 1;
     const double eps = 2.22e-16;
     const double eps2 = std::sqrt(eps);
     const double epsr = std::pow(eps, 0.9);
@@ -269,7 +305,6 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         std::fflush(stdout);
     }
     if (gradNorm < (epsr * (1 + std::fabs(f)))) return finish(1);
-    std::array<double, hist> alpha{};
     double f_old = 0.0;
     size_t iter = 0;
     double H0k = 1;
@@ -279,18 +314,23 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         copy(ctx, grad_old.as<double>(), grad.as<double>(), n3);
         copy(ctx, q.as<double>(), grad.as<double>(), n3);
         const int k = static_cast<int>(std::min(hist, iter));
+        // the two-loop recursion with its scalars on the device: alpha_i = rho_i s_i . q,
+        // q -= alpha_i y_i; then beta = rho_i y_i . q, q += (alpha_i - beta) s_i
         for (int i = k - 1; i >= 0; i--) {
-            const double rho = 1.0 / L.mydot(sV[i].as<double>(), yV[i].as<double>());
-            alpha[i] = rho * L.mydot(sV[i].as<double>(), q.as<double>());
-            axpy(ctx, q.as<double>(), -alpha[i], yV[i].as<double>(), n3);
+            dot_dev(ctx, sV[i].as<double>(), q.as<double>(), n3, d_dot);
+            lbfgs_alpha(ctx, ds + i, d_coef, d_dot, rho[i]);
+            axpy_dev(ctx, q.as<double>(), d_coef, yV[i].as<double>(), n3);
         }
         scale(ctx, q.as<double>(), H0k, n3);
         for (int i = 0; i < k; i++) {
-            const double rho = 1.0 / L.mydot(sV[i].as<double>(), yV[i].as<double>());
-            const double beta = rho * L.mydot(yV[i].as<double>(), q.as<double>());
-            axpy(ctx, q.as<double>(), alpha[i] - beta, sV[i].as<double>(), n3);
+            dot_dev(ctx, yV[i].as<double>(), q.as<double>(), n3, d_dot);
+            lbfgs_beta(ctx, d_coef, ds + i, d_dot, rho[i]);
+            axpy_dev(ctx, q.as<double>(), d_coef, sV[i].as<double>(), n3);
         }
         const double phiPrime0 = -L.mydot(grad.as<double>(), q.as<double>());
+        // the line search's g . dir for dir = -q is phiPrime0 exactly (every product and partial
+        // sum only changes sign); after the reset to q = grad it is computed
+        double dginit = phiPrime0;
         if (phiPrime0 > 0) {
             copy(ctx, q.as<double>(), grad.as<double>(), n3);
             iter = 0;
@@ -298,10 +338,12 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         // searchDir = -q (multiplying by -1 is exact)
         copy(ctx, dir.as<double>(), q.as<double>(), n3);
         scale(ctx, dir.as<double>(), -1.0, n3);
+        if (phiPrime0 > 0) dginit = L.mydot(grad.as<double>(), dir.as<double>());
         double rate = 1.0;
-        cvsrch(L, m, x_old.as<double>(), f, grad.as<double>(), rate, dir.as<double>(), tolerance);
+        double gmax = -1.0;
+        cvsrch(L, m, x_old.as<double>(), f, grad.as<double>(), rate, dir.as<double>(), tolerance, dginit, gmax);
         const double f1 = 1 + std::fabs(f);
-        gradNorm = L.maxabs(grad.as<double>());
+        gradNorm = gmax >= 0.0 ? gmax : L.maxabs(grad.as<double>());
         if (verbose > 1) {
             std::printf("bb> %zu %.17g  %.17g %lld %.17g\n", globIter, f, gradNorm, static_cast<long long>(L.evals),
                         rate);
@@ -325,20 +367,29 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
             break;
         }
         sub(ctx, y.as<double>(), grad.as<double>(), grad_old.as<double>(), n3);
-        const double ys = L.mydot(y.as<double>(), s.as<double>());
-        if (ys <= eps2 * L.mynorm(y.as<double>()) * L.mynorm(s.as<double>())) {
+        // y . s, |y|^2 and |s|^2 (lbfgs_minimizer.cpp:160-170) in one transfer
+        dot_dev(ctx, y.as<double>(), s.as<double>(), n3, ds + hist + 2);
+        dot_dev(ctx, y.as<double>(), y.as<double>(), n3, ds + hist + 3);
+        dot_dev(ctx, s.as<double>(), s.as<double>(), n3, ds + hist + 4);
+        double pr[3];
+        read_device(ctx, ds + hist + 2, 3, pr);
+        const double ys = pr[0], yy = pr[1], ss = pr[2];
+        if (ys <= eps2 * std::sqrt(yy) * std::sqrt(ss)) {
             // Dennis-Schnabel 9.4.1: skip the update
         } else {
             if (iter < hist) {
                 copy(ctx, sV[iter].as<double>(), s.as<double>(), n3);
                 copy(ctx, yV[iter].as<double>(), y.as<double>(), n3);
+                rho[iter] = 1.0 / ys;
             } else {
                 std::rotate(sV.begin(), sV.begin() + 1, sV.end());
                 std::rotate(yV.begin(), yV.begin() + 1, yV.end());
+                std::rotate(rho.begin(), rho.begin() + 1, rho.end());
                 copy(ctx, sV[hist - 1].as<double>(), s.as<double>(), n3);
                 copy(ctx, yV[hist - 1].as<double>(), y.as<double>(), n3);
+                rho[hist - 1] = 1.0 / ys;
             }
-            H0k = ys / L.mydot(y.as<double>(), y.as<double>());
+            H0k = ys / yy;
             iter++;
         }
         globIter++;

@@ -181,6 +181,10 @@ struct EvalArgs {
 // are reduced by finish_energy.
 void evaluate(mafx_ctx* ctx, const EvalArgs& a);
 void finish_energy(mafx_ctx* ctx, uint32_t mask, double* E_terms, double* E_total);
+// The two halves of finish_energy: the reduction into ctx->scal[0, slots) (returns the slot count),
+// and the term energies from those slots once they are in ctx->host_scal.
+int finish_energy_launch(mafx_ctx* ctx);
+void finish_energy_combine(mafx_ctx* ctx, uint32_t mask, double* E_terms, double* E_total);
 void build_tensor(mafx_ctx* ctx);
 
 // ------------------------------------------------------------------ vector kernels (vec.hip)

@@ -294,6 +294,66 @@ void mean_m(mafx_ctx* ctx, const double* m, double out[3]) {
     for (int c = 0; c < 3; ++c) out[c] = ctx->host_scal[c] / ctx->host_scal[3];
 }
 
+namespace {
+__global__ void k_axpy_dev(double* __restrict__ y, const double* __restrict__ a, const double* __restrict__ x, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    if (i < n) y[i] = y[i] + (*a) * x[i];
+}
+// LBFGS two-loop scalars (lbfgs_minimizer.cpp:86-108): first loop alpha_i = rho d, coef = -alpha_i;
+// second loop beta = rho d, coef = alpha_i - beta
+__global__ void k_lbfgs_alpha(double* alpha_i, double* coef, const double* d, double rho) {
+    const double a = rho * (*d);
+    *alpha_i = a;
+    *coef = -a;
+}
+__global__ void k_lbfgs_beta(double* coef, const double* alpha_i, const double* d, double rho) {
+    // beta is rounded before the subtraction, as the reference's host arithmetic does; a fused
+    // multiply-subtract changes the iterates by an ulp and, over thousands of iterations, the path
+#pragma clang fp contract(off)
+    const double beta = rho * (*d);
+    *coef = *alpha_i - beta;
+}
+} // namespace
+
+void dot_dev(mafx_ctx* ctx, const double* a, const double* b, int64_t n, double* out_dev) {
+    const unsigned nb = red_blocks(n);
+    ProfScope ps(ctx, "dot_f64", 16.0 * n);
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kB), 0, ctx->stream, a, b, n, ctx->partials.as<double>());
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(), static_cast<int>(nb),
+                       1, 1, out_dev);
+    MAFX_HIP(hipGetLastError());
+}
+
+void max_abs_dev(mafx_ctx* ctx, const double* a, int64_t n, double* out_dev) {
+    const unsigned nb = red_blocks(n);
+    ProfScope ps(ctx, "maxabs_f64", 8.0 * n);
+    hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(kB), 0, ctx->stream, a, n, ctx->partials.as<double>());
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(), static_cast<int>(nb),
+                       1, 0, out_dev);
+    MAFX_HIP(hipGetLastError());
+}
+
+void axpy_dev(mafx_ctx* ctx, double* y, const double* a_dev, const double* x, int64_t n) {
+    hipLaunchKernelGGL(k_axpy_dev, dim3(grid_for(n, kB)), dim3(kB), 0, ctx->stream, y, a_dev, x, n);
+    MAFX_HIP(hipGetLastError());
+}
+
+void lbfgs_alpha(mafx_ctx* ctx, double* alpha_i, double* coef, const double* d, double rho) {
+    hipLaunchKernelGGL(k_lbfgs_alpha, dim3(1), dim3(1), 0, ctx->stream, alpha_i, coef, d, rho);
+    MAFX_HIP(hipGetLastError());
+}
+
+void lbfgs_beta(mafx_ctx* ctx, double* coef, const double* alpha_i, const double* d, double rho) {
+    hipLaunchKernelGGL(k_lbfgs_beta, dim3(1), dim3(1), 0, ctx->stream, coef, alpha_i, d, rho);
+    MAFX_HIP(hipGetLastError());
+}
+
+void read_device(mafx_ctx* ctx, const double* src_dev, int n, double* dst_host) {
+    MAFX_HIP(hipMemcpyAsync(ctx->host_scal, src_dev, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    MAFX_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i) dst_host[i] = ctx->host_scal[i];
+}
+
 void llg_rhs(mafx_ctx* ctx, double alpha, double dt, const double* m, const double* h, double* k) {
     hipLaunchKernelGGL(k_llg_rhs, dim3(grid_for(ctx->N, kB)), dim3(kB), 0, ctx->stream, m, h, k, ctx->N, alpha, dt);
     MAFX_HIP(hipGetLastError());

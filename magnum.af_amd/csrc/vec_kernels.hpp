@@ -18,5 +18,14 @@ void scale(mafx_ctx* ctx, double* y, double a, int64_t n);
 void sub(mafx_ctx* ctx, double* out, const double* a, const double* b, int64_t n);
 void copy(mafx_ctx* ctx, double* dst, const double* src, int64_t n);
 int64_t reduction_partials_needed(int64_t n);
+// Device-side scalars (no host synchronisation): *out_dev = a . b and max |a| with the kernels and
+// reduction tree of dot() / max_abs(), y += (*a_dev) x, the LBFGS two-loop recurrences
+// (alpha_i = rho d, coef = -alpha_i; coef = alpha_i - rho d), and one read of n device doubles.
+void dot_dev(mafx_ctx* ctx, const double* a, const double* b, int64_t n, double* out_dev);
+void max_abs_dev(mafx_ctx* ctx, const double* a, int64_t n, double* out_dev);
+void axpy_dev(mafx_ctx* ctx, double* y, const double* a_dev, const double* x, int64_t n);
+void lbfgs_alpha(mafx_ctx* ctx, double* alpha_i, double* coef, const double* d, double rho);
+void lbfgs_beta(mafx_ctx* ctx, double* coef, const double* alpha_i, const double* d, double rho);
+void read_device(mafx_ctx* ctx, const double* src_dev, int n, double* dst_host);
 
 } // namespace mafx
