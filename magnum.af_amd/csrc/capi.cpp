@@ -172,6 +172,10 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
             throw Error(MAFX_ERR_DEVICE, "no HIP device visible: the MI355X path needs a GPU");
         auto ctx = std::make_unique<mafx_ctx>();
+        int dev = 0, cus = 0;
+        MAFX_HIP(hipGetDevice(&dev));
+        MAFX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        if (cus > 0) ctx->num_cus = cus;
         ctx->mesh = *mesh;
         ctx->N = mesh->nx * mesh->ny * mesh->nz;
         ctx->Ms = Ms;

@@ -231,6 +231,9 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
     }
 }
 
+// Resident 512-lane groups per CU of the persistent x forward transform (LDS: three 41 KiB tiles)
+constexpr int kXGroupsPerCU = 3;
+
 // Compile-time shaped forms of k_fft_x_fwd / k_fft_x_inv for the demag field: power-of-two
 // M = nx = n0 / 2, rows of exactly M reals, W = 2^(LOGE - LOGM) rows per work-group taken from
 // one (c, z) plane (ny % W == 0), so a group covers rows y0 .. y0 + W - 1 of one plane and every
@@ -238,11 +241,11 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
 // Packed input pairs (x[2n], x[2n+1]), n < M / 2, are single 16-byte loads; n >= M / 2 is the
 // zero padding.  Spectrum rows are written and read as in the generic kernels: column k of row y
 // at tcol(k, n1T) + 4 y, so the W rows of a column tile form 4 W contiguous elements.
-template <class C, int LOGM, int LOGE, int BS, int MINW>
+template <class C, int LOGM, int LOGE, int BS, int MINW, bool MS>
 __global__ void __launch_bounds__(BS, MINW)
     k_fft_x_fwd_ct(const double* __restrict__ in, const double* __restrict__ ms, double scale, int ny, FastDiv gyd,
                    FastDiv nzd, const C* __restrict__ twh, const C* __restrict__ twL, C* __restrict__ out,
-                   int64_t plane, int64_t n2o, int n1T, int64_t N) {
+                   int64_t plane, int64_t n2o, int n1T, int64_t N, int ngroups) {
     using T = typename real_of<C>::type;
     constexpr int M = 1 << LOGM, LOGW = LOGE - LOGM, W = 1 << LOGW, E = 1 << LOGE, MH = M / 2;
     static_assert(LOGW >= 0 && E % BS == 0 && (W * MH) % BS == 0, "whole lane loops");
@@ -250,49 +253,77 @@ __global__ void __launch_bounds__(BS, MINW)
     C* buf = reinterpret_cast<C*>(smem_raw);
     C* tws = buf + lds_padded(E);
     for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
-    const uint32_t grp = blockIdx.x, zc = fdiv(grp, gyd), yg = grp - zc * gyd.d;
-    const uint32_t c = fdiv(zc, nzd), z = zc - c * nzd.d;
-    const int y0 = static_cast<int>(yg) << LOGW;
-    const int64_t cell0 = (static_cast<int64_t>(z) * ny + y0) << LOGM; // cell of (x = 0, y0, z)
-    const double* src = in + static_cast<int64_t>(c) * N + cell0;
     const int tid = static_cast<int>(threadIdx.x);
     constexpr int IT = W * MH / BS;
+    // Persistent groups: group g takes row groups g, g + gridDim.x, ... and loads the next one's
+    // input into registers while the current one is transformed, so each CU keeps loads in flight
+    // through the butterfly stages.
+    double2 p[IT], q[MS ? IT : 1];
+    const auto cell_of = [&](int grp, uint32_t& c, uint32_t& z, int& y0) -> int64_t {
+        const uint32_t zc = fdiv(static_cast<uint32_t>(grp), gyd), yg = static_cast<uint32_t>(grp) - zc * gyd.d;
+        c = fdiv(zc, nzd);
+        z = zc - c * nzd.d;
+        y0 = static_cast<int>(yg) << LOGW;
+        return (static_cast<int64_t>(z) * ny + y0) << LOGM; // cell of (x = 0, y0, z)
+    };
+    const auto fetch = [&](int grp) {
+        uint32_t c, z;
+        int y0;
+        const int64_t cell0 = cell_of(grp, c, z, y0);
+        const double* src = in + static_cast<int64_t>(c) * N + cell0;
+        const int ft = opaque_int(tid);
 #pragma unroll
-    for (int i = 0; i < IT; ++i) {
-        const int g = tid + i * BS, w = g >> (LOGM - 1), n = g & (MH - 1);
-        const int o = (w << LOGM) + 2 * n;
-        const double2 p = *reinterpret_cast<const double2*>(src + o);
-        T re, im;
-        if (ms) {
-            const double2 q = *reinterpret_cast<const double2*>(ms + cell0 + o);
-            re = static_cast<T>(p.x * scale * q.x);
-            im = static_cast<T>(p.y * scale * q.y);
-        } else {
-            re = static_cast<T>(p.x * scale);
-            im = static_cast<T>(p.y * scale);
+        for (int i = 0; i < IT; ++i) {
+            const int g = ft + i * BS, w = g >> (LOGM - 1), n = g & (MH - 1);
+            const int o = (w << LOGM) + 2 * n;
+            p[i] = *reinterpret_cast<const double2*>(src + o);
+            if constexpr (MS) q[i] = *reinterpret_cast<const double2*>(ms + cell0 + o);
         }
-        buf[lds_pad((n << LOGW) + w)] = C{re, im};
-        buf[lds_pad(((n + MH) << LOGW) + w)] = C{0, 0};
-    }
-    __syncthreads();
-    ct_fft<C, LOGM, LOGW, BS, false>(buf, tws);
-    // X[k] = E + w_L^k O, E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2, k <= M
-    const T half = static_cast<T>(0.5);
-    C* dst = out + (static_cast<int64_t>(c) * n2o + z) * plane + kTile * y0;
-    constexpr int NOUT = (M / kTile + 1) * W * kTile, OT = (NOUT + BS - 1) / BS;
+    };
+    int grp = static_cast<int>(blockIdx.x);
+    fetch(grp);
+    for (; grp < ngroups; grp += static_cast<int>(gridDim.x)) {
+        const int lt = opaque_int(tid);
 #pragma unroll
-    for (int i = 0; i < OT; ++i) {
-        const int g = tid + i * BS;
-        if (g < NOUT) {
-            const int kk = g & 3, w = (g >> 2) & (W - 1), kt = g >> (LOGW + 2), k = kt * kTile + kk;
-            if (k <= M) {
-                const C Zk = buf[lds_pad(((k & (M - 1)) << LOGW) + w)];
-                const C Zm = buf[lds_pad((((M - k) & (M - 1)) << LOGW) + w)];
-                const C Ev{half * (Zk.x + Zm.x), half * (Zk.y - Zm.y)};
-                const C Ov{half * (Zk.y + Zm.y), -half * (Zk.x - Zm.x)};
-                dst[kt * n1T + kTile * w + kk] = cadd(Ev, cmul(twL[k], Ov));
+        for (int i = 0; i < IT; ++i) {
+            const int g = lt + i * BS, w = g >> (LOGM - 1), n = g & (MH - 1);
+            T re, im;
+            if constexpr (MS) {
+                re = static_cast<T>(p[i].x * scale * q[i].x);
+                im = static_cast<T>(p[i].y * scale * q[i].y);
+            } else {
+                re = static_cast<T>(p[i].x * scale);
+                im = static_cast<T>(p[i].y * scale);
+            }
+            buf[lds_pad((n << LOGW) + w)] = C{re, im};
+            buf[lds_pad(((n + MH) << LOGW) + w)] = C{0, 0};
+        }
+        __syncthreads();
+        const int next = grp + static_cast<int>(gridDim.x);
+        if (next < ngroups) fetch(next);
+        ct_fft<C, LOGM, LOGW, BS, false>(buf, tws, lt);
+        // X[k] = E + w_L^k O, E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2, k <= M
+        uint32_t c, z;
+        int y0;
+        cell_of(grp, c, z, y0);
+        const T half = static_cast<T>(0.5);
+        C* dst = out + (static_cast<int64_t>(c) * n2o + z) * plane + kTile * y0;
+        constexpr int NOUT = (M / kTile + 1) * W * kTile, OT = (NOUT + BS - 1) / BS;
+#pragma unroll
+        for (int i = 0; i < OT; ++i) {
+            const int g = lt + i * BS;
+            if (g < NOUT) {
+                const int kk = g & 3, w = (g >> 2) & (W - 1), kt = g >> (LOGW + 2), k = kt * kTile + kk;
+                if (k <= M) {
+                    const C Zk = buf[lds_pad(((k & (M - 1)) << LOGW) + w)];
+                    const C Zm = buf[lds_pad((((M - k) & (M - 1)) << LOGW) + w)];
+                    const C Ev{half * (Zk.x + Zm.x), half * (Zk.y - Zm.y)};
+                    const C Ov{half * (Zk.y + Zm.y), -half * (Zk.x - Zm.x)};
+                    dst[kt * n1T + kTile * w + kk] = cadd(Ev, cmul(twL[k], Ov));
+                }
             }
         }
+        __syncthreads(); // the next group's input overwrites buf
     }
 }
 
@@ -828,11 +859,15 @@ void fft_x_forward(mafx_ctx* ctx, const double* in, const double* ms, double sca
     if (ncomp == 3 && Xi == M && Yi == Yr && Zi == Zr && ctx->n0 == 2 * M && M >= 256 && M <= 2048 &&
         (M & (M - 1)) == 0 && Yr % std::max(1, (1 << LOGE) / M) == 0) {
         using KC = void (*)(const double*, const double*, double, int, FastDiv, FastDiv, const C*, const C*, C*, int64_t,
-                            int64_t, int, int64_t);
-        KC fc = M == 256 ? k_fft_x_fwd_ct<C, 8, LOGE, 512, 6>
-                : M == 512 ? k_fft_x_fwd_ct<C, 9, LOGE, 512, 6>
-                : M == 1024 ? k_fft_x_fwd_ct<C, 10, LOGE, 512, 6>
-                            : k_fft_x_fwd_ct<C, 11, LOGE, 512, 6>;
+                            int64_t, int, int64_t, int);
+        const auto pick = [M](auto ms_tag) -> KC {
+            constexpr bool HM = decltype(ms_tag)::value;
+            return M == 256 ? k_fft_x_fwd_ct<C, 8, LOGE, 512, 6, HM>
+                   : M == 512 ? k_fft_x_fwd_ct<C, 9, LOGE, 512, 6, HM>
+                   : M == 1024 ? k_fft_x_fwd_ct<C, 10, LOGE, 512, 6, HM>
+                               : k_fft_x_fwd_ct<C, 11, LOGE, 512, 6, HM>;
+        };
+        const KC fc = ms ? pick(std::true_type{}) : pick(std::false_type{});
         const int W = std::max(1, (1 << LOGE) / M);
         const int64_t groups = 3 * Zr * (Yr / W);
         require(groups < (int64_t{1} << 31) && kTile * ctx->n1 * (M / kTile + 1) < (int64_t{1} << 31),
@@ -841,10 +876,11 @@ void fft_x_forward(mafx_ctx* ctx, const double* in, const double* ms, double sca
         ProfScope ps(ctx, nm("fft_x_fwd_f32", "fft_x_fwd_f64", sizeof(C)), bytes);
         const size_t lds = (lds_padded(1 << LOGE) + M) * sizeof(C);
         allow_lds(reinterpret_cast<const void*>(fc), lds);
-        hipLaunchKernelGGL(fc, dim3(static_cast<unsigned>(groups)), dim3(512), lds, ctx->stream, in, ms, scale,
-                           static_cast<int>(Yr), make_fastdiv(static_cast<uint32_t>(Yr / W)),
-                           make_fastdiv(static_cast<uint32_t>(Zr)), tw_of<C>(ph), tw_of<C>(ctx->px), out,
-                           ctx->n1 * ctx->hxs, n2o, static_cast<int>(kTile * ctx->n1), Xi * Yi * Zi);
+        const unsigned grid = static_cast<unsigned>(std::min<int64_t>(groups, int64_t{kXGroupsPerCU} * ctx->num_cus));
+        hipLaunchKernelGGL(fc, dim3(grid), dim3(512), lds, ctx->stream, in, ms, scale, static_cast<int>(Yr),
+                           make_fastdiv(static_cast<uint32_t>(Yr / W)), make_fastdiv(static_cast<uint32_t>(Zr)),
+                           tw_of<C>(ph), tw_of<C>(ctx->px), out, ctx->n1 * ctx->hxs, n2o,
+                           static_cast<int>(kTile * ctx->n1), Xi * Yi * Zi, static_cast<int>(groups));
         MAFX_HIP(hipGetLastError());
         return;
     }

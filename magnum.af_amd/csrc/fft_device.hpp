@@ -210,12 +210,11 @@ constexpr int ct_logr(int logm, int s) { return s < logm / 3 ? 3 : logm % 3; }
 constexpr int ct_logns(int logm, int s) { return 3 * (s < logm / 3 ? s : logm / 3); }
 
 template <class C, int LOGM, int LOGW2, int LOGR, int LOGNS, int BS, bool INV>
-__device__ __forceinline__ void ct_stage(C* buf, const C* __restrict__ tw) {
+__device__ __forceinline__ void ct_stage(C* buf, const C* __restrict__ tw, const int tid) {
     constexpr int R = 1 << LOGR, W2 = 1 << LOGW2, NS = 1 << LOGNS;
     constexpr int NB = (1 << (LOGM - LOGR)) << LOGW2; // butterflies; also the stride between inputs
     constexpr int NBT = (NB + BS - 1) / BS;
     constexpr int TWS = LOGM - LOGNS - LOGR; // log2 of the twiddle step M / (Ns R)
-    const int tid = static_cast<int>(threadIdx.x);
     C v[NBT][R];
 #pragma unroll
     for (int i = 0; i < NBT; ++i) {
@@ -244,13 +243,21 @@ __device__ __forceinline__ void ct_stage(C* buf, const C* __restrict__ tw) {
     __syncthreads();
 }
 
-// All stages of W2 lines of length M held in buf[pos * W2 + line]; tw = w_M^t, t < M.
+// All stages of W2 lines of length M held in buf[pos * W2 + line]; tw = w_M^t, t < M.  tid is
+// the lane's index in the group (threadIdx.x, or an opaque copy of it in persistent loops).
 template <class C, int LOGM, int LOGW2, int BS, bool INV, int S = 0, int SEND = ct_nstages(LOGM)>
-__device__ __forceinline__ void ct_fft(C* buf, const C* __restrict__ tw) {
+__device__ __forceinline__ void ct_fft(C* buf, const C* __restrict__ tw, const int tid = static_cast<int>(threadIdx.x)) {
     if constexpr (S < SEND) {
-        ct_stage<C, LOGM, LOGW2, ct_logr(LOGM, S), ct_logns(LOGM, S), BS, INV>(buf, tw);
-        ct_fft<C, LOGM, LOGW2, BS, INV, S + 1, SEND>(buf, tw);
+        ct_stage<C, LOGM, LOGW2, ct_logr(LOGM, S), ct_logns(LOGM, S), BS, INV>(buf, tw, tid);
+        ct_fft<C, LOGM, LOGW2, BS, INV, S + 1, SEND>(buf, tw, tid);
     }
+}
+
+// v unchanged, but unknown to the optimiser: address arithmetic derived from it stays inside a
+// persistent loop instead of being hoisted out and held in registers across iterations.
+__device__ __forceinline__ int opaque_int(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
 }
 
 // Pieces of stage S for kernels that keep its inputs or outputs in registers.  Butterfly b of

@@ -120,6 +120,7 @@ struct mafx_ctx {
     bool has_ms_field{false};
     mafx::DevBuf ms_field; // N f64
     int precision{MAFX_F64};
+    int num_cus{256};         // compute units of the current device (persistent grids)
     std::vector<mafx::TermDev> terms;
     int demag_term{-1};       // index of the (micro or atomistic) FFT term, -1 if none
     bool atom_dipole{false};
