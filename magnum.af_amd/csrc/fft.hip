@@ -233,6 +233,8 @@ __global__ void __launch_bounds__(kAxisBlock, POW2 ? MINW : 1)
 
 // Resident 512-lane groups per CU of the persistent x forward transform (LDS: three 41 KiB tiles)
 constexpr int kXGroupsPerCU = 3;
+// Resident 1024-lane FP32 groups per CU of the persistent y forward transform (LDS: two 72 KiB tiles)
+constexpr int kYGroupsPerCU = 2;
 
 // Compile-time shaped forms of k_fft_x_fwd / k_fft_x_inv for the demag field: power-of-two
 // M = nx = n0 / 2, rows of exactly M reals, W = 2^(LOGE - LOGM) rows per work-group taken from
@@ -513,7 +515,7 @@ __global__ void __launch_bounds__(BS, POW2 ? MINW : 1)
 template <class C, int LOGM, int LOGE, int BS, int MINW>
 __global__ void __launch_bounds__(BS, MINW)
     k_fft_y_fwd_ct(C* __restrict__ data, int64_t sa, int64_t sb, int hx, const C* __restrict__ twh,
-                   const C* __restrict__ twL) {
+                   const C* __restrict__ twL, FastDiv gxd, FastDiv gyd, int ngroups) {
     constexpr int M = 1 << LOGM, LOGW2 = LOGE - LOGM, W2 = 1 << LOGW2, LOGW = LOGW2 - 1, W = W2 / 2;
     constexpr int n1T = 8 * M; // tile stride; offsets inside a work-group's tiles stay below 2^LOGE
     constexpr int NST = ct_nstages(LOGM);
@@ -524,43 +526,63 @@ __global__ void __launch_bounds__(BS, MINW)
     C* buf = reinterpret_cast<C*>(smem_raw);
     C* tws = buf + lds_padded(1 << LOGE);
     for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
-    const uint3 blk = swizzled_block();
-    const int kx0 = static_cast<int>(blk.x) * W;
-    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + (kx0 / kTile) * n1T;
     const int tid = static_cast<int>(threadIdx.x);
+    // group g = (plane b * gy + a) * gx + column tile; persistent groups take g, g + gridDim.x, ...
+    // and load the next group's columns into registers while the current one is transformed
+    const auto base_of = [&](int grp, int& kx0) -> C* {
+        const uint32_t q = fdiv(static_cast<uint32_t>(grp), gxd), bx = static_cast<uint32_t>(grp) - q * gxd.d;
+        const uint32_t bz = fdiv(q, gyd), by = q - bz * gyd.d;
+        kx0 = static_cast<int>(bx) * W;
+        return data + static_cast<int64_t>(bz) * sa + static_cast<int64_t>(by) * sb + (kx0 / kTile) * n1T;
+    };
     // inputs y < M of the W columns, contiguous: g = (tile * M + y) * 4 + w % 4; both lines of
     // column w are filled (x and x w_L^y) so the even and odd outputs share one pass
     constexpr int IT = M * W / BS;
     C v[IT], t[IT];
+    const auto fetch = [&](int grp) {
+        int kx0;
+        const C* base = base_of(grp, kx0);
+        const int ft = opaque_int(tid);
 #pragma unroll
-    for (int i = 0; i < IT; ++i) {
-        const int g = tid + i * BS;
-        const int tl = g >> (LOGM + 2), y = (g >> 2) & (M - 1), w = (tl << 2) | (g & 3);
-        v[i] = kx0 + w < hx ? base[tl * n1T + (g & (4 * M - 1))] : C{0, 0};
-        t[i] = twL[y];
-    }
-#pragma unroll
-    for (int i = 0; i < IT; ++i) {
-        const int g = tid + i * BS;
-        const int tl = g >> (LOGM + 2), y = (g >> 2) & (M - 1), w = (tl << 2) | (g & 3);
-        buf[lds_pad((y << LOGW2) + w)] = v[i];
-        buf[lds_pad((y << LOGW2) + W + w)] = cmul(v[i], t[i]);
-    }
-    __syncthreads();
-    ct_fft<C, LOGM, LOGW2, BS, false, 0, NST - 1>(buf, tws);
-    constexpr int NBL = SL::NB / BS;
-#pragma unroll
-    for (int i = 0; i < NBL; ++i) {
-        const int b = tid + i * BS, j = b >> LOGW2, l = b & (W2 - 1), w = l & (W - 1), odd = l >> LOGW;
-        C u[SL::R];
-#pragma unroll
-        for (int r = 0; r < SL::R; ++r) u[r] = buf[lds_pad(b + r * SL::NB)];
-        SL::template bfly<C, false>(u, b, tws);
-        if (kx0 + w < hx) {
-            C* dst = base + (w >> 2) * n1T + 4 * odd + (w & 3);
-#pragma unroll
-            for (int r = 0; r < SL::R; ++r) dst[8 * (j + r * SL::NS)] = u[r];
+        for (int i = 0; i < IT; ++i) {
+            const int g = ft + i * BS;
+            const int tl = g >> (LOGM + 2), y = (g >> 2) & (M - 1), w = (tl << 2) | (g & 3);
+            v[i] = kx0 + w < hx ? base[tl * n1T + (g & (4 * M - 1))] : C{0, 0};
+            t[i] = twL[y];
         }
+    };
+    int grp = static_cast<int>(blockIdx.x);
+    fetch(grp);
+    for (; grp < ngroups; grp += static_cast<int>(gridDim.x)) {
+        const int lt = opaque_int(tid);
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int g = lt + i * BS;
+            const int tl = g >> (LOGM + 2), y = (g >> 2) & (M - 1), w = (tl << 2) | (g & 3);
+            buf[lds_pad((y << LOGW2) + w)] = v[i];
+            buf[lds_pad((y << LOGW2) + W + w)] = cmul(v[i], t[i]);
+        }
+        __syncthreads();
+        const int next = grp + static_cast<int>(gridDim.x);
+        if (next < ngroups) fetch(next);
+        ct_fft<C, LOGM, LOGW2, BS, false, 0, NST - 1>(buf, tws, lt);
+        int kx0;
+        C* base = base_of(grp, kx0);
+        constexpr int NBL = SL::NB / BS;
+#pragma unroll
+        for (int i = 0; i < NBL; ++i) {
+            const int b = lt + i * BS, j = b >> LOGW2, l = b & (W2 - 1), w = l & (W - 1), odd = l >> LOGW;
+            C u[SL::R];
+#pragma unroll
+            for (int r = 0; r < SL::R; ++r) u[r] = buf[lds_pad(b + r * SL::NB)];
+            SL::template bfly<C, false>(u, b, tws);
+            if (kx0 + w < hx) {
+                C* dst = base + (w >> 2) * n1T + 4 * odd + (w & 3);
+#pragma unroll
+                for (int r = 0; r < SL::R; ++r) dst[8 * (j + r * SL::NS)] = u[r];
+            }
+        }
+        __syncthreads(); // the next group's columns overwrite buf
     }
 }
 
@@ -575,7 +597,7 @@ template <class C> __device__ __forceinline__ C shfl_xor_c(C v, int mask) {
 template <class C, int LOGM, int LOGE, int BS, int MINW>
 __global__ void __launch_bounds__(BS, MINW)
     k_fft_y_inv_ct(C* __restrict__ data, int64_t sa, int64_t sb, int hx, const C* __restrict__ twh,
-                   const C* __restrict__ twL) {
+                   const C* __restrict__ twL, FastDiv gxd, FastDiv gyd, int ngroups) {
     constexpr int M = 1 << LOGM, LOGW2 = LOGE - LOGM, W2 = 1 << LOGW2, LOGW = LOGW2 - 1, W = W2 / 2;
     constexpr int n1T = 8 * M;
     constexpr int NST = ct_nstages(LOGM);
@@ -587,47 +609,67 @@ __global__ void __launch_bounds__(BS, MINW)
     C* buf = reinterpret_cast<C*>(smem_raw);
     C* tws = buf + lds_padded(1 << LOGE);
     for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
-    const uint3 blk = swizzled_block();
-    const int kx0 = static_cast<int>(blk.x) * W;
-    C* base = data + static_cast<int64_t>(blk.z) * sa + static_cast<int64_t>(blk.y) * sb + (kx0 / kTile) * n1T;
     const int tid = static_cast<int>(threadIdx.x);
+    // groups as in k_fft_y_fwd_ct; the next group's first-stage inputs are loaded into registers
+    // while the current one is transformed
+    const auto base_of = [&](int grp, int& kx0) -> C* {
+        const uint32_t q = fdiv(static_cast<uint32_t>(grp), gxd), bx = static_cast<uint32_t>(grp) - q * gxd.d;
+        const uint32_t bz = fdiv(q, gyd), by = q - bz * gyd.d;
+        kx0 = static_cast<int>(bx) * W;
+        return data + static_cast<int64_t>(bz) * sa + static_cast<int64_t>(by) * sb + (kx0 / kTile) * n1T;
+    };
     constexpr int NB0 = S0::NB / BS;
     C v0[NB0][S0::R];
+    const auto fetch = [&](int grp) {
+        int kx0;
+        const C* base = base_of(grp, kx0);
+        const int ft = opaque_int(tid);
 #pragma unroll
-    for (int i = 0; i < NB0; ++i) {
-        const int b = tid + i * BS, j = b >> LOGW2, l = b & (W2 - 1), w = l & (W - 1), odd = l >> LOGW;
-        const C* src = base + (w >> 2) * n1T + 4 * odd + (w & 3);
-        const bool ok = kx0 + w < hx;
+        for (int i = 0; i < NB0; ++i) {
+            const int b = ft + i * BS, j = b >> LOGW2, l = b & (W2 - 1), w = l & (W - 1), odd = l >> LOGW;
+            const C* src = base + (w >> 2) * n1T + 4 * odd + (w & 3);
+            const bool ok = kx0 + w < hx;
 #pragma unroll
-        for (int r = 0; r < S0::R; ++r) v0[i][r] = ok ? src[8 * (j + r * (M / S0::R))] : C{0, 0};
-    }
-#pragma unroll
-    for (int i = 0; i < NB0; ++i) {
-        const int b = tid + i * BS;
-        S0::template bfly<C, true>(v0[i], b, tws);
-#pragma unroll
-        for (int r = 0; r < S0::R; ++r) buf[lds_pad((S0::opos(b, r) << LOGW2) + (b & (W2 - 1)))] = v0[i][r];
-    }
-    __syncthreads();
-    ct_fft<C, LOGM, LOGW2, BS, true, 1, NST - 1>(buf, tws);
-    constexpr int NBL = SL::NB / BS, H = SL::R / 2;
-#pragma unroll
-    for (int i = 0; i < NBL; ++i) {
-        const int b = tid + i * BS, j = b >> LOGW2, l = b & (W2 - 1), w = l & (W - 1), odd = l >> LOGW;
-        C v[SL::R];
-#pragma unroll
-        for (int r = 0; r < SL::R; ++r) v[r] = buf[lds_pad(b + r * SL::NB)];
-        SL::template bfly<C, true>(v, b, tws);
-        const bool ok = kx0 + w < hx;
-        C* dst = base + (w >> 2) * n1T + (w & 3);
-#pragma unroll
-        for (int q = 0; q < H; ++q) {
-            // the even lane keeps positions q, the odd lane q + H
-            const C recv = shfl_xor_c(odd ? v[q] : v[q + H], W);
-            const int n = j + (odd ? q + H : q) * SL::NS;
-            const C e = odd ? recv : v[q], o = odd ? v[q + H] : recv;
-            if (ok) dst[4 * n] = cadd(e, cmul(conj_if(twL[n], true), o));
+            for (int r = 0; r < S0::R; ++r) v0[i][r] = ok ? src[8 * (j + r * (M / S0::R))] : C{0, 0};
         }
+    };
+    int grp = static_cast<int>(blockIdx.x);
+    fetch(grp);
+    for (; grp < ngroups; grp += static_cast<int>(gridDim.x)) {
+        const int lt = opaque_int(tid);
+#pragma unroll
+        for (int i = 0; i < NB0; ++i) {
+            const int b = lt + i * BS;
+            S0::template bfly<C, true>(v0[i], b, tws);
+#pragma unroll
+            for (int r = 0; r < S0::R; ++r) buf[lds_pad((S0::opos(b, r) << LOGW2) + (b & (W2 - 1)))] = v0[i][r];
+        }
+        __syncthreads();
+        const int next = grp + static_cast<int>(gridDim.x);
+        if (next < ngroups) fetch(next);
+        ct_fft<C, LOGM, LOGW2, BS, true, 1, NST - 1>(buf, tws, lt);
+        int kx0;
+        C* base = base_of(grp, kx0);
+        constexpr int NBL = SL::NB / BS, H = SL::R / 2;
+#pragma unroll
+        for (int i = 0; i < NBL; ++i) {
+            const int b = lt + i * BS, j = b >> LOGW2, l = b & (W2 - 1), w = l & (W - 1), odd = l >> LOGW;
+            C v[SL::R];
+#pragma unroll
+            for (int r = 0; r < SL::R; ++r) v[r] = buf[lds_pad(b + r * SL::NB)];
+            SL::template bfly<C, true>(v, b, tws);
+            const bool ok = kx0 + w < hx;
+            C* dst = base + (w >> 2) * n1T + (w & 3);
+#pragma unroll
+            for (int q = 0; q < H; ++q) {
+                // the even lane keeps positions q, the odd lane q + H
+                const C recv = shfl_xor_c(odd ? v[q] : v[q + H], W);
+                const int n = j + (odd ? q + H : q) * SL::NS;
+                const C e = odd ? recv : v[q], o = odd ? v[q + H] : recv;
+                if (ok) dst[4 * n] = cadd(e, cmul(conj_if(twL[n], true), o));
+            }
+        }
+        __syncthreads(); // the next group's first stage overwrites buf
     }
 }
 
@@ -991,7 +1033,7 @@ void fft_y_split(mafx_ctx* ctx, C* data, int64_t sa, int na, int64_t sb, int nb,
     // (1024 lanes; FP32 under an 8-wave register cap, two groups per CU; FP64 one group per CU)
     if (stride == kTile && ctx->n1 == 2 * M && (M == 256 || M == 512 || M == 1024)) {
         constexpr int LOGE = 13, MW = kF32 ? 8 : 4;
-        using KC = void (*)(C*, int64_t, int64_t, int, const C*, const C*);
+        using KC = void (*)(C*, int64_t, int64_t, int, const C*, const C*, FastDiv, FastDiv, int);
         KC fc = nullptr;
         if (M == 1024) fc = inv ? k_fft_y_inv_ct<C, 10, LOGE, 1024, MW> : k_fft_y_fwd_ct<C, 10, LOGE, 1024, MW>;
         if (M == 512) fc = inv ? k_fft_y_inv_ct<C, 9, LOGE, 1024, MW> : k_fft_y_fwd_ct<C, 9, LOGE, 1024, MW>;
@@ -1005,8 +1047,15 @@ void fft_y_split(mafx_ctx* ctx, C* data, int64_t sa, int na, int64_t sb, int nb,
                      bytes);
         const size_t lds = (lds_padded(1 << LOGE) + M) * sizeof(C);
         allow_lds(reinterpret_cast<const void*>(fc), lds);
-        hipLaunchKernelGGL(fc, dim3(gx, nb, na), dim3(1024), lds, ctx->stream, data, sa, sb, static_cast<int>(ctx->hx),
-                           tw_of<C>(ph), tw_of<C>(ctx->py));
+        // The FP32 forward runs two persistent groups per CU (72 KiB tiles); measured at 1024 x
+        // 1024 x 4, the inverse and the FP64 forms are faster with one column tile per group
+        const int64_t groups = static_cast<int64_t>(gx) * nb * na;
+        require(groups < (int64_t{1} << 31), "y transform: too many column tiles");
+        const unsigned grid = static_cast<unsigned>(
+            kF32 && !inv ? std::min<int64_t>(groups, int64_t{kYGroupsPerCU} * ctx->num_cus) : groups);
+        hipLaunchKernelGGL(fc, dim3(grid), dim3(1024), lds, ctx->stream, data, sa, sb, static_cast<int>(ctx->hx),
+                           tw_of<C>(ph), tw_of<C>(ctx->py), make_fastdiv(gx), make_fastdiv(static_cast<uint32_t>(nb)),
+                           static_cast<int>(groups));
         MAFX_HIP(hipGetLastError());
         return;
     }
