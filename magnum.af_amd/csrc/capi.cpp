@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -176,6 +177,12 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
         MAFX_HIP(hipGetDevice(&dev));
         MAFX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         if (cus > 0) ctx->num_cus = cus;
+        // MAFX_NUM_CUS overrides the CU count that sizes the persistent grids (tuning, and tests
+        // that make each resident group take many tiles on small meshes)
+        if (const char* e = std::getenv("MAFX_NUM_CUS")) {
+            const int v = std::atoi(e);
+            if (v > 0) ctx->num_cus = v;
+        }
         ctx->mesh = *mesh;
         ctx->N = mesh->nx * mesh->ny * mesh->nz;
         ctx->Ms = Ms;

@@ -249,6 +249,32 @@ def test_heff_is_deterministic(precision):
     assert e1[0] == e2[0] and np.array_equal(e1[1], e2[1])
 
 
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("mesh,ms_field", [((256, 256, 2, 2e-9, 2e-9, 2e-9), False),
+                                           ((1024, 64, 2, 2e-9, 1e-9, 2e-9), False),
+                                           ((512, 512, 1, 1e-9, 1e-9, 2e-9), True)])
+def test_persistent_passes_bitwise(mesh, ms_field, precision, monkeypatch):
+    """The x forward pass (and the FP32 y forward pass) run one resident set of groups sized by
+    the CU count, each taking tiles g, g + grid, ... and prefetching the next.  With the CU count
+    forced to 1 every group takes many tiles; the field must equal the default launch (one tile
+    per group on these meshes) to the last bit, also with an Ms field (the x forward's scaled
+    input variant)."""
+    M = mf()
+    print(f"\n[persistent passes] mesh={mesh} ms_field={ms_field} {precision}", flush=True)
+    g = M.Mesh(*mesh)
+    m = rand_m(mesh[:3], 7)
+    if ms_field:
+        m[: mesh[0] // 4] = 0  # an air region: State derives an Ms field from |m| = 0 cells
+    terms = lambda: [M.DemagField(g), M.ExchangeField(1.5e-11), M.DmiField(3e-3)]
+    state = M.State(g, 1.1e6, m)
+    assert (state.Ms_field is not None) == ms_field
+    h_default = state._context(terms(), precision).heff(state.m_dev).cpu().numpy().copy()
+    monkeypatch.setenv("MAFX_NUM_CUS", "1")
+    state1 = M.State(g, 1.1e6, m)
+    h_one = state1._context(terms(), precision).heff(state1.m_dev).cpu().numpy().copy()
+    assert np.array_equal(h_default.view(np.uint64), h_one.view(np.uint64))
+
+
 def test_demag_cube_energy_gpu():
     # magnumaf/test/micro_demag.cpp:6-29
     x = y = z = 1e-9
