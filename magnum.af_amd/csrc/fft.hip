@@ -268,6 +268,26 @@ __global__ void __launch_bounds__(BS, MINW)
         y0 = static_cast<int>(yg) << LOGW;
         return (static_cast<int64_t>(z) * ny + y0) << LOGM; // cell of (x = 0, y0, z)
     };
+    // RS: the first stage (radix 8, one butterfly per lane per NB0 round) runs from registers.
+    // Positions >= M / 2 are the zero padding, so butterfly (j, w) has the non-zero inputs
+    // j + r M / 8, r < 4, which are this lane's 4 pair loads; lanes take j fastest so a wave
+    // reads 64 consecutive pairs of one row per input.
+    using S0 = CtStage<LOGM, LOGW, 0>;
+    constexpr bool RS = S0::R == 8 && S0::NB % BS == 0 && S0::NB / BS * 4 == IT;
+    constexpr int NB0 = RS ? S0::NB / BS : 1, M8 = M / 8;
+    if constexpr (RS) __syncthreads(); // tws is complete before any butterfly runs
+    const auto slot = [&](int t, int i, int& w, int& o) { // load i of lane t: row w, offset o
+        if constexpr (RS) {
+            const int bt = t + (i >> 2) * BS, r = i & 3;
+            const int j = bt & (M8 - 1);
+            w = bt >> (LOGM - 3);
+            o = (w << LOGM) + 2 * (j + r * M8);
+        } else {
+            const int g = t + i * BS, n = g & (MH - 1);
+            w = g >> (LOGM - 1);
+            o = (w << LOGM) + 2 * n;
+        }
+    };
     const auto fetch = [&](int grp) {
         uint32_t c, z;
         int y0;
@@ -276,8 +296,8 @@ __global__ void __launch_bounds__(BS, MINW)
         const int ft = opaque_int(tid);
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
-            const int g = ft + i * BS, w = g >> (LOGM - 1), n = g & (MH - 1);
-            const int o = (w << LOGM) + 2 * n;
+            int w, o;
+            slot(ft, i, w, o);
             p[i] = *reinterpret_cast<const double2*>(src + o);
             if constexpr (MS) q[i] = *reinterpret_cast<const double2*>(ms + cell0 + o);
         }
@@ -286,24 +306,40 @@ __global__ void __launch_bounds__(BS, MINW)
     fetch(grp);
     for (; grp < ngroups; grp += static_cast<int>(gridDim.x)) {
         const int lt = opaque_int(tid);
+        const auto val = [&](int i) {
+            if constexpr (MS)
+                return C{static_cast<T>(p[i].x * scale * q[i].x), static_cast<T>(p[i].y * scale * q[i].y)};
+            else
+                return C{static_cast<T>(p[i].x * scale), static_cast<T>(p[i].y * scale)};
+        };
+        if constexpr (RS) {
 #pragma unroll
-        for (int i = 0; i < IT; ++i) {
-            const int g = lt + i * BS, w = g >> (LOGM - 1), n = g & (MH - 1);
-            T re, im;
-            if constexpr (MS) {
-                re = static_cast<T>(p[i].x * scale * q[i].x);
-                im = static_cast<T>(p[i].y * scale * q[i].y);
-            } else {
-                re = static_cast<T>(p[i].x * scale);
-                im = static_cast<T>(p[i].y * scale);
+            for (int bi = 0; bi < NB0; ++bi) {
+                const int bt = lt + bi * BS, j = bt & (M8 - 1), w = bt >> (LOGM - 3);
+                const int b = (j << LOGW) + w; // the stage's butterfly index of (j, w)
+                C v[8];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[r] = val(bi * 4 + r);
+#pragma unroll
+                for (int r = 4; r < 8; ++r) v[r] = C{0, 0};
+                S0::template bfly<C, false>(v, b, tws);
+#pragma unroll
+                for (int r = 0; r < 8; ++r) buf[lds_pad((S0::opos(b, r) << LOGW) + w)] = v[r];
             }
-            buf[lds_pad((n << LOGW) + w)] = C{re, im};
-            buf[lds_pad(((n + MH) << LOGW) + w)] = C{0, 0};
+        } else {
+#pragma unroll
+            for (int i = 0; i < IT; ++i) {
+                int w, o;
+                slot(lt, i, w, o);
+                const int n = (o - (w << LOGM)) >> 1;
+                buf[lds_pad((n << LOGW) + w)] = val(i);
+                buf[lds_pad(((n + MH) << LOGW) + w)] = C{0, 0};
+            }
         }
         __syncthreads();
         const int next = grp + static_cast<int>(gridDim.x);
         if (next < ngroups) fetch(next);
-        ct_fft<C, LOGM, LOGW, BS, false>(buf, tws, lt);
+        ct_fft<C, LOGM, LOGW, BS, false, RS ? 1 : 0>(buf, tws, lt);
         // X[k] = E + w_L^k O, E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2, k <= M
         uint32_t c, z;
         int y0;
