@@ -397,17 +397,43 @@ __global__ void __launch_bounds__(BS, MINW)
         buf[lds_pad((k << LOGW) + w)] = c2r_combine(a[i], b[i], twL[k]);
     }
     __syncthreads();
-    ct_fft<C, LOGM, LOGW, BS, true>(buf, tws);
-    // x[2n] + i x[2n+1] = Z[n], n < M / 2 (x < nx = M), scaled by 1 / P
+    constexpr int NST = ct_nstages(LOGM);
+    using SL = CtStage<LOGM, LOGW, NST - 1>;
     T* orow = out + static_cast<int64_t>(c) * N + ((static_cast<int64_t>(z) * ny + y0) << LOGM);
     using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
-    constexpr int OT = W * MH / BS;
+    if constexpr (SL::NB % BS != 0) { // fewer last-stage butterflies than lanes: all stages in LDS
+        ct_fft<C, LOGM, LOGW, BS, true>(buf, tws);
+        constexpr int OT = W * MH / BS;
 #pragma unroll
-    for (int i = 0; i < OT; ++i) {
-        const int g = tid + i * BS, w = g >> (LOGM - 1), n = g & (MH - 1);
-        const C v = buf[lds_pad((n << LOGW) + w)];
-        *reinterpret_cast<T2*>(orow + (w << LOGM) + 2 * n) =
-            T2{static_cast<T>(static_cast<double>(v.x) * hscale), static_cast<T>(static_cast<double>(v.y) * hscale)};
+        for (int i = 0; i < OT; ++i) {
+            const int g = tid + i * BS, w = g >> (LOGM - 1), n = g & (MH - 1);
+            const C v = buf[lds_pad((n << LOGW) + w)];
+            *reinterpret_cast<T2*>(orow + (w << LOGM) + 2 * n) = T2{
+                static_cast<T>(static_cast<double>(v.x) * hscale), static_cast<T>(static_cast<double>(v.y) * hscale)};
+        }
+        return;
+    }
+    ct_fft<C, LOGM, LOGW, BS, true, 0, NST - 1>(buf, tws);
+    // x[2n] + i x[2n+1] = Z[n], n < M / 2 (x < nx = M), scaled by 1 / P.  The last stage runs from
+    // registers: butterfly (j, w) has outputs j + r M / R, and only r < R / 2 (positions below
+    // M / 2) survive the crop, so those go straight to memory; lanes take j fastest so a wave
+    // writes consecutive pairs of one row.
+    static_assert(SL::NS == (M >> SL::LOGR), "last stage shape");
+    constexpr int NBL = SL::NB / BS, LOGJ = LOGM - SL::LOGR;
+#pragma unroll
+    for (int i = 0; i < NBL; ++i) {
+        const int bt = tid + i * BS, j = bt & ((1 << LOGJ) - 1), w = bt >> LOGJ;
+        const int b = (j << LOGW) + w;
+        C v[SL::R];
+#pragma unroll
+        for (int r = 0; r < SL::R; ++r) v[r] = buf[lds_pad(b + r * SL::NB)];
+        SL::template bfly<C, true>(v, b, tws);
+#pragma unroll
+        for (int r = 0; r < SL::R / 2; ++r) {
+            const int n = j + r * SL::NS;
+            *reinterpret_cast<T2*>(orow + (w << LOGM) + 2 * n) = T2{
+                static_cast<T>(static_cast<double>(v[r].x) * hscale), static_cast<T>(static_cast<double>(v[r].y) * hscale)};
+        }
     }
 }
 
