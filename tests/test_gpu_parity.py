@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from golden_values import known
 
 pytestmark = pytest.mark.gpu
 
@@ -602,9 +603,10 @@ def test_sp4_relaxation_and_switch_gpu():
         iz += mz * dt
     print(f"\n[sp4 relax] dints = {ix - 9.81206172824e-10:.3e} {iy - 9.14350283169e-11:.3e} "
           f"{iz + 5.74381785359e-13:.3e}", flush=True)
-    assert abs(ix - 9.81206172824e-10) < 1e-15
-    assert abs(iy - 9.14350283169e-11) < 1e-15
-    assert abs(iz + 5.74381785359e-13) < 1e-15
+    g = known("sp4_relax")
+    assert abs(ix - g["int_mx_dt"]) < g["abs_tol"]
+    assert abs(iy - g["int_my_dt"]) < g["abs_tol"]
+    assert abs(iz - g["int_mz_dt"]) < g["abs_tol"]
     llg.alpha = 0.02
     z = np.zeros((nx, ny, nz, 3))
     z[..., 0] = -24.6e-3 / MU0
@@ -621,9 +623,10 @@ def test_sp4_relaxation_and_switch_gpu():
         iz += mz * dt
     print(f"[sp4 switch] dints = {ix + 6.41261165705e-10:.3e} {iy - 1.47353233738e-10:.3e} "
           f"{iz + 1.78144535231e-11:.3e}", flush=True)
-    assert abs(ix + 6.41261165705e-10) < 1e-15
-    assert abs(iy - 1.47353233738e-10) < 1e-15
-    assert abs(iz + 1.78144535231e-11) < 1e-15
+    g = known("sp4_switch")
+    assert abs(ix - g["int_mx_dt"]) < g["abs_tol"]
+    assert abs(iy - g["int_my_dt"]) < g["abs_tol"]
+    assert abs(iz - g["int_mz_dt"]) < g["abs_tol"]
 
 
 def test_integrate_dense_matches_oracle():

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from golden_values import known
 
 MU0 = O.MU0
 
@@ -341,7 +342,8 @@ def test_rkky_mumax3_energy_sweep():
         mm[:, :, 1, 1] = math.sin(i * math.pi / 180.0)
         s.set_m(mm, normalize=False)
         Es.append(s.energy()[0])
-    assert abs((max(Es) - min(Es)) - 2.13934e-19) < 5e-26
+    g = known("rkky_energy_sweep")
+    assert abs((max(Es) - min(Es)) - g["max_minus_min_J"]) < g["abs_tol"]
 
 
 # ---------------------------------------------------------------- RK tableaux (magnumaf/test/adaptive_runge_kutta.cpp:9-62)
@@ -418,9 +420,10 @@ def test_sp4_relaxation_and_switch():
         ix += mm[0] * dt
         iy += mm[1] * dt
         iz += mm[2] * dt
-    assert abs(ix - 9.81206172824e-10) < 1e-15
-    assert abs(iy - 9.14350283169e-11) < 1e-15
-    assert abs(iz + 5.74381785359e-13) < 1e-15
+    g = known("sp4_relax")
+    assert abs(ix - g["int_mx_dt"]) < g["abs_tol"]
+    assert abs(iy - g["int_my_dt"]) < g["abs_tol"]
+    assert abs(iz - g["int_mz_dt"]) < g["abs_tol"]
     s.set_alpha(0.02)
     z = np.zeros((nx, ny, nz, 3))
     z[..., 0] = -24.6e-3 / MU0
@@ -435,9 +438,10 @@ def test_sp4_relaxation_and_switch():
         ix += mm[0] * dt
         iy += mm[1] * dt
         iz += mm[2] * dt
-    assert abs(ix + 6.41261165705e-10) < 1e-15
-    assert abs(iy - 1.47353233738e-10) < 1e-15
-    assert abs(iz + 1.78144535231e-11) < 1e-15
+    g = known("sp4_switch")
+    assert abs(ix - g["int_mx_dt"]) < g["abs_tol"]
+    assert abs(iy - g["int_my_dt"]) < g["abs_tol"]
+    assert abs(iz - g["int_mz_dt"]) < g["abs_tol"]
 
 
 # ---------------------------------------------------------------- state semantics (magnumaf/test/state.cpp:38-135)
