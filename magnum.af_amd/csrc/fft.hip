@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(BS, MINW)
     extern __shared__ unsigned char smem_raw[];
     C* buf = reinterpret_cast<C*>(smem_raw);
     C* tws = buf + lds_padded(E);
-    for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
+    for (int t = threadIdx.x; t < M / 2; t += BS) tws[t] = twh[t]; // half table: w^(t + M/2) = -w^t
     const int tid = static_cast<int>(threadIdx.x);
     constexpr int IT = W * MH / BS;
     // Persistent groups: group g takes row groups g, g + gridDim.x, ... and loads the next one's
@@ -322,7 +322,7 @@ __global__ void __launch_bounds__(BS, MINW)
                 for (int r = 0; r < 4; ++r) v[r] = val(bi * 4 + r);
 #pragma unroll
                 for (int r = 4; r < 8; ++r) v[r] = C{0, 0};
-                S0::template bfly<C, false>(v, b, tws);
+                S0::template bfly<C, false, true>(v, b, tws);
 #pragma unroll
                 for (int r = 0; r < 8; ++r) buf[lds_pad((S0::opos(b, r) << LOGW) + w)] = v[r];
             }
@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(BS, MINW)
         __syncthreads();
         const int next = grp + static_cast<int>(gridDim.x);
         if (next < ngroups) fetch(next);
-        ct_fft<C, LOGM, LOGW, BS, false, RS ? 1 : 0>(buf, tws, lt);
+        ct_fft<C, LOGM, LOGW, BS, false, RS ? 1 : 0, ct_nstages(LOGM), true>(buf, tws, lt);
         // X[k] = E + w_L^k O, E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2, k <= M
         uint32_t c, z;
         int y0;
@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(BS, MINW)
     extern __shared__ unsigned char smem_raw[];
     C* buf = reinterpret_cast<C*>(smem_raw);
     C* tws = buf + lds_padded(E);
-    for (int t = threadIdx.x; t < M; t += BS) tws[t] = twh[t];
+    for (int t = threadIdx.x; t < M / 2; t += BS) tws[t] = twh[t]; // half table: w^(t + M/2) = -w^t
     const uint32_t grp = blockIdx.x, zc = fdiv(grp, gyd), yg = grp - zc * gyd.d;
     const uint32_t c = fdiv(zc, nzd), z = zc - c * nzd.d;
     const int y0 = static_cast<int>(yg) << LOGW;
@@ -402,7 +402,7 @@ __global__ void __launch_bounds__(BS, MINW)
     T* orow = out + static_cast<int64_t>(c) * N + ((static_cast<int64_t>(z) * ny + y0) << LOGM);
     using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
     if constexpr (SL::NB % BS != 0) { // fewer last-stage butterflies than lanes: all stages in LDS
-        ct_fft<C, LOGM, LOGW, BS, true>(buf, tws);
+        ct_fft<C, LOGM, LOGW, BS, true, 0, NST, true>(buf, tws);
         constexpr int OT = W * MH / BS;
 #pragma unroll
         for (int i = 0; i < OT; ++i) {
@@ -413,7 +413,7 @@ __global__ void __launch_bounds__(BS, MINW)
         }
         return;
     }
-    ct_fft<C, LOGM, LOGW, BS, true, 0, NST - 1>(buf, tws);
+    ct_fft<C, LOGM, LOGW, BS, true, 0, NST - 1, true>(buf, tws);
     // x[2n] + i x[2n+1] = Z[n], n < M / 2 (x < nx = M), scaled by 1 / P.  The last stage runs from
     // registers: butterfly (j, w) has outputs j + r M / R, and only r < R / 2 (positions below
     // M / 2) survive the crop, so those go straight to memory; lanes take j fastest so a wave
@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(BS, MINW)
         C v[SL::R];
 #pragma unroll
         for (int r = 0; r < SL::R; ++r) v[r] = buf[lds_pad(b + r * SL::NB)];
-        SL::template bfly<C, true>(v, b, tws);
+        SL::template bfly<C, true, true>(v, b, tws);
 #pragma unroll
         for (int r = 0; r < SL::R / 2; ++r) {
             const int n = j + r * SL::NS;
@@ -978,7 +978,7 @@ void fft_x_forward(mafx_ctx* ctx, const double* in, const double* ms, double sca
                 "x transform: too many rows");
         const double bytes = static_cast<double>(3 * Zr * Yr) * (Xi * 8.0 + ctx->hx * sizeof(C));
         ProfScope ps(ctx, nm("fft_x_fwd_f32", "fft_x_fwd_f64", sizeof(C)), bytes);
-        const size_t lds = (lds_padded(1 << LOGE) + M) * sizeof(C);
+        const size_t lds = (lds_padded(1 << LOGE) + M / 2) * sizeof(C);
         allow_lds(reinterpret_cast<const void*>(fc), lds);
         const unsigned grid = static_cast<unsigned>(std::min<int64_t>(groups, int64_t{kXGroupsPerCU} * ctx->num_cus));
         hipLaunchKernelGGL(fc, dim3(grid), dim3(512), lds, ctx->stream, in, ms, scale, static_cast<int>(Yr),
@@ -1023,17 +1023,19 @@ void fft_x_inverse(mafx_ctx* ctx, const C* spec, int64_t n2s, T* out, double hsc
         g.ny % std::max(1, (1 << LOGE) / M) == 0) {
         using KC = void (*)(const C*, int64_t, int64_t, int, int, FastDiv, FastDiv, const C*, const C*, T*, int64_t,
                             double);
-        KC fc = M == 256 ? k_fft_x_inv_ct<C, T, 8, LOGE, 512, 6>
-                : M == 512 ? k_fft_x_inv_ct<C, T, 9, LOGE, 512, 6>
-                : M == 1024 ? k_fft_x_inv_ct<C, T, 10, LOGE, 512, 6>
-                            : k_fft_x_inv_ct<C, T, 11, LOGE, 512, 6>;
+        // FP32 groups (38 KiB of LDS with the half twiddle table) fit four per CU under an 8-wave cap
+        constexpr int XW = sizeof(C) == sizeof(float2) ? 8 : 6;
+        KC fc = M == 256 ? k_fft_x_inv_ct<C, T, 8, LOGE, 512, XW>
+                : M == 512 ? k_fft_x_inv_ct<C, T, 9, LOGE, 512, XW>
+                : M == 1024 ? k_fft_x_inv_ct<C, T, 10, LOGE, 512, XW>
+                            : k_fft_x_inv_ct<C, T, 11, LOGE, 512, XW>;
         const int W = std::max(1, (1 << LOGE) / M);
         const int64_t groups = 3 * g.nz * (g.ny / W);
         require(groups < (int64_t{1} << 31) && kTile * ctx->n1 * (M / kTile + 1) < (int64_t{1} << 31),
                 "x transform: too many rows");
         const double bytes = static_cast<double>(3 * g.nz * g.ny) * (ctx->hx * sizeof(C) + g.nx * sizeof(T));
         ProfScope ps(ctx, nm("fft_x_inv_f32", "fft_x_inv_f64", sizeof(C)), bytes);
-        const size_t lds = (lds_padded(1 << LOGE) + M) * sizeof(C);
+        const size_t lds = (lds_padded(1 << LOGE) + M / 2) * sizeof(C);
         allow_lds(reinterpret_cast<const void*>(fc), lds);
         hipLaunchKernelGGL(fc, dim3(static_cast<unsigned>(groups)), dim3(512), lds, ctx->stream, spec, ctx->n1 * ctx->hxs,
                            n2s, static_cast<int>(kTile * ctx->n1), static_cast<int>(g.ny),

@@ -209,7 +209,17 @@ constexpr int ct_nstages(int logm) { return logm / 3 + (logm % 3 ? 1 : 0); }
 constexpr int ct_logr(int logm, int s) { return s < logm / 3 ? 3 : logm % 3; }
 constexpr int ct_logns(int logm, int s) { return 3 * (s < logm / 3 ? s : logm / 3); }
 
-template <class C, int LOGM, int LOGW2, int LOGR, int LOGNS, int BS, bool INV>
+// HT: tw holds only w_M^t for t < M / 2; w_M^(t + M/2) = -w_M^t.
+template <int LOGM, bool HT, class C> __device__ __forceinline__ C ct_tw(const C* __restrict__ tw, int idx) {
+    if constexpr (HT) {
+        const C u = tw[idx & ((1 << (LOGM - 1)) - 1)];
+        return idx < (1 << (LOGM - 1)) ? u : C{-u.x, -u.y};
+    } else {
+        return tw[idx];
+    }
+}
+
+template <class C, int LOGM, int LOGW2, int LOGR, int LOGNS, int BS, bool INV, bool HT = false>
 __device__ __forceinline__ void ct_stage(C* buf, const C* __restrict__ tw, const int tid) {
     constexpr int R = 1 << LOGR, W2 = 1 << LOGW2, NS = 1 << LOGNS;
     constexpr int NB = (1 << (LOGM - LOGR)) << LOGW2; // butterflies; also the stride between inputs
@@ -232,7 +242,7 @@ __device__ __forceinline__ void ct_stage(C* buf, const C* __restrict__ tw, const
             const int j = b >> LOGW2, l = b & (W2 - 1), k = j & (NS - 1);
             if constexpr (NS > 1) {
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], conj_if(tw[(r * k) << TWS], INV));
+                for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], conj_if(ct_tw<LOGM, HT>(tw, (r * k) << TWS), INV));
             }
             dft<C, R>(v[i], INV, tw, 1 << LOGM);
             const int o = ((((j - k) << LOGR) + k) << LOGW2) + l;
@@ -245,11 +255,11 @@ __device__ __forceinline__ void ct_stage(C* buf, const C* __restrict__ tw, const
 
 // All stages of W2 lines of length M held in buf[pos * W2 + line]; tw = w_M^t, t < M.  tid is
 // the lane's index in the group (threadIdx.x, or an opaque copy of it in persistent loops).
-template <class C, int LOGM, int LOGW2, int BS, bool INV, int S = 0, int SEND = ct_nstages(LOGM)>
+template <class C, int LOGM, int LOGW2, int BS, bool INV, int S = 0, int SEND = ct_nstages(LOGM), bool HT = false>
 __device__ __forceinline__ void ct_fft(C* buf, const C* __restrict__ tw, const int tid = static_cast<int>(threadIdx.x)) {
     if constexpr (S < SEND) {
-        ct_stage<C, LOGM, LOGW2, ct_logr(LOGM, S), ct_logns(LOGM, S), BS, INV>(buf, tw, tid);
-        ct_fft<C, LOGM, LOGW2, BS, INV, S + 1, SEND>(buf, tw, tid);
+        ct_stage<C, LOGM, LOGW2, ct_logr(LOGM, S), ct_logns(LOGM, S), BS, INV, HT>(buf, tw, tid);
+        ct_fft<C, LOGM, LOGW2, BS, INV, S + 1, SEND, HT>(buf, tw, tid);
     }
 }
 
@@ -271,12 +281,13 @@ template <int LOGM, int LOGW2, int S> struct CtStage {
         const int j = b >> LOGW2, k = j & (NS - 1);
         return ((j - k) << LOGR) + k + (r << LOGNS);
     }
-    template <class C, bool INV>
+    template <class C, bool INV, bool HT = false>
     __device__ static __forceinline__ void bfly(C (&v)[R], int b, const C* __restrict__ tw) {
         if constexpr (NS > 1) {
             const int k = (b >> LOGW2) & (NS - 1);
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], conj_if(tw[(r * k) << (LOGM - LOGNS - LOGR)], INV));
+            for (int r = 1; r < R; ++r)
+                v[r] = cmul(v[r], conj_if(ct_tw<LOGM, HT>(tw, (r * k) << (LOGM - LOGNS - LOGR)), INV));
         }
         dft<C, R>(v, INV, tw, 1 << LOGM);
     }
