@@ -101,7 +101,9 @@ int mafx_device_count(int* count);
  * Ms_field_host != NULL, State ctor semantics of state.cpp:71-111), one term list and one
  * precision.  Uploads parameters, builds the demag / dipole tensor on the GPU in FP64
  * (demag_field.cpp:104-147), plans the FFTs and allocates the workspace.  Replaces the
- * construction of the reference's terms plus State::Ms/Ms_field. */
+ * construction of the reference's terms plus State::Ms/Ms_field.  Reads MAFX_NUM_CUS (CU count
+ * for the persistent transform grids; default: the device's) and, with MAFX_TERM_CACHING,
+ * MAFX_CACHE_DIR / MAFX_CACHE_MAX_BYTES. */
 int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_host,
                     const mafx_term_desc* terms, int nterms, int precision, mafx_ctx** out);
 int mafx_ctx_destroy(mafx_ctx* ctx);
