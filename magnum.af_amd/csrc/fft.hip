@@ -302,9 +302,13 @@ __global__ void __launch_bounds__(BS, MINW)
             if constexpr (MS) q[i] = *reinterpret_cast<const double2*>(ms + cell0 + o);
         }
     };
+    // FP64 with an Ms field would spill its doubled prefetch: those groups take one row group
+    // each and load it at the top
+    constexpr bool PF = !(MS && sizeof(T) == 8);
     int grp = static_cast<int>(blockIdx.x);
-    fetch(grp);
+    if constexpr (PF) fetch(grp);
     for (; grp < ngroups; grp += static_cast<int>(gridDim.x)) {
+        if constexpr (!PF) fetch(grp);
         const int lt = opaque_int(tid);
         const auto val = [&](int i) {
             if constexpr (MS)
@@ -338,7 +342,7 @@ __global__ void __launch_bounds__(BS, MINW)
         }
         __syncthreads();
         const int next = grp + static_cast<int>(gridDim.x);
-        if (next < ngroups) fetch(next);
+        if (PF && next < ngroups) fetch(next);
         ct_fft<C, LOGM, LOGW, BS, false, RS ? 1 : 0, ct_nstages(LOGM), true>(buf, tws, lt);
         // X[k] = E + w_L^k O, E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2, k <= M
         uint32_t c, z;
@@ -980,7 +984,9 @@ void fft_x_forward(mafx_ctx* ctx, const double* in, const double* ms, double sca
         ProfScope ps(ctx, nm("fft_x_fwd_f32", "fft_x_fwd_f64", sizeof(C)), bytes);
         const size_t lds = (lds_padded(1 << LOGE) + M / 2) * sizeof(C);
         allow_lds(reinterpret_cast<const void*>(fc), lds);
-        const unsigned grid = static_cast<unsigned>(std::min<int64_t>(groups, int64_t{kXGroupsPerCU} * ctx->num_cus));
+        const bool pf = !(ms && sizeof(C) == sizeof(double2)); // the kernel's PF
+        const unsigned grid =
+            static_cast<unsigned>(pf ? std::min<int64_t>(groups, int64_t{kXGroupsPerCU} * ctx->num_cus) : groups);
         hipLaunchKernelGGL(fc, dim3(grid), dim3(512), lds, ctx->stream, in, ms, scale, static_cast<int>(Yr),
                            make_fastdiv(static_cast<uint32_t>(Yr / W)), make_fastdiv(static_cast<uint32_t>(Zr)),
                            tw_of<C>(ph), tw_of<C>(ctx->px), out, ctx->n1 * ctx->hxs, n2o,
