@@ -9,6 +9,7 @@ allocator); every field evaluation, integrator step and minimizer iteration runs
 import ctypes as C
 import math
 import os
+import sys
 
 import numpy as np
 
@@ -106,8 +107,9 @@ class Context:
         self.h = h
         check(lib().mafx_ctx_set_stream(self.h, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
-    def __del__(self):
-        if getattr(self, "h", None):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the process releases the device memory itself
+        if getattr(self, "h", None) and not _finalizing():
             lib().mafx_ctx_destroy(self.h)
             self.h = None
 
@@ -465,8 +467,9 @@ class LLGIntegrator:
         if self._ctrl is not None:
             check(lib().mafx_llg_set_controller_state(self._llg, *self._ctrl))
 
-    def __del__(self):
-        if getattr(self, "_llg", None):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the process releases the device memory itself
+        if getattr(self, "_llg", None) and not _finalizing():
             lib().mafx_llg_destroy(self._llg)
             self._llg = None
 

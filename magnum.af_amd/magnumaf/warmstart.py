@@ -18,6 +18,7 @@ D in [1e-3, 4e-3] J/m^2, Ku in [2e5, 9e5] J/m^3, Bz in [0, 0.5] T.
 """
 import ctypes as C
 import math
+import sys
 import threading
 import time
 
@@ -72,8 +73,9 @@ class WarmStartNet:
         check(lib().mafx_net_param_count(self.h, C.byref(n)))
         self.n_params = n.value
 
-    def __del__(self):
-        if getattr(self, "h", None):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the process releases the device memory itself
+        if getattr(self, "h", None) and not _finalizing():
             lib().mafx_net_destroy(self.h)
             self.h = None
 
