@@ -73,10 +73,15 @@ enum mafx_term_kind {
  *   flags:   DEMAG / ATOM_DIPOLE: MAFX_TERM_CACHING reads the transformed tensor from the on-disk
  *            cache and writes it there when absent, MAFX_TERM_VERBOSE reports cache activity on
  *            stdout (the caching and verbose arguments of DemagField(mesh, verbose, caching),
- *            demag_field.cpp:185-204, cache_manager.cpp:49-114)
+ *            demag_field.cpp:185-204, cache_manager.cpp:49-114).  MAFX_TERM_HOST_TENSOR (no
+ *            counterpart in the reference, whose only tensor is this one): on the MAFX_F32_FIELD
+ *            path, build the real-space tensor on the host in the reference's operation order,
+ *            as the FP64 path does, instead of on the GPU; the setup takes seconds to minutes,
+ *            and the field then carries the reference's far-field rounding noise
  */
 #define MAFX_TERM_CACHING 1
 #define MAFX_TERM_VERBOSE 2
+#define MAFX_TERM_HOST_TENSOR 4
 typedef struct mafx_term_desc {
     int32_t kind;
     int32_t flags;

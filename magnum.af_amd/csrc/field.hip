@@ -909,7 +909,9 @@ void build_tensor(mafx_ctx* ctx) {
     const uint64_t stored = ctx->precision == MAFX_F64
                                 ? sizeof(double2) * 6 * static_cast<uint64_t>(K)
                                 : sizeof(float) * 6 * static_cast<uint64_t>(ctx->n2h * ctx->n1h * hxs);
-    const TensorCacheKey key{n0, n1, n2, hxs, ctx->precision, ctx->atom_dipole ? 1 : 0,
+    // the FP32 layout from the host-built real-space tensor is stored under its own key
+    const bool host32 = ctx->precision != MAFX_F64 && (ctx->demag_flags & MAFX_TERM_HOST_TENSOR);
+    const TensorCacheKey key{n0, n1, n2, hxs, ctx->precision | (host32 ? kCacheHostTensor : 0), ctx->atom_dipole ? 1 : 0,
                              ctx->mesh.dx, ctx->mesh.dy, ctx->mesh.dz};
     if (caching) {
         std::vector<char> host;
@@ -936,7 +938,7 @@ void build_tensor_uncached(mafx_ctx* ctx) {
     const int64_t P = n0 * n1 * n2, K = n2 * n1 * hxs;
     DevBuf Nd, T64;
     Nd.alloc(sizeof(double) * 6 * P);
-    if (ctx->precision == MAFX_F64) {
+    if (ctx->precision == MAFX_F64 || (ctx->demag_flags & MAFX_TERM_HOST_TENSOR)) {
         auto Nh = host_tensor(ctx->mesh, ctx->atom_dipole, P);
         MAFX_HIP(hipMemcpy(Nd.p, Nh->data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
     } else {

@@ -88,7 +88,7 @@ std::string tensor_cache_name(const TensorCacheKey& k) {
     std::string s = "Nfft_n0exp_" + std::to_string(k.n0) + "_n1exp_" + std::to_string(k.n1) + "_n2exp_" +
                     std::to_string(k.n2) + "_dx_" + std::to_string(1e9 * k.dx) + "_dy_" + std::to_string(1e9 * k.dy) +
                     "_dz_" + std::to_string(1e9 * k.dz);
-    s += k.precision == 0 ? "_mafx_c64" : "_mafx_f32sym";
+    s += k.precision == 0 ? "_mafx_c64" : (k.precision & kCacheHostTensor) ? "_mafx_f32sym_host" : "_mafx_f32sym";
     if (k.dipole) s += "_dipole";
     s += "_" + hex_bits(k.dx) + hex_bits(k.dy) + hex_bits(k.dz);
     return s;

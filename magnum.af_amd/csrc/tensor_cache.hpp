@@ -13,9 +13,12 @@
 
 namespace mafx {
 
+constexpr int32_t kCacheHostTensor = 0x100;
+
 struct TensorCacheKey {
     int64_t n0, n1, n2, hxs; // expanded lengths and the padded row stride of the stored planes
-    int32_t precision;       // MAFX_F64 (complex half spectrum) or MAFX_F32_FIELD (real quarter)
+    int32_t precision;       // MAFX_F64 (complex half spectrum) or MAFX_F32_FIELD (real quarter),
+                             // | kCacheHostTensor when the FP32 tensor comes from the host tensor
     int32_t dipole;          // 1 for the atomistic point-dipole tensor
     double dx, dy, dz;
 };

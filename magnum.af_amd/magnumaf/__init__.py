@@ -298,15 +298,19 @@ class DemagField(FieldTerm):
     magnumaf.pyx:1086-1121).  caching reads the transformed tensor from the on-disk cache
     (MAFX_CACHE_DIR, else <tmp>/magnumaf.cache) and writes it there when absent; verbose reports
     the cache activity.  nthreads is accepted for the reference's signature (the host tensor uses
-    every hardware thread)."""
+    every hardware thread).  host_tensor (keyword only, not in the reference): the FP32-field path
+    builds its real-space tensor on the host in the reference's operation order, as the FP64 path
+    does, instead of on the GPU."""
 
-    def __init__(self, mesh, verbose=False, caching=False, nthreads=4):
+    def __init__(self, mesh, verbose=False, caching=False, nthreads=4, *, host_tensor=False):
         self.mesh = mesh
         self.verbose, self.caching, self.nthreads = bool(verbose), bool(caching), nthreads
+        self.host_tensor = bool(host_tensor)
 
     def _desc(self, mesh):
         d = _td(_abi.DEMAG)
-        d.flags = (_abi.TERM_CACHING if self.caching else 0) | (_abi.TERM_VERBOSE if self.verbose else 0)
+        d.flags = ((_abi.TERM_CACHING if self.caching else 0) | (_abi.TERM_VERBOSE if self.verbose else 0)
+                   | (_abi.TERM_HOST_TENSOR if self.host_tensor else 0))
         return d, []
 
 

@@ -8,7 +8,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libmafx.so")
 
 MAFX_F64, MAFX_F32_FIELD = 0, 1
 DEMAG, EXCHANGE, DMI, UNIAXIAL, ZEEMAN, RKKY = 1, 2, 3, 4, 5, 6
-TERM_CACHING, TERM_VERBOSE = 1, 2  # mafx_term_desc.flags (include/mafx.h)
+TERM_CACHING, TERM_VERBOSE, TERM_HOST_TENSOR = 1, 2, 4  # mafx_term_desc.flags (include/mafx.h)
 ATOM_DIPOLE, ATOM_EXCHANGE, ATOM_DMI, ATOM_UNIAXIAL, ATOM_ZEEMAN = 11, 12, 13, 14, 15
 
 # every entry point include/mafx.h declares

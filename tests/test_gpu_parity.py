@@ -327,6 +327,23 @@ def test_heff_fp32_field_error(mesh):
     assert err_d <= (FP32_TOL if mesh[0] < 2048 else FP32_TOL_LONG_ROWS), err_d
 
 
+def test_fp32_host_tensor():
+    """DemagField(host_tensor=True) on the FP32-field path: the real-space tensor comes from the host
+    in the reference's operation order (the FP64 path's), so the field matches the oracle within the
+    FP32 tolerance and differs from the default device-built tensor only by far-field rounding."""
+    M = mf()
+    mesh = (256, 16, 2, 2e-9, 2e-9, 2e-9)
+    print(f"\n[fp32 host tensor] mesh={mesh}", flush=True)
+    m = rand_m(mesh[:3], 19)
+    g = M.Mesh(*mesh)
+    sim = O.Sim(mesh, 8e5, m, [O.term(O.DEMAG)])
+    href = sim.heff()
+    h_host = M.LLGIntegrator(0.1, [M.DemagField(g, host_tensor=True)], precision="f32").H_in_Apm(M.State(g, 8e5, m))
+    h_dev = M.LLGIntegrator(0.1, [M.DemagField(g)], precision="f32").H_in_Apm(M.State(g, 8e5, m))
+    assert normwise(h_host, href) <= FP32_TOL, normwise(h_host, href)
+    assert normwise(h_host, h_dev) <= FP32_TOL, normwise(h_host, h_dev)
+
+
 def test_fp32_ms_field_airbox():
     x = y = z = 1e-9
     nx, ny, nz = 6, 5, 4
