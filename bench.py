@@ -205,6 +205,8 @@ def main():
 
     # ---------------------------------------------------------------- CPU baseline (oracle)
     cpu = None
+    sim = None
+    ref_total = None
     if not args.no_cpu:
         import oracle_lib as O
         threads = int(os.environ.get("OMP_NUM_THREADS", str(os.cpu_count() or 1)))
@@ -214,12 +216,16 @@ def main():
                     nthreads=threads)
         log(f"[bench] oracle setup {time.time() - t0:.1f} s")
         t0 = time.perf_counter()
-        sim.heff_flat()
+        ref_total = sim.heff_flat()
         cpu_s = time.perf_counter() - t0
         cpu = {"value": 1.0 / cpu_s, "unit": "H_eff evals/s", "cores": threads, "kind": "port",
-               "sample": f"one FP64 H_eff of the restated reference (zero-padded FFT demag + 27-tap stencils) "
-                         f"at the full {nx}x{ny}x{nz} mesh on {threads} host threads; tensor setup untimed"}
+               "sample": f"one FP64 H_eff of the oracle, the unoptimised correctness restatement of the reference "
+                         f"(complex 3-D FFTs through a recursive mixed-radix DFT, 27-tap stencils; not a tuned CPU "
+                         f"path), at the full {nx}x{ny}x{nz} mesh on {threads} host threads; tensor setup untimed"}
         log(f"[bench] oracle H_eff {cpu_s:.2f} s")
+
+    # ---------------------------------------------------------------- field errors per term
+    errors = field_errors(M, torch, mesh, state, terms, ctx32, ctx64, sim, ref_total)
 
     out = {
         "metric": "H_eff evals/s & DP45 steps/s at 1024^2x4 cells; LBFGS time-to-tolerance",
@@ -246,6 +252,7 @@ def main():
             "dp45_attempts_per_s_fp32": at32, "dp45_attempts_per_s_fp64": at64,
             "dp45_fp32_over_fp64": dp32 / dp64 if dp64 > 0 else None,
             "heff_fp32_normwise_error_vs_fp64": heff_err32,
+            "field_errors": errors,
             "heff_algorithmic_GB_fp32": bytes_heff32 / 1e9, "heff_algorithmic_GB_fp64": bytes_heff64 / 1e9,
             "heff_kernel_ms_fp32": per_heff_ms32, "heff_kernel_ms_fp64": per_heff_ms64,
             "heff_frac_of_hbm_peak_fp32": bytes_heff32 / (per_heff_ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -255,6 +262,59 @@ def main():
         },
     }
     print(json.dumps(out))
+
+
+def field_errors(M, torch, mesh, state, terms, ctx32, ctx64, sim, ref_total):
+    """Normwise errors (max|dH| / max|H_ref|) of each term's field and of H_eff at the start state,
+    against the oracle's FP64 fields (or the GPU FP64 path, which matches the oracle to 1e-12, when
+    the oracle is not run).  The demag error is split three ways: the FP64 pipeline on the FP32
+    path's GPU-built tensor isolates the tensor's far-field rounding noise, and the FP32 path against
+    that run isolates its FP32 arithmetic; the FP32 path on the host-built tensor is the third."""
+    names = ["demag", "exchange", "dmi", "uniaxial"]
+    n3 = 3 * mesh.n
+    buf = torch.empty(n3, dtype=torch.float64, device="cuda")
+
+    def gpu(ctx, mask):
+        ctx.heff(state.m_dev, mask=mask, out=buf)
+        torch.cuda.synchronize()
+        return buf.cpu().numpy()
+
+    def nw(a, b):
+        return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+    ref_name = "oracle" if sim is not None else "gpu_fp64"
+    out = {"reference": ref_name, "terms": {}}
+    ref_demag = None
+    for k, name in enumerate(names):
+        mask = 1 << k
+        h64 = gpu(ctx64, mask)
+        ref = sim.heff_flat(mask) if sim is not None else h64
+        h32 = gpu(ctx32, mask)
+        out["terms"][name] = {"max_abs_H_A_per_m": float(np.max(np.abs(ref))),
+                              "fp64_vs_ref": nw(h64, ref), "fp32_vs_ref": nw(h32, ref),
+                              "fp32_max_abs_err_A_per_m": float(np.max(np.abs(h32 - ref)))}
+        if name == "demag":
+            ref_demag = ref
+            h32_gpu_tensor = h32
+        log(f"[bench] {name}: FP64 {out['terms'][name]['fp64_vs_ref']:.2e}, FP32 {out['terms'][name]['fp32_vs_ref']:.2e} "
+            f"normwise vs {ref_name}")
+    h64 = gpu(ctx64, 0xFFFFFFFF)
+    href = ref_total if ref_total is not None else h64
+    out["heff"] = {"fp64_vs_ref": nw(h64, href), "fp32_vs_ref": nw(gpu(ctx32, 0xFFFFFFFF), href)}
+    # demag split: the FP64 pipeline on the GPU-built tensor, the FP32 path on the host-built tensor
+    c64dev = state._context([M.DemagField(mesh, device_tensor=True)], "f64")
+    d64dev = gpu(c64dev, 1)
+    del state._ctx[(tuple(id(t) for t in c64dev.terms), "f64")]
+    c32host = state._context([M.DemagField(mesh, host_tensor=True)], "f32")
+    d32host = gpu(c32host, 1)
+    del state._ctx[(tuple(id(t) for t in c32host.terms), "f32")]
+    out["demag_split"] = {
+        "fp64_on_gpu_tensor_vs_ref": nw(d64dev, ref_demag),        # far-field tensor noise alone
+        "fp32_vs_fp64_on_the_same_gpu_tensor": nw(h32_gpu_tensor, d64dev),  # FP32 arithmetic alone
+        "fp32_on_host_tensor_vs_ref": nw(d32host, ref_demag),
+        "fp32_on_gpu_tensor_vs_ref": out["terms"]["demag"]["fp32_vs_ref"]}
+    log(f"[bench] demag split: {out['demag_split']}")
+    return out
 
 
 def config_c1(M, torch, args):

@@ -60,6 +60,18 @@ enum mafx_term_kind {
     MAFX_ATOM_ZEEMAN = 15    /* AtomisticExternalField(H)   field_terms/atom/atomistic_external_field.cpp:5-11 */
 };
 
+/* Time-dependent field of an ExternalField callback (external_field.cpp:13-17,28-36, whose
+ * callbacks take the State and return the field).  Called on the calling host thread at every
+ * evaluation that includes the term, with the evaluation time t (the context time of
+ * mafx_ctx_set_time, or t + c_i h at stage i of an integrator attempt) and the magnetisation being
+ * evaluated (m, device, [3N] f64).  It writes the field in A/m into h_out and returns 0, any other
+ * value fails the evaluation with MAFX_ERR_ARG.  h_out is the library's device buffer [3N] f64,
+ * to be filled on `stream` (a hipStream_t) before the callback returns.  With
+ * MAFX_TERM_HOMOGENEOUS_CALLBACK h_out is a host double[3] holding one homogeneous field; with
+ * MAFX_TERM_HOST_CALLBACK m and h_out are host buffers [3N] (the library copies m down before the
+ * call and the field up after it), for callers that compute the field on the host. */
+typedef int (*mafx_field_fn)(void* user, double t, const double* m, double* h_out, void* stream);
+
 /* One field term.  Unused members are ignored.
  *   value:   A [J/m] (EXCHANGE), D [J/m^2] (DMI), Ku1 [J/m^3] (UNIAXIAL),
  *            J_atom / D_atom / K_atom [J] (ATOM_*)
@@ -67,9 +79,17 @@ enum mafx_term_kind {
  *            reference constructors do (dmi_field.cpp:43-47, uniaxial_anisotropy_field.cpp:11-12)
  *   field:   homogeneous external field [A/m] (ZEEMAN / ATOM_ZEEMAN) when array0 is NULL
  *   array0:  ZEEMAN / ATOM_ZEEMAN: host vector field [3*N] in A/m (overrides `field`);
- *            RKKY: host scalar field of RKKY constants [N] (J/m, the reference's rkky_values)
- *   array1:  RKKY: host scalar field of exchange constants A [N] (J/m)
+ *            RKKY: host scalar field of RKKY constants [N] (J/m, the reference's rkky_values);
+ *            DMI: host scalar field of D [N] in J/m^2, overrides `value`
+ *            (DmiField(af::array D_constants, axis), dmi_field.cpp:17-19,97-109);
+ *            UNIAXIAL: host scalar field of Ku1 [N] in J/m^3, overrides `value`
+ *            (UniaxialAnisotropyField(af::array Ku1_field, ...), uniaxial_anisotropy_field.cpp:14-16,78-88)
+ *   array1:  RKKY: host scalar field of exchange constants A [N] (J/m);
+ *            UNIAXIAL: host vector field [3*N] of anisotropy axes, overrides `axis`; each cell's
+ *            axis is divided by its length as util::normalize does (uniaxial_anisotropy_field.cpp:23-25,60-69)
  *   index0:  RKKY: host scalar field of uint32 layer indices [N], or NULL for all zero
+ *   callback, callback_user: ZEEMAN / ATOM_ZEEMAN: a time-dependent field (mafx_field_fn),
+ *            overrides `field` and array0
  *   flags:   DEMAG / ATOM_DIPOLE: MAFX_TERM_CACHING reads the transformed tensor from the on-disk
  *            cache and writes it there when absent, MAFX_TERM_VERBOSE reports cache activity on
  *            stdout (the caching and verbose arguments of DemagField(mesh, verbose, caching),
@@ -77,11 +97,20 @@ enum mafx_term_kind {
  *            counterpart in the reference, whose only tensor is this one): on the MAFX_F32_FIELD
  *            path, build the real-space tensor on the host in the reference's operation order,
  *            as the FP64 path does, instead of on the GPU; the setup takes seconds to minutes,
- *            and the field then carries the reference's far-field rounding noise
+ *            and the field then carries the reference's far-field rounding noise.
+ *            MAFX_TERM_DEVICE_TENSOR (no counterpart in the reference either): on the MAFX_F64
+ *            path, use the real-space tensor the MAFX_F32_FIELD path builds on the GPU instead of
+ *            the host-built one, so that the two paths can be compared on one tensor.
+ *            ZEEMAN / ATOM_ZEEMAN: MAFX_TERM_HOMOGENEOUS_CALLBACK, the callback writes one
+ *            homogeneous field into a host double[3]; MAFX_TERM_HOST_CALLBACK, the callback
+ *            reads m from and writes the field into host buffers
  */
 #define MAFX_TERM_CACHING 1
 #define MAFX_TERM_VERBOSE 2
 #define MAFX_TERM_HOST_TENSOR 4
+#define MAFX_TERM_HOMOGENEOUS_CALLBACK 8
+#define MAFX_TERM_HOST_CALLBACK 16
+#define MAFX_TERM_DEVICE_TENSOR 32
 typedef struct mafx_term_desc {
     int32_t kind;
     int32_t flags;
@@ -91,6 +120,8 @@ typedef struct mafx_term_desc {
     const double* array0;
     const double* array1;
     const uint32_t* index0;
+    mafx_field_fn callback;
+    void* callback_user;
 } mafx_term_desc;
 
 typedef struct mafx_ctx mafx_ctx;
@@ -117,6 +148,19 @@ int mafx_ctx_set_stream(mafx_ctx* ctx, void* stream);
 int mafx_ctx_sync(mafx_ctx* ctx);
 /* Device bytes held by the context (tensor, FFT workspace, parameters). */
 int mafx_ctx_device_bytes(const mafx_ctx* ctx, int64_t* bytes);
+/* Time of the State the next field and energy evaluations see (State::t, state.hpp), used by
+ * time-dependent ExternalField callbacks only.  The integrator sets it per RK stage itself. */
+int mafx_ctx_set_time(mafx_ctx* ctx, double t);
+/* Replaces the parameters of term `index` (same kind, same mesh) in place: the values, axes and
+ * arrays of desc are uploaded again, as assigning a term's members does in the reference
+ * (UniaxialAnisotropyField::Ku1, ExternalField::set_homogeneous_field, external_field.cpp:21-26).
+ * A DEMAG / ATOM_DIPOLE term keeps its tensor.  Integrators and minimizers of the context see the
+ * new parameters from their next evaluation on. */
+int mafx_ctx_update_term(mafx_ctx* ctx, int index, const mafx_term_desc* desc);
+/* Replaces the material: Ms (A/m, or mu_s in J/T for atomistic terms) or, when Ms_field_host is
+ * not NULL, the cell-wise Ms_field [N] (State::Ms, State::set_Ms_field, state.cpp:135,
+ * magnumaf.pyx:777-784,839-842). */
+int mafx_ctx_set_material(mafx_ctx* ctx, double Ms, const double* Ms_field_host);
 
 /* Real-space Newell demag tensor on the expanded grid, [6][n2][n1][n0] with n0 = 2 nx,
  * n1 = 2 ny, n2 = (nz == 1 ? 1 : 2 nz), components (xx, xy, xz, yy, yz, zz), evaluated on the host
@@ -132,9 +176,11 @@ int mafx_demag_calc_N(const mafx_mesh* mesh, double* N_host, int nthreads);
  * max_size_in_byte) the oldest files are removed until it is below a tenth of that (2e8 by
  * default, the reference's shrink_size_in_byte).  A file that fails its header or checksum check
  * is ignored and the tensor computed instead, as the reference does when af::readArray fails.
- * mafx_demag_cache_name writes the file name for a mesh (needs no GPU); mafx_ctx_tensor_source
+ * mafx_demag_cache_name writes the file name for a mesh, precision and DEMAG term flags (the
+ * MAFX_TERM_HOST_TENSOR / MAFX_TERM_DEVICE_TENSOR layouts have names of their own; needs no GPU); mafx_ctx_tensor_source
  * tells where a context's tensor came from: 0 no tensor, 1 computed, 2 read from the cache. */
-int mafx_demag_cache_name(const mafx_mesh* mesh, int precision, int dipole, char* buf, int64_t buf_len);
+int mafx_demag_cache_name(const mafx_mesh* mesh, int precision, int term_flags, int dipole, char* buf,
+                          int64_t buf_len);
 int mafx_ctx_tensor_source(const mafx_ctx* ctx, int* source);
 
 /* ------------------------------------------------------------------ field and energy */
@@ -175,16 +221,21 @@ int mafx_llg_create(mafx_ctx* ctx, double alpha, const char* scheme, double hmin
                     double atol, double rtol, int dissipation_term_only, mafx_llg** out);
 int mafx_llg_destroy(mafx_llg* llg);
 int mafx_llg_set_alpha(mafx_llg* llg, double alpha);
+/* Cell-wise damping (LLGIntegrator(af::array alpha_field, ...), llg_integrator.hpp:32-40,
+ * llg_integrator.cpp:18-38): alpha_host [N] is copied to the device and replaces the scalar alpha
+ * in every later right-hand side; NULL returns to the scalar alpha. */
+int mafx_llg_set_alpha_field(mafx_llg* llg, const double* alpha_host);
 /* One accepted step of AdaptiveRungeKutta::step (adaptive_runge_kutta.cpp:39-71): repeats
  * attempts until the controller accepts, then t += h, m += dm, renormalise.  m_dev is
  * updated in place, *t_inout advanced. */
 int mafx_llg_step(mafx_llg* llg, double* m_dev, double* t_inout);
 /* LLGIntegrator::relax (llg_integrator.cpp:43-66): while the relative energy change
  * |(E_prev - E) / E_prev| between two checks exceeds precision (E_prev starts at 1e20), takes
- * eval_E steps; with verbose, prints the relative change and <m> whenever the integrator's step
- * count is a multiple of iwritecout (> 0).  E is the total energy of all terms of ctx. */
+ * eval_E steps; with verbose, prints the relative change and <m> whenever the State's step count
+ * (state_steps before the call plus the steps taken) is a multiple of iwritecout (> 0).  E is the
+ * total energy of all terms of ctx. */
 int mafx_llg_relax(mafx_llg* llg, double* m_dev, double* t_inout, double precision, int64_t eval_E,
-                   int64_t iwritecout, int verbose, int64_t* nsteps_done);
+                   int64_t iwritecout, int verbose, int64_t state_steps, int64_t* nsteps_done);
 /* Runs mafx_llg_step until *t_inout >= t_end or nsteps_max steps were taken. */
 int mafx_llg_run_until(mafx_llg* llg, double* m_dev, double* t_inout, double t_end,
                        int64_t nsteps_max, int64_t* nsteps_done);

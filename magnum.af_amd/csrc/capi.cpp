@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "fft_kernels.hpp"
 #include "mafx_internal.hpp"
@@ -114,6 +115,84 @@ ProfScope::~ProfScope() {
     }
 }
 
+// Parameters of one term: scalars, normalised axes, and the arrays uploaded to the device.
+void set_term(mafx_ctx* ctx, TermDev& t, const mafx_term_desc& d) {
+    const int64_t N = ctx->N;
+    require(t.kind == 0 || t.kind == d.kind, "a term keeps its kind");
+    t.kind = d.kind;
+    t.flags = d.flags;
+    t.value = d.value;
+    for (int c = 0; c < 3; ++c) t.field[c] = d.field[c];
+    t.callback = nullptr;
+    t.callback_user = nullptr;
+    t.callback_homogeneous = false;
+    t.callback_host = false;
+    switch (d.kind) {
+    case MAFX_DEMAG:
+    case MAFX_ATOM_DIPOLE:
+    case MAFX_EXCHANGE:
+    case MAFX_ATOM_EXCHANGE: break;
+    case MAFX_DMI:
+    case MAFX_UNIAXIAL:
+    case MAFX_ATOM_DMI:
+    case MAFX_ATOM_UNIAXIAL:
+        normalize_axis(d.axis, t.axis);
+        t.pfield.release();
+        t.axfield.release();
+        if ((d.kind == MAFX_DMI || d.kind == MAFX_UNIAXIAL) && d.array0) upload(t.pfield, d.array0, sizeof(double) * N);
+        if (d.kind == MAFX_UNIAXIAL && d.array1) {
+            // util::normalize (util.cpp): every cell's axis divided by sqrt(a0 a0 + a1 a1 + a2 a2)
+            std::vector<double> ax(d.array1, d.array1 + 3 * N);
+            for (int64_t i = 0; i < N; ++i) {
+                const double a0 = ax[i], a1 = ax[N + i], a2 = ax[2 * N + i];
+                const double n = std::sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+                ax[i] = a0 / n;
+                ax[N + i] = a1 / n;
+                ax[2 * N + i] = a2 / n;
+            }
+            upload(t.axfield, ax.data(), sizeof(double) * 3 * N);
+        }
+        break;
+    case MAFX_ZEEMAN:
+    case MAFX_ATOM_ZEEMAN:
+        t.has_zee_array = false;
+        if (d.callback) {
+            t.callback = d.callback;
+            t.callback_user = d.callback_user;
+            t.callback_homogeneous = (d.flags & MAFX_TERM_HOMOGENEOUS_CALLBACK) != 0;
+            t.callback_host = !t.callback_homogeneous && (d.flags & MAFX_TERM_HOST_CALLBACK) != 0;
+            if (t.callback_host) {
+                t.cb_m.resize(3 * N);
+                t.cb_h.assign(3 * N, 0.0);
+            }
+            if (!t.callback_homogeneous) {
+                if (t.zee.bytes != sizeof(double) * 3 * N) t.zee.alloc(sizeof(double) * 3 * N);
+                t.has_zee_array = true;
+            } else {
+                t.zee.release();
+            }
+        } else if (d.array0) {
+            upload(t.zee, d.array0, sizeof(double) * 3 * N);
+            t.has_zee_array = true;
+        } else {
+            t.zee.release();
+        }
+        break;
+    case MAFX_RKKY:
+        require(d.array0 && d.array1, "RKKY needs rkky values (array0) and exchange values (array1)");
+        upload(t.rkky, d.array0, sizeof(double) * N);
+        upload(t.aex, d.array1, sizeof(double) * N);
+        t.has_ridx = false;
+        t.ridx.release();
+        if (d.index0) {
+            upload(t.ridx, d.index0, sizeof(uint32_t) * N);
+            t.has_ridx = true;
+        }
+        break;
+    default: throw Error(MAFX_ERR_ARG, "unknown term kind " + std::to_string(d.kind));
+    }
+}
+
 } // namespace mafx
 
 mafx_ctx::~mafx_ctx() {
@@ -134,12 +213,13 @@ int mafx_device_count(int* count) {
     });
 }
 
-int mafx_demag_cache_name(const mafx_mesh* mesh, int precision, int dipole, char* buf, int64_t buf_len) {
+int mafx_demag_cache_name(const mafx_mesh* mesh, int precision, int term_flags, int dipole, char* buf,
+                          int64_t buf_len) {
     return guarded([&] {
         require(mesh && buf && buf_len > 0, "mafx_demag_cache_name: null argument");
         require(precision == MAFX_F64 || precision == MAFX_F32_FIELD, "unknown precision");
         TensorCacheKey k{2 * mesh->nx, 2 * mesh->ny, mesh->nz == 1 ? 1 : 2 * mesh->nz, (mesh->nx + 1 + 15) / 16 * 16,
-                         precision, dipole ? 1 : 0, mesh->dx, mesh->dy, mesh->dz};
+                         tensor_cache_layout(precision, term_flags), dipole ? 1 : 0, mesh->dx, mesh->dy, mesh->dz};
         const std::string s = tensor_cache_name(k);
         require(static_cast<int64_t>(s.size()) < buf_len, "mafx_demag_cache_name: buffer too small");
         std::memcpy(buf, s.c_str(), s.size() + 1);
@@ -195,41 +275,13 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
         for (int i = 0; i < nterms; ++i) {
             const mafx_term_desc& d = terms[i];
             TermDev t;
-            t.kind = d.kind;
-            t.value = d.value;
-            for (int c = 0; c < 3; ++c) t.field[c] = d.field[c];
-            switch (d.kind) {
-            case MAFX_DEMAG:
-            case MAFX_ATOM_DIPOLE:
+            if (d.kind == MAFX_DEMAG || d.kind == MAFX_ATOM_DIPOLE) {
                 require(ctx->demag_term < 0, "at most one demag / dipole term per context");
                 ctx->demag_term = i;
                 ctx->atom_dipole = d.kind == MAFX_ATOM_DIPOLE;
                 ctx->demag_flags = d.flags;
-                break;
-            case MAFX_EXCHANGE:
-            case MAFX_ATOM_EXCHANGE: break;
-            case MAFX_DMI:
-            case MAFX_UNIAXIAL:
-            case MAFX_ATOM_DMI:
-            case MAFX_ATOM_UNIAXIAL: normalize_axis(d.axis, t.axis); break;
-            case MAFX_ZEEMAN:
-            case MAFX_ATOM_ZEEMAN:
-                if (d.array0) {
-                    upload(t.zee, d.array0, sizeof(double) * 3 * N);
-                    t.has_zee_array = true;
-                }
-                break;
-            case MAFX_RKKY:
-                require(d.array0 && d.array1, "RKKY needs rkky values (array0) and exchange values (array1)");
-                upload(t.rkky, d.array0, sizeof(double) * N);
-                upload(t.aex, d.array1, sizeof(double) * N);
-                if (d.index0) {
-                    upload(t.ridx, d.index0, sizeof(uint32_t) * N);
-                    t.has_ridx = true;
-                }
-                break;
-            default: throw Error(MAFX_ERR_ARG, "unknown term kind " + std::to_string(d.kind));
             }
+            set_term(ctx.get(), t, d);
             ctx->terms.push_back(std::move(t));
         }
         // partial-sum blocks: epilogue row groups, per-cell tiles (row tiles of 256 cells or
@@ -263,7 +315,8 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
                                                  ctx->hbuf.bytes + ctx->spec.bytes + ctx->tensor.bytes +
                                                  ctx->hdem.bytes);
         for (auto& t : ctx->terms)
-            ctx->device_bytes += static_cast<int64_t>(t.zee.bytes + t.rkky.bytes + t.aex.bytes + t.ridx.bytes);
+            ctx->device_bytes += static_cast<int64_t>(t.zee.bytes + t.rkky.bytes + t.aex.bytes + t.ridx.bytes +
+                                                      t.pfield.bytes + t.axfield.bytes);
         *out = ctx.release();
     });
 }
@@ -290,6 +343,46 @@ int mafx_ctx_sync(mafx_ctx* ctx) {
 
 int mafx_ctx_device_bytes(const mafx_ctx* ctx, int64_t* bytes) {
     return guarded([&] { *bytes = ctx->device_bytes; });
+}
+
+int mafx_ctx_set_time(mafx_ctx* ctx, double t) {
+    return guarded([&] {
+        require(ctx, "null context");
+        ctx->time = t;
+    });
+}
+
+int mafx_ctx_update_term(mafx_ctx* ctx, int index, const mafx_term_desc* desc) {
+    return guarded([&] {
+        require(ctx && desc, "null argument");
+        require(index >= 0 && index < static_cast<int>(ctx->terms.size()), "term index out of range");
+        require(desc->kind == ctx->terms[index].kind, "mafx_ctx_update_term: the term kind cannot change");
+        // kernels of earlier calls may still read the arrays being replaced
+        MAFX_HIP(hipStreamSynchronize(ctx->stream));
+        TermDev& t = ctx->terms[index];
+        const int64_t before = static_cast<int64_t>(t.zee.bytes + t.rkky.bytes + t.aex.bytes + t.ridx.bytes +
+                                                    t.pfield.bytes + t.axfield.bytes);
+        set_term(ctx, t, *desc);
+        ctx->device_bytes += static_cast<int64_t>(t.zee.bytes + t.rkky.bytes + t.aex.bytes + t.ridx.bytes +
+                                                  t.pfield.bytes + t.axfield.bytes) - before;
+    });
+}
+
+int mafx_ctx_set_material(mafx_ctx* ctx, double Ms, const double* Ms_field_host) {
+    return guarded([&] {
+        require(ctx, "null context");
+        MAFX_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->device_bytes -= static_cast<int64_t>(ctx->ms_field.bytes);
+        ctx->Ms = Ms;
+        if (Ms_field_host) {
+            upload(ctx->ms_field, Ms_field_host, sizeof(double) * ctx->N);
+            ctx->has_ms_field = true;
+        } else {
+            ctx->ms_field.release();
+            ctx->has_ms_field = false;
+        }
+        ctx->device_bytes += static_cast<int64_t>(ctx->ms_field.bytes);
+    });
 }
 
 int mafx_heff(mafx_ctx* ctx, const double* m_dev, uint32_t term_mask, void* h_dev, int h_is_f32) {
@@ -373,6 +466,20 @@ int mafx_llg_set_alpha(mafx_llg* llg, double alpha) {
     return guarded([&] { llg->alpha = alpha; });
 }
 
+int mafx_llg_set_alpha_field(mafx_llg* llg, const double* alpha_host) {
+    return guarded([&] {
+        require(llg, "null integrator");
+        MAFX_HIP(hipStreamSynchronize(llg->ctx->stream));
+        if (alpha_host) {
+            upload(llg->alpha_field, alpha_host, sizeof(double) * llg->ctx->N);
+            llg->has_alpha_field = true;
+        } else {
+            llg->alpha_field.release();
+            llg->has_alpha_field = false;
+        }
+    });
+}
+
 int mafx_llg_step(mafx_llg* llg, double* m_dev, double* t_inout) {
     return guarded([&] {
         require(llg && m_dev && t_inout, "null argument");
@@ -381,12 +488,13 @@ int mafx_llg_step(mafx_llg* llg, double* m_dev, double* t_inout) {
 }
 
 int mafx_llg_relax(mafx_llg* llg, double* m_dev, double* t_inout, double precision, int64_t eval_E,
-                   int64_t iwritecout, int verbose, int64_t* nsteps_done) {
+                   int64_t iwritecout, int verbose, int64_t state_steps, int64_t* nsteps_done) {
     return guarded([&] {
         require(llg && m_dev && t_inout, "null argument");
         require(eval_E > 0, "eval_E must be positive");
         mafx_ctx* ctx = llg->ctx;
         auto energy = [&] {
+            ctx->time = *t_inout;
             EvalArgs a;
             a.m = m_dev;
             a.mode = OutMode::ENERGY;
@@ -407,11 +515,13 @@ int mafx_llg_relax(mafx_llg* llg, double* m_dev, double* t_inout, double precisi
                 ++n;
             }
             E = energy();
-            if (verbose && iwritecout > 0 && llg->steps % iwritecout == 0) {
+            // llg_integrator.cpp:52: the State's step count decides the print cadence
+            const int64_t steps = state_steps + n;
+            if (verbose && iwritecout > 0 && steps % iwritecout == 0) {
                 double mm[3];
                 mean_m(ctx, m_dev, mm);
                 std::printf("relaxing: step=%lld, rel_diff= %e, <mx>=%f, <my>=%f, <mz>=%f\n",
-                            static_cast<long long>(llg->steps), std::fabs((E_prev - E) / E_prev), mm[0], mm[1], mm[2]);
+                            static_cast<long long>(steps), std::fabs((E_prev - E) / E_prev), mm[0], mm[1], mm[2]);
             }
         }
         if (verbose) std::printf("relaxed %e [s]. Current state.t= %e\n", *t_inout - t0, *t_inout);

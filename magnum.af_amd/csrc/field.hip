@@ -40,6 +40,8 @@ struct EpiTerm {
     const double* rk;
     const double* aex;
     const uint32_t* ridx;
+    const double* pf; // N: D_constants (DMI) or Ku1_field (UNIAXIAL), or null
+    const double* ax; // 3N: normalised Ku1_axis_field (UNIAXIAL), or null
 };
 
 struct EpiParams {
@@ -53,11 +55,13 @@ struct EpiParams {
     int nterms;
     int fft_slot; // term index fed by the FFT convolution, -1 if none
     int needs_nb; // some selected term reads the six neighbours (loaded with the centre cell)
+    int has_fields; // some selected term has a parameter field (run-time term loop only)
     EpiTerm t[kMaxTerms];
     int mode;
     void* out;
     int out_f32;
     double alpha, dt;
+    const double* alpha_f; // N cell-wise damping or null
     int diss_only;
     int want_energy;
     double* partials; // [nslots][nblocks]
@@ -181,8 +185,19 @@ __device__ __forceinline__ void cell_terms(const EpiParams& P, int64_t x, int64_
         const EpiTerm& tm = P.t[t];
         T h0 = 0, h1 = 0, h2 = 0;
         T e_extra = 0; // exchange energy boundary correction (h_withedges - h) . m
-        // micro coefficient with the Ms_field branch (cells with Ms == 0 give 0)
+        // micro coefficient with the Ms_field branch (cells with Ms == 0 give 0); the compile-time
+        // term programs are only used for lists without parameter fields
         auto micro_coef = [&](T& c) -> bool {
+            if (prog_count(PROG) == 0 && tm.pf) { // D_constants / Ku1_field: 2 X_i / (mu0 Ms)
+                const double num = 2. * tm.pf[i];
+                if (!P.ms) {
+                    c = static_cast<T>(num / (kMu0 * P.Ms));
+                    return true;
+                }
+                if (msi == 0) return false;
+                c = static_cast<T>(num / (kMu0 * msi));
+                return true;
+            }
             if (!P.ms) {
                 c = static_cast<T>(tm.c0);
                 return true;
@@ -256,7 +271,12 @@ __device__ __forceinline__ void cell_terms(const EpiParams& P, int64_t x, int64_
         case MAFX_UNIAXIAL: {
             T c;
             if (micro_coef(c)) {
-                const T u0 = static_cast<T>(tm.n[0]), u1 = static_cast<T>(tm.n[1]), u2 = static_cast<T>(tm.n[2]);
+                T u0 = static_cast<T>(tm.n[0]), u1 = static_cast<T>(tm.n[1]), u2 = static_cast<T>(tm.n[2]);
+                if (prog_count(PROG) == 0 && tm.ax) {
+                    u0 = static_cast<T>(tm.ax[i]);
+                    u1 = static_cast<T>(tm.ax[N + i]);
+                    u2 = static_cast<T>(tm.ax[2 * N + i]);
+                }
                 const T an = u0 * m0 + u1 * m1 + u2 * m2;
                 h0 = c * (u0 * an);
                 h1 = c * (u1 * an);
@@ -382,9 +402,11 @@ __device__ __forceinline__ void cell_terms(const EpiParams& P, int64_t x, int64_
         // equations.hpp:19-33 with math::cross4 (math.cpp:10-19); k = dt * f (adaptive_runge_kutta.cpp)
         const double a0 = mc1 * Hz - mc2 * Hy, a1 = mc2 * Hx - mc0 * Hz, a2 = mc0 * Hy - mc1 * Hx;
         const double b0 = mc1 * a2 - mc2 * a1, b1 = mc2 * a0 - mc0 * a2, b2 = mc0 * a1 - mc1 * a0;
-        const double a2p = P.alpha * P.alpha;
+        // llg_integrator.cpp:18-38: scalar alpha, or the cell's entry of alpha_field
+        const double al = P.alpha_f ? P.alpha_f[i] : P.alpha;
+        const double a2p = al * al;
         const double cp = -kGamma / (1. + a2p);
-        const double cd = -P.alpha * kGamma / (1. + a2p);
+        const double cd = -al * kGamma / (1. + a2p);
         double* o = static_cast<double*>(P.out);
         if (P.diss_only) {
             stb(o, o8, P.dt * (cd * b0));
@@ -586,6 +608,9 @@ EpiParams make_params(mafx_ctx* ctx, const EvalArgs& a, int64_t nblocks) {
         e.rk = td.rkky.as<double>();
         e.aex = td.aex.as<double>();
         e.ridx = td.has_ridx ? td.ridx.as<uint32_t>() : nullptr;
+        e.pf = td.pfield.p ? td.pfield.as<double>() : nullptr;
+        e.ax = td.axfield.p ? td.axfield.as<double>() : nullptr;
+        if (e.pf || e.ax) P.has_fields = 1;
         const double Ms = ctx->Ms;
         switch (td.kind) {
         case MAFX_EXCHANGE: e.num = 2. * td.value; e.c0 = (2. * td.value) / (kMu0 * Ms); break;
@@ -606,6 +631,7 @@ EpiParams make_params(mafx_ctx* ctx, const EvalArgs& a, int64_t nblocks) {
     P.out = a.out;
     P.out_f32 = a.out_f32 ? 1 : 0;
     P.alpha = a.alpha;
+    P.alpha_f = a.alpha_field;
     P.dt = a.dt;
     P.diss_only = a.dissipation_only ? 1 : 0;
     P.want_energy = (a.want_energy || a.mode == OutMode::ENERGY) ? 1 : 0;
@@ -681,8 +707,8 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
         // compile-time term programs of the common micromagnetic term lists (field and LLG
         // evaluations); any other list runs the run-time term loop
         if (!red) {
-            uint64_t prog = static_cast<uint64_t>(P.nterms);
-            for (int t = 0; t < P.nterms; ++t) prog |= static_cast<uint64_t>(P.t[t].kind) << (4 + 4 * t);
+            uint64_t prog = P.has_fields ? 0 : static_cast<uint64_t>(P.nterms);
+            for (int t = 0; t < P.nterms && !P.has_fields; ++t) prog |= static_cast<uint64_t>(P.t[t].kind) << (4 + 4 * t);
             constexpr uint64_t kP1 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_DMI, MAFX_UNIAXIAL});
             constexpr uint64_t kP2 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_ZEEMAN});
             constexpr uint64_t kP3 = prog_of({MAFX_DEMAG, MAFX_RKKY, MAFX_DMI, MAFX_UNIAXIAL, MAFX_ZEEMAN});
@@ -726,6 +752,30 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
 
 void evaluate(mafx_ctx* ctx, const EvalArgs& a) {
     require(ctx->terms.size() <= static_cast<size_t>(kMaxTerms), "at most 8 terms per context");
+    // time-dependent external fields (external_field.cpp:28-36): the callback fills the term's
+    // field for this evaluation's time and magnetisation
+    for (size_t k = 0; k < ctx->terms.size(); ++k) {
+        TermDev& td = ctx->terms[k];
+        if (!td.callback || !(a.mask & (1u << k))) continue;
+        int rc;
+        if (td.callback_homogeneous) {
+            double v[3] = {0, 0, 0};
+            rc = td.callback(td.callback_user, ctx->time, a.m, v, ctx->stream);
+            for (int c = 0; c < 3; ++c) td.field[c] = v[c];
+        } else if (td.callback_host) {
+            const size_t bytes = sizeof(double) * 3 * static_cast<size_t>(ctx->N);
+            MAFX_HIP(hipMemcpyAsync(td.cb_m.data(), a.m, bytes, hipMemcpyDeviceToHost, ctx->stream));
+            MAFX_HIP(hipStreamSynchronize(ctx->stream));
+            rc = td.callback(td.callback_user, ctx->time, td.cb_m.data(), td.cb_h.data(), nullptr);
+            if (rc == 0) {
+                MAFX_HIP(hipMemcpyAsync(td.zee.p, td.cb_h.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+                MAFX_HIP(hipStreamSynchronize(ctx->stream));
+            }
+        } else {
+            rc = td.callback(td.callback_user, ctx->time, a.m, td.zee.as<double>(), ctx->stream);
+        }
+        require(rc == 0, "external field callback failed (returned " + std::to_string(rc) + ")");
+    }
     if (ctx->precision == MAFX_F64)
         evaluate_impl<double2, double>(ctx, a);
     else
@@ -909,10 +959,10 @@ void build_tensor(mafx_ctx* ctx) {
     const uint64_t stored = ctx->precision == MAFX_F64
                                 ? sizeof(double2) * 6 * static_cast<uint64_t>(K)
                                 : sizeof(float) * 6 * static_cast<uint64_t>(ctx->n2h * ctx->n1h * hxs);
-    // the FP32 layout from the host-built real-space tensor is stored under its own key
-    const bool host32 = ctx->precision != MAFX_F64 && (ctx->demag_flags & MAFX_TERM_HOST_TENSOR);
-    const TensorCacheKey key{n0, n1, n2, hxs, ctx->precision | (host32 ? kCacheHostTensor : 0), ctx->atom_dipole ? 1 : 0,
-                             ctx->mesh.dx, ctx->mesh.dy, ctx->mesh.dz};
+    // the FP32 layout from the host-built and the FP64 layout from the GPU-built real-space tensor
+    // are stored under keys of their own
+    const TensorCacheKey key{n0, n1, n2, hxs, tensor_cache_layout(ctx->precision, ctx->demag_flags),
+                             ctx->atom_dipole ? 1 : 0, ctx->mesh.dx, ctx->mesh.dy, ctx->mesh.dz};
     if (caching) {
         std::vector<char> host;
         if (tensor_cache_read(key, stored, host, verbose)) {
@@ -938,7 +988,9 @@ void build_tensor_uncached(mafx_ctx* ctx) {
     const int64_t P = n0 * n1 * n2, K = n2 * n1 * hxs;
     DevBuf Nd, T64;
     Nd.alloc(sizeof(double) * 6 * P);
-    if (ctx->precision == MAFX_F64 || (ctx->demag_flags & MAFX_TERM_HOST_TENSOR)) {
+    const bool on_host = ctx->precision == MAFX_F64 ? !(ctx->demag_flags & MAFX_TERM_DEVICE_TENSOR)
+                                                    : (ctx->demag_flags & MAFX_TERM_HOST_TENSOR) != 0;
+    if (on_host) {
         auto Nh = host_tensor(ctx->mesh, ctx->atom_dipole, P);
         MAFX_HIP(hipMemcpy(Nd.p, Nh->data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
     } else {

@@ -68,6 +68,7 @@ namespace {
 struct Tableau {
     int stages;                             // number of f evaluations
     double a[kMaxStages + 1][kMaxStages + 1]; // a[s][j], s = 2..stages
+    double c[kMaxStages + 1];               // stage times t + c[s] h (the reference's tempstate.t)
     double b[kMaxStages + 1];               // weights of the propagated solution (sumbk)
     double e[kMaxStages + 1];               // error weights
     bool rkf_form;                          // rk_error = sumbk - sum e_j k_j
@@ -90,6 +91,7 @@ Tableau dp45() { // adaptive_runge_kutta.cpp:117-128
     t.b[7] = 0.0;
     t.e[1] = 71.0 / 57600.0, t.e[2] = 0.0, t.e[3] = -71.0 / 16695.0, t.e[4] = 71.0 / 1920.0,
     t.e[5] = -17253.0 / 339200.0, t.e[6] = 22.0 / 525.0, t.e[7] = -1.0 / 40.0;
+    t.c[2] = 0.2, t.c[3] = 0.3, t.c[4] = 0.8, t.c[5] = 8.0 / 9.0, t.c[6] = 1, t.c[7] = 1;
     return t;
 }
 
@@ -107,6 +109,7 @@ Tableau rkf45() { // adaptive_runge_kutta.cpp:75-110 (subtractions written as ne
     t.b[6] = 2. / 55.;
     t.e[1] = 25. / 216., t.e[2] = 0.0, t.e[3] = 1408. / 2565., t.e[4] = 2197. / 4104., t.e[5] = -1. / 5.,
     t.e[6] = 0.0;
+    t.c[2] = 1. / 4., t.c[3] = 3. / 8., t.c[4] = 12. / 13., t.c[5] = 1., t.c[6] = 1. / 2.;
     return t;
 }
 
@@ -126,6 +129,7 @@ Tableau bs23() { // adaptive_runge_kutta.cpp:496-529
     t.a[4][1] = 2. / 9., t.a[4][2] = 1. / 3., t.a[4][3] = 4. / 9.;
     t.b[1] = 2. / 9., t.b[2] = 1. / 3., t.b[3] = 4. / 9.;
     t.e[1] = 7. / 24., t.e[2] = 1. / 4., t.e[3] = 1. / 3., t.e[4] = 1. / 8.;
+    t.c[2] = 1. / 2., t.c[3] = 3. / 4., t.c[4] = 1.;
     return t;
 }
 
@@ -146,6 +150,8 @@ Tableau bs45() { // adaptive_runge_kutta.cpp:176-282 (eight stages, the last one
     for (int j = 1; j <= 7; ++j) t.b[j] = t.a[8][j];
     t.e[1] = 2479.0 / 34992.0, t.e[3] = 123.0 / 416.0, t.e[4] = 612941.0 / 3411720.0, t.e[5] = 43.0 / 1440.0,
     t.e[6] = 2272.0 / 6561.0, t.e[7] = 79937.0 / 1113912.0, t.e[8] = 3293.0 / 556956.0;
+    t.c[2] = 1.0 / 6.0, t.c[3] = 2.0 / 9.0, t.c[4] = 3.0 / 7.0, t.c[5] = 2.0 / 3.0, t.c[6] = 3.0 / 4.0, t.c[7] = 1.0,
+    t.c[8] = 1.0;
     return t;
 }
 
@@ -193,6 +199,9 @@ Tableau dp78() { // adaptive_runge_kutta.cpp:285-493 (advances with the order-8 
     t.e[7] = 3.11240900051118327929913751627e-1, t.e[8] = 2.46734519059988698196468570407e0,
     t.e[9] = -2.54694165184190873912738007542e0, t.e[10] = 1.44354858367677524030187495069e0,
     t.e[11] = 7.94155958811272872713019541622e-2, t.e[12] = 4.44444444444444444444444444445e-2;
+    t.c[2] = 5.55555555555555555555555555556e-2, t.c[3] = 8.33333333333333333333333333334e-2, t.c[4] = 1.25e-1,
+    t.c[5] = 3.125e-1, t.c[6] = 3.75e-1, t.c[7] = 1.475e-1, t.c[8] = 4.65e-1, t.c[9] = 5.64865451382259575398358501426e-1,
+    t.c[10] = 6.5e-1, t.c[11] = 9.24656277640504446745013574318e-1, t.c[12] = 1.0, t.c[13] = 1.0;
     return t;
 }
 
@@ -208,19 +217,21 @@ Tableau tableau_of(const std::string& s) {
 
 int scheme_stages(const char* scheme) { return tableau_of(scheme).stages; }
 
-// One attempt of the scheme at step dt from m; returns the error norm, sumbk in llg->sumbk.
-double attempt(mafx_llg* L, const double* m, double dt, const char* scheme) {
+// One attempt of the scheme at step dt from (m, t); returns the error norm, sumbk in llg->sumbk.
+double attempt(mafx_llg* L, const double* m, double t, double dt, const char* scheme) {
     mafx_ctx* ctx = L->ctx;
     const Tableau T = tableau_of(scheme);
     const int64_t len = 3 * ctx->N;
     EvalArgs ea;
     ea.mode = OutMode::LLG;
     ea.alpha = L->alpha;
+    ea.alpha_field = L->has_alpha_field ? L->alpha_field.as<double>() : nullptr;
     ea.dt = dt;
     ea.dissipation_only = L->dissipation_only;
     // stage 1: k1 = dt f(m) (recomputed every attempt: normalize_ is true, adaptive_runge_kutta.cpp:132)
     ea.m = m;
     ea.out = L->k[1].p;
+    ctx->time = t;
     evaluate(ctx, ea);
     L->heff_evals++;
     for (int s = 2; s <= T.stages; ++s) {
@@ -233,6 +244,7 @@ double attempt(mafx_llg* L, const double* m, double dt, const char* scheme) {
         lincomb(ctx, L->mtmp.as<double>(), m, s - 1, v, c, len);
         ea.m = L->mtmp.as<double>();
         ea.out = L->k[s].p;
+        ctx->time = t + T.c[s] * dt;
         evaluate(ctx, ea);
         L->heff_evals++;
     }
@@ -245,10 +257,11 @@ double attempt(mafx_llg* L, const double* m, double dt, const char* scheme) {
 void llg_step(mafx_llg* L, double* m, double* t) {
     double err = 0;
     do {
-        err = attempt(L, m, L->h, L->scheme.c_str());
+        err = attempt(L, m, *t, L->h, L->scheme.c_str());
         if (!std::isfinite(err)) throw Error(MAFX_ERR_NUMERIC, "non-finite RK error norm");
     } while (!L->ctrl.success(err, L->h));
     *t += L->h;
+    L->ctx->time = *t;
     L->h_last = L->h;
     L->err_last = err;
     L->h = L->ctrl.hnext;
@@ -282,10 +295,12 @@ int64_t llg_integrate_dense(mafx_llg* L, double* m, double* t, double time_in_s,
         llg_step(L, m, t);
         if (*t >= t_next) {
             const double back = t_next - *t;
-            attempt(L, m, back, "RKF45");
+            attempt(L, m, *t, back, "RKF45");
+            ctx->time = *t;
             if (t_next >= tf - 1e-20) {
                 accept(ctx, m, L->sumbk.as<double>(), false, false);
                 *t += back;
+                ctx->time = *t;
                 emit(m, *t);
                 break;
             }

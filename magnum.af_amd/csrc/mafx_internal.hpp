@@ -100,14 +100,24 @@ struct FftPlan {
 // ------------------------------------------------------------------ terms
 struct TermDev {
     int kind{0};
+    int flags{0};
     double value{0};
     double axis[3]{0, 0, 1};
     double field[3]{0, 0, 0};
-    DevBuf zee;       // 3N f64 when the external field is an array
+    DevBuf zee;       // 3N f64 when the external field is an array or a full-field callback
     DevBuf rkky, aex; // N f64
     DevBuf ridx;      // N u32 (optional)
+    DevBuf pfield;    // N f64: D_constants (DMI) or Ku1_field (UNIAXIAL)
+    DevBuf axfield;   // 3N f64: normalised Ku1_axis_field (UNIAXIAL)
     bool has_zee_array{false}, has_ridx{false};
+    mafx_field_fn callback{nullptr}; // time-dependent external field
+    void* callback_user{nullptr};
+    bool callback_homogeneous{false};
+    bool callback_host{false};        // m and the field pass through the pinned host buffers below
+    std::vector<double> cb_m, cb_h;   // host staging of MAFX_TERM_HOST_CALLBACK (3N each)
 };
+// Uploads desc's parameters into t (construction and mafx_ctx_update_term).
+void set_term(mafx_ctx* ctx, TermDev& t, const mafx_term_desc& desc);
 
 struct Workspace; // precision-specific demag buffers
 
@@ -117,6 +127,7 @@ struct mafx_ctx {
     mafx_mesh mesh{};
     int64_t N{0};
     double Ms{0};
+    double time{0};           // State::t of the next evaluation (time-dependent external fields)
     bool has_ms_field{false};
     mafx::DevBuf ms_field; // N f64
     int precision{MAFX_F64};
@@ -175,6 +186,7 @@ struct EvalArgs {
     void* out{nullptr};
     bool out_f32{false};
     double alpha{0}, dt{0};
+    const double* alpha_field{nullptr}; // N f64 cell-wise damping, or null for `alpha`
     bool dissipation_only{false};
     bool want_energy{false};
 };

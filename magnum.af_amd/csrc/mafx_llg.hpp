@@ -33,10 +33,13 @@ struct mafx_llg {
     int64_t heff_evals{0}, steps{0};
     mafx::DevBuf k[14]; // k[1..stages], stages <= 13 (DP78)
     mafx::DevBuf mtmp, mtmp2, sumbk;
+    mafx::DevBuf alpha_field; // N f64 (LLGIntegrator(af::array alpha_field, ...)), used when has_alpha_field
+    bool has_alpha_field{false};
 };
 
 namespace mafx {
-double attempt(mafx_llg* L, const double* m, double dt, const char* scheme);
+// One attempt from (m, t) with step dt; stage i evaluates the field at time t + c_i dt.
+double attempt(mafx_llg* L, const double* m, double t, double dt, const char* scheme);
 // f evaluations per attempt of a scheme ("RKF45", "DP45", "BS23", "BS45", "DP78")
 int scheme_stages(const char* scheme);
 void llg_step(mafx_llg* L, double* m, double* t);

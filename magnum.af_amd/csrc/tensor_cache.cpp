@@ -2,6 +2,8 @@
 // and the cache branch of get_Nfft (demag_field.cpp:185-204).
 #include "tensor_cache.hpp"
 
+#include "../../include/mafx.h"
+
 #include <algorithm>
 #include <cinttypes>
 #include <cstdio>
@@ -83,12 +85,19 @@ fs::path tensor_cache_dir() {
     return t / "magnumaf.cache";
 }
 
+int32_t tensor_cache_layout(int precision, int term_flags) {
+    if (precision == MAFX_F64) return (term_flags & MAFX_TERM_DEVICE_TENSOR) ? kCacheDeviceTensor : 0;
+    return precision | ((term_flags & MAFX_TERM_HOST_TENSOR) ? kCacheHostTensor : 0);
+}
+
 std::string tensor_cache_name(const TensorCacheKey& k) {
     // demag_field.cpp:185-190: "Nfft_n0exp_<n0>_n1exp_<n1>_n2exp_<n2>_dx_<1e9 dx>_dy_.._dz_.."
     std::string s = "Nfft_n0exp_" + std::to_string(k.n0) + "_n1exp_" + std::to_string(k.n1) + "_n2exp_" +
                     std::to_string(k.n2) + "_dx_" + std::to_string(1e9 * k.dx) + "_dy_" + std::to_string(1e9 * k.dy) +
                     "_dz_" + std::to_string(1e9 * k.dz);
-    s += k.precision == 0 ? "_mafx_c64" : (k.precision & kCacheHostTensor) ? "_mafx_f32sym_host" : "_mafx_f32sym";
+    const int32_t layout = k.precision & 0xff;
+    s += layout == 0 ? ((k.precision & kCacheDeviceTensor) ? "_mafx_c64_dev" : "_mafx_c64")
+                     : ((k.precision & kCacheHostTensor) ? "_mafx_f32sym_host" : "_mafx_f32sym");
     if (k.dipole) s += "_dipole";
     s += "_" + hex_bits(k.dx) + hex_bits(k.dy) + hex_bits(k.dz);
     return s;

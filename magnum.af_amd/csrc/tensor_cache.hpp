@@ -13,15 +13,20 @@
 
 namespace mafx {
 
-constexpr int32_t kCacheHostTensor = 0x100;
+constexpr int32_t kCacheHostTensor = 0x100;   // FP32 layout from the host-built real-space tensor
+constexpr int32_t kCacheDeviceTensor = 0x200; // FP64 layout from the GPU-built real-space tensor
 
 struct TensorCacheKey {
     int64_t n0, n1, n2, hxs; // expanded lengths and the padded row stride of the stored planes
     int32_t precision;       // MAFX_F64 (complex half spectrum) or MAFX_F32_FIELD (real quarter),
-                             // | kCacheHostTensor when the FP32 tensor comes from the host tensor
+                             // | kCacheHostTensor when the FP32 tensor comes from the host tensor,
+                             // | kCacheDeviceTensor when the FP64 tensor comes from the GPU tensor
     int32_t dipole;          // 1 for the atomistic point-dipole tensor
     double dx, dy, dz;
 };
+
+// The key's layout field of a context of this precision and DEMAG term flags.
+int32_t tensor_cache_layout(int precision, int term_flags);
 
 // $MAFX_CACHE_DIR, else std::filesystem::temp_directory_path() / "magnumaf.cache"
 std::filesystem::path tensor_cache_dir();

@@ -58,6 +58,11 @@ def _scalar_flat(s, mesh):
     return np.ascontiguousarray(s.ravel())
 
 
+def from_scalar_flat(f, mesh):
+    """flat [nz][ny][nx] -> (nx, ny, nz)."""
+    return np.ascontiguousarray(np.asarray(f).reshape(mesh.nz, mesh.ny, mesh.nx).transpose(2, 1, 0))
+
+
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
@@ -81,22 +86,26 @@ class Mesh:
 
 
 class Context:
-    """One libmafx evaluation context: mesh, material, an ordered term list, precision."""
+    """One libmafx evaluation context: mesh, material, an ordered term list, precision.
 
-    def __init__(self, mesh, Ms, Ms_field, terms, precision="f64"):
+    The context remembers the version of every term and of the state's material it was built
+    from; refresh() uploads what changed since (mafx_ctx_update_term, mafx_ctx_set_material), so
+    a term or material changed between evaluations is seen by the next one, as in the reference,
+    whose terms and State are read at every evaluation."""
+
+    def __init__(self, mesh, Ms, Ms_field, terms, precision="f64", material_version=0):
         self.mesh = mesh
         self.terms = list(terms)
-        descs, self._keep = [], []
+        descs, self._keep_terms = [], []
         for t in self.terms:
             d, keep = t._desc(mesh)
             descs.append(d)
-            self._keep.extend(keep)
+            self._keep_terms.append((d, keep))
+        self._versions = [t._version for t in self.terms]
         arr = (_abi.TermDesc * len(descs))(*descs)
-        self._keep.append(arr)
         msf = None
         if Ms_field is not None:
             msf = np.ascontiguousarray(Ms_field, dtype=np.float64)
-            self._keep.append(msf)
         prec = _abi.MAFX_F64 if precision in ("f64", _abi.MAFX_F64) else _abi.MAFX_F32_FIELD
         self.precision = prec
         h = C.c_void_p()
@@ -105,7 +114,27 @@ class Context:
         check(lib().mafx_ctx_create(C.byref(mesh._c()), float(Ms) if Ms is not None else 0.0,
                                     _dp(msf) if msf is not None else None, arr, len(descs), prec, C.byref(h)))
         self.h = h
+        self.material_version = material_version
         check(lib().mafx_ctx_set_stream(self.h, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    def refresh(self, state):
+        """Uploads the parameters of terms changed since the last call, the state's material when it
+        changed, and the state's time (read by time-dependent external fields)."""
+        for i, t in enumerate(self.terms):
+            if t._version != self._versions[i]:
+                d, keep = t._desc(self.mesh)
+                check(lib().mafx_ctx_update_term(self.h, i, C.byref(d)))
+                self._keep_terms[i] = (d, keep)
+                self._versions[i] = t._version
+        if state is not None:
+            if state._material_version != self.material_version:
+                msf = state.Ms_field
+                if msf is not None:
+                    msf = np.ascontiguousarray(msf, dtype=np.float64)
+                Ms = state._Ms if msf is None else 0.0
+                check(lib().mafx_ctx_set_material(self.h, float(Ms), _dp(msf) if msf is not None else None))
+                self.material_version = state._material_version
+            check(lib().mafx_ctx_set_time(self.h, float(state.t)))
 
     def __del__(self, _finalizing=sys.is_finalizing):
         # at interpreter exit the process releases the device memory itself
@@ -155,40 +184,72 @@ class Context:
 
 
 class State:
-    """State(mesh, Ms, m) (state.cpp:91-111, magnumaf.pyx:694-878).
+    """State(mesh, Ms, m, verbose=True, mute_warning=False) (state.cpp:91-111, magnumaf.pyx:694-878).
 
     m is normalised with zero vectors kept at zero; when Ms is a float and some |m| == 0, an
-    Ms_field = Ms * (|m| != 0) is built as the reference does (state.cpp:71-89)."""
+    Ms_field = Ms * (|m| != 0) is built as the reference does (state.cpp:71-89).  Ms may be a
+    float or a cell-wise field (nx, ny, nz) / (nx, ny, nz, 1); assigning state.Ms or state.Ms_field
+    later changes the material of every following evaluation (magnumaf.pyx:777-784,839-842)."""
 
-    def __init__(self, mesh, Ms, m, verbose=False, mute_warning=False, precision="f64"):
+    def __init__(self, mesh, Ms, m, verbose=True, mute_warning=False, precision="f64"):
         torch = _torch()
         self.mesh = mesh
         self.t = 0.0
         self.steps = 0
+        self.verbose = verbose
         mf = to_flat(m) if np.ndim(m) == 4 else np.array(m, dtype=np.float64).ravel()
         N = mesh.n
         a, b, c = mf[:N], mf[N:2 * N], mf[2 * N:]
         nrm = np.sqrt(a * a + b * b + c * c)
         with np.errstate(invalid="ignore", divide="ignore"):
             mf = np.where(np.concatenate([nrm, nrm, nrm]) != 0, mf / np.concatenate([nrm, nrm, nrm]), 0.0)
+        self._material_version = 0
         if np.ndim(Ms) == 0:
-            self.Ms = float(Ms)
-            self.Ms_field = None
+            self._Ms = float(Ms)
+            self._Ms_field = None
             if np.min(nrm) == 0:
-                self.Ms_field = self.Ms * (nrm != 0).astype(np.float64)
+                if verbose:
+                    print("Info: in state.cpp: initial m has values with zero norm, building Ms_field array")
+                self._Ms_field = self._Ms * (nrm != 0).astype(np.float64)
         else:
-            self.Ms = float("nan")
-            self.Ms_field = _scalar_flat(Ms, mesh)
+            self._Ms = float("nan")
+            self._Ms_field = _scalar_flat(Ms, mesh)
         self._m = torch.from_numpy(mf).to("cuda")
         self._ctx = {}
+
+    # -- material: State::Ms, State::Ms_field (magnumaf.pyx:777-784,839-842)
+    @property
+    def Ms(self):
+        """Ms as the reference's Python returns it: the float, or the field when one is set."""
+        return self._Ms if self._Ms_field is None else from_scalar_flat(self._Ms_field, self.mesh)
+
+    @Ms.setter
+    def Ms(self, value):
+        if np.ndim(value) == 0:
+            self._Ms = float(value)
+        else:
+            self._Ms_field = _scalar_flat(value, self.mesh)
+        self._material_version += 1
+
+    @property
+    def Ms_field(self):
+        """The cell-wise Ms (flat [N], x fastest) or None."""
+        return self._Ms_field
+
+    @Ms_field.setter
+    def Ms_field(self, value):
+        self._Ms_field = None if value is None else _scalar_flat(value, self.mesh)
+        self._material_version += 1
 
     # -- material key for context reuse
     def _context(self, terms, precision="f64"):
         key = (tuple(id(t) for t in terms), precision)
         ctx = self._ctx.get(key)
         if ctx is None:
-            ctx = Context(self.mesh, self.Ms, self.Ms_field, terms, precision)
+            ctx = Context(self.mesh, self._Ms if self._Ms_field is None else 0.0, self._Ms_field, terms, precision,
+                          self._material_version)
             self._ctx[key] = ctx
+        ctx.refresh(self)
         return ctx
 
     @property
@@ -217,8 +278,8 @@ class State:
     def _util_ctx(self):
         # a context with one zero external field serves the vector helpers
         if "_util" not in self._ctx:
-            self._ctx["_util"] = Context(self.mesh, self.Ms if self.Ms_field is None else 0.0, self.Ms_field,
-                                         [ExternalField(homogeneous=(0.0, 0.0, 0.0))])
+            self._ctx["_util"] = Context(self.mesh, self._Ms if self._Ms_field is None else 0.0, self._Ms_field,
+                                         [ExternalField(homogeneous=(0.0, 0.0, 0.0))], "f64", self._material_version)
         return self._ctx["_util"]
 
     def normalize(self):
@@ -241,7 +302,12 @@ class State:
         return self.mean_m(2)
 
     def __str__(self):
-        """'t<TAB><mx><TAB><my><TAB><mz>' as the reference's operator<< prints it (state.cpp:26-30)."""
+        """'t <mx> <my> <mz>' as the reference's Python State prints it (magnumaf.pyx:754-757)."""
+        mx, my, mz = self.mean_m()
+        return str(self.t) + " " + str(mx) + " " + str(my) + " " + str(mz)
+
+    def stream_line(self):
+        """'t<TAB><mx><TAB><my><TAB><mz>' as the reference's C++ operator<< prints it (state.cpp:26-30)."""
         from . import io
         return io.stream_line(self.t, self.mean_m())
 
@@ -263,7 +329,17 @@ class State:
 
 # ---------------------------------------------------------------- field terms
 class FieldTerm:
-    """Base of the field terms (field_term.hpp:12-51): H_in_Apm(state), Energy_in_J(state)."""
+    """Base of the field terms (field_term.hpp:12-51): H_in_Apm(state), Energy_in_J(state).
+
+    Assigning any public attribute of a term bumps its version; contexts holding the term upload
+    the new parameters before their next evaluation (Context.refresh)."""
+
+    _version = 0
+
+    def __setattr__(self, name, value):
+        object.__setattr__(self, name, value)
+        if not name.startswith("_"):
+            object.__setattr__(self, "_version", self._version + 1)
 
     def _desc(self, mesh):
         raise NotImplementedError
@@ -293,24 +369,31 @@ def _td(kind, value=0.0, axis=(0.0, 0.0, 1.0), field=(0.0, 0.0, 0.0)):
     return d
 
 
+def _is_array(v):
+    return not isinstance(v, (int, float)) and np.ndim(v) > 0 and np.size(v) > 3
+
+
 class DemagField(FieldTerm):
     """DemagField(mesh, verbose=False, caching=False, nthreads=4) (demag_field.hpp:14-30,
     magnumaf.pyx:1086-1121).  caching reads the transformed tensor from the on-disk cache
     (MAFX_CACHE_DIR, else <tmp>/magnumaf.cache) and writes it there when absent; verbose reports
     the cache activity.  nthreads is accepted for the reference's signature (the host tensor uses
-    every hardware thread).  host_tensor (keyword only, not in the reference): the FP32-field path
-    builds its real-space tensor on the host in the reference's operation order, as the FP64 path
-    does, instead of on the GPU."""
+    every hardware thread).  Keyword only, not in the reference: host_tensor makes the FP32-field
+    path build its real-space tensor on the host in the reference's operation order, as the FP64
+    path does, instead of on the GPU; device_tensor makes the FP64 path use the FP32 path's
+    GPU-built tensor instead (to compare the two paths on one tensor)."""
 
-    def __init__(self, mesh, verbose=False, caching=False, nthreads=4, *, host_tensor=False):
+    def __init__(self, mesh, verbose=False, caching=False, nthreads=4, *, host_tensor=False, device_tensor=False):
         self.mesh = mesh
         self.verbose, self.caching, self.nthreads = bool(verbose), bool(caching), nthreads
         self.host_tensor = bool(host_tensor)
+        self.device_tensor = bool(device_tensor)
 
     def _desc(self, mesh):
         d = _td(_abi.DEMAG)
         d.flags = ((_abi.TERM_CACHING if self.caching else 0) | (_abi.TERM_VERBOSE if self.verbose else 0)
-                   | (_abi.TERM_HOST_TENSOR if self.host_tensor else 0))
+                   | (_abi.TERM_HOST_TENSOR if self.host_tensor else 0)
+                   | (_abi.TERM_DEVICE_TENSOR if self.device_tensor else 0))
         return d, []
 
 
@@ -325,42 +408,107 @@ class ExchangeField(FieldTerm):
 
 
 class DmiField(FieldTerm):
-    """DmiField(D, D_axis=[0, 0, 1]) interface DMI in J/m^2 (dmi_field.hpp:8-23)."""
+    """DmiField(D, D_axis=[0, 0, 1]) interface DMI (dmi_field.hpp:8-23, magnumaf.pyx:1324-1344): D in
+    J/m^2, a float or a cell-wise field (nx, ny, nz) / (nx, ny, nz, 1) (dmi_field.cpp:17-19,97-109)."""
 
     def __init__(self, D, D_axis=(0.0, 0.0, 1.0)):
-        self.D = float(D)
+        self.D = np.asarray(D, dtype=np.float64) if _is_array(D) else float(D)
         self.D_axis = list(D_axis)
 
     def _desc(self, mesh):
-        return _td(_abi.DMI, self.D, self.D_axis), []
+        if isinstance(self.D, float):
+            return _td(_abi.DMI, self.D, self.D_axis), []
+        f = _scalar_flat(self.D, mesh)
+        d = _td(_abi.DMI, 0.0, self.D_axis)
+        d.array0 = _dp(f)
+        return d, [f]
 
 
 class UniaxialAnisotropyField(FieldTerm):
-    """UniaxialAnisotropyField(Ku1, Ku1_axis=[0, 0, 1]) in J/m^3 (uniaxial_anisotropy_field.hpp)."""
+    """UniaxialAnisotropyField(Ku1, Ku1_axis=[0, 0, 1]) (uniaxial_anisotropy_field.hpp:8-35,
+    magnumaf.pyx:1493-1503): Ku1 in J/m^3, a float or a cell-wise field (nx, ny, nz) / (nx, ny, nz, 1);
+    Ku1_axis a 3-vector or a cell-wise axis field (nx, ny, nz, 3), normalised per cell
+    (uniaxial_anisotropy_field.cpp:11-25,58-89)."""
 
     def __init__(self, Ku1, Ku1_axis=(0.0, 0.0, 1.0)):
-        self.Ku1 = float(Ku1)
-        n = math.sqrt(sum(float(v) ** 2 for v in Ku1_axis))
-        self.Ku1_axis = [float(v) / n for v in Ku1_axis]
+        self.Ku1 = np.asarray(Ku1, dtype=np.float64) if _is_array(Ku1) else float(Ku1)
+        if _is_array(Ku1_axis):
+            self.Ku1_axis = np.asarray(Ku1_axis, dtype=np.float64)
+        else:
+            n = math.sqrt(sum(float(v) ** 2 for v in Ku1_axis))
+            self.Ku1_axis = [float(v) / n for v in Ku1_axis]
+
+    @property
+    def Ku1_field(self):
+        return None if isinstance(self.Ku1, float) else self.Ku1
 
     def _desc(self, mesh):
-        return _td(_abi.UNIAXIAL, self.Ku1, self.Ku1_axis), []
+        keep = []
+        scalar_axis = not isinstance(self.Ku1_axis, np.ndarray)
+        d = _td(_abi.UNIAXIAL, self.Ku1 if isinstance(self.Ku1, float) else 0.0,
+                self.Ku1_axis if scalar_axis else (0.0, 0.0, 1.0))
+        if not isinstance(self.Ku1, float):
+            k = self.Ku1
+            if k.ndim == 4 and k.shape[3] == 3:
+                if not (np.array_equal(k[..., 0], k[..., 1]) and np.array_equal(k[..., 0], k[..., 2])):
+                    raise ValueError("UniaxialAnisotropyField: a Ku1 field of shape (nx, ny, nz, 3) must hold equal "
+                                     "components (the legacy tiled form); pass (nx, ny, nz)")
+                k = k[..., 0]
+            f = _scalar_flat(k, mesh)
+            d.array0 = _dp(f)
+            keep.append(f)
+        if not scalar_axis:
+            a = self.Ku1_axis
+            if a.shape[:3] != (mesh.nx, mesh.ny, mesh.nz):
+                a = np.broadcast_to(a, (mesh.nx, mesh.ny, mesh.nz, 3))
+            f = to_flat(a)
+            d.array1 = _dp(f)
+            keep.append(f)
+        return d, keep
+
+
+class _CallbackState:
+    """What a field callback sees of the State: t, mesh and (for full-field callbacks) m."""
+
+    def __init__(self, t, mesh, m_flat):
+        self.t = t
+        self.mesh = mesh
+        self._m = m_flat
+
+    @property
+    def m(self):
+        if self._m is None:
+            raise AttributeError("homogeneous field callbacks do not receive m")
+        return from_flat(self._m, self.mesh.nx, self.mesh.ny, self.mesh.nz)
 
 
 class ExternalField(FieldTerm):
-    """ExternalField(array (nx, ny, nz, 3) in A/m) or ExternalField(homogeneous=(hx, hy, hz))."""
+    """ExternalField(array (nx, ny, nz, 3) in A/m), ExternalField(homogeneous=(hx, hy, hz)), or a
+    time-dependent field ExternalField(func) with func(state) -> (nx, ny, nz, 3) array
+    (external_field.hpp:11-13, external_field.cpp:28-36: evaluated at every field evaluation, at
+    the stage times of the integrator).  With homogeneous_callback=True func(state) returns one
+    (hx, hy, hz) and the state it receives carries t and mesh only.  set_homogeneous_field changes
+    the field between evaluations (external_field.cpp:21-26)."""
 
     kind = _abi.ZEEMAN
 
-    def __init__(self, array_in=None, homogeneous=None):
+    def __init__(self, array_in=None, homogeneous=None, homogeneous_callback=False):
+        self.callback = None
+        self.homogeneous_callback = bool(homogeneous_callback)
+        if callable(array_in):
+            self.callback = array_in
+            array_in = None
         self.array = None if array_in is None else np.asarray(array_in, dtype=np.float64)
         self.homogeneous = None if homogeneous is None else [float(v) for v in homogeneous]
 
     def set_homogeneous_field(self, x, y, z):
         self.homogeneous = [float(x), float(y), float(z)]
         self.array = None
+        self.callback = None
 
     def _desc(self, mesh):
+        if self.callback is not None:
+            return self._callback_desc(mesh)
         if self.array is not None:
             a = self.array
             if a.shape[:3] != (mesh.nx, mesh.ny, mesh.nz):
@@ -370,6 +518,33 @@ class ExternalField(FieldTerm):
             d.array0 = _dp(flat)
             return d, [flat]
         return _td(self.kind, field=self.homogeneous or (0, 0, 0)), []
+
+    def _callback_desc(self, mesh):
+        func, hom, n3 = self.callback, self.homogeneous_callback, 3 * mesh.n
+
+        def fn(user, t, m_ptr, h_ptr, stream):
+            try:
+                if hom:
+                    v = np.asarray(func(_CallbackState(t, mesh, None)), dtype=np.float64).ravel()
+                    if v.size != 3:
+                        return 2
+                    C.memmove(h_ptr, v.ctypes.data, 24)
+                else:
+                    m = np.ctypeslib.as_array(C.cast(m_ptr, C.POINTER(C.c_double)), shape=(n3,)).copy()
+                    v = np.asarray(func(_CallbackState(t, mesh, m)), dtype=np.float64)
+                    if v.ndim == 4 or v.shape[:1] != (n3,):
+                        v = to_flat(np.broadcast_to(v, (mesh.nx, mesh.ny, mesh.nz, 3)))
+                    C.memmove(h_ptr, np.ascontiguousarray(v).ctypes.data, 8 * n3)
+                return 0
+            except Exception as e:  # noqa: BLE001 - a failed evaluation, reported through the status
+                sys.stderr.write(f"ExternalField callback raised: {e!r}\n")
+                return 1
+
+        f = _abi.FIELD_FN(fn)
+        d = _td(self.kind)
+        d.flags = _abi.TERM_HOMOGENEOUS_CALLBACK if hom else _abi.TERM_HOST_CALLBACK
+        d.callback = f
+        return d, [f]
 
 
 class RKKYExchangeField(FieldTerm):
@@ -441,11 +616,19 @@ class LLGIntegrator:
     dissipation_term_only=False) (magnumaf.pyx:912-1033, llg_integrator.hpp:26-82).  mode is one of
     "RKF45", "DP45", "BS45", "DP78", "BS23".
 
+    alpha is a float or a cell-wise damping field (nx, ny, nz) / (nx, ny, nz, 1)
+    (LLGIntegrator(af::array alpha_field, ...), llg_integrator.hpp:32-40, magnumaf.pyx:962-967).
+
     precision="f32" selects the FP32-field path (demag FFT and stencils in FP32, state and RK in FP64)."""
 
     def __init__(self, alpha, terms, mode="RKF45", hmin=1e-15, hmax=3.5e-10, atol=1e-6, rtol=1e-6,
                  dissipation_term_only=False, precision="f64"):
-        self._alpha = float(alpha)
+        self._alpha_field = None
+        if _is_array(alpha):
+            self._alpha_field = np.asarray(alpha, dtype=np.float64)
+            self._alpha = 0.0
+        else:
+            self._alpha = float(alpha)
         self.terms = list(terms)
         self.mode = mode
         self.hmin, self.hmax, self.atol, self.rtol = hmin, hmax, atol, rtol
@@ -468,8 +651,14 @@ class LLGIntegrator:
         check(lib().mafx_llg_create(ctx.h, self._alpha, self.mode.encode(), self.hmin, self.hmax, self.atol,
                                     self.rtol, int(self.dissipation_term_only), C.byref(h)))
         self._llg, self._ctx = h, ctx
+        if self._alpha_field is not None:
+            self._upload_alpha_field()
         if self._ctrl is not None:
             check(lib().mafx_llg_set_controller_state(self._llg, *self._ctrl))
+
+    def _upload_alpha_field(self):
+        f = _scalar_flat(self._alpha_field, self._ctx.mesh)
+        check(lib().mafx_llg_set_alpha_field(self._llg, _dp(f)))
 
     def __del__(self, _finalizing=sys.is_finalizing):
         # at interpreter exit the process releases the device memory itself
@@ -479,12 +668,20 @@ class LLGIntegrator:
 
     @property
     def alpha(self):
-        return self._alpha
+        """The scalar alpha, or the alpha field when the integrator was given one."""
+        return self._alpha if self._alpha_field is None else self._alpha_field
 
     @alpha.setter
     def alpha(self, v):
+        if _is_array(v):
+            self._alpha_field = np.asarray(v, dtype=np.float64)
+            if self._llg is not None:
+                self._upload_alpha_field()
+            return
         self._alpha = float(v)
+        self._alpha_field = None
         if self._llg is not None:
+            check(lib().mafx_llg_set_alpha_field(self._llg, None))
             check(lib().mafx_llg_set_alpha(self._llg, self._alpha))
 
     def controller_state(self):
@@ -526,7 +723,7 @@ class LLGIntegrator:
         t = C.c_double(state.t)
         n = C.c_int64()
         check(lib().mafx_llg_relax(self._llg, C.c_void_p(state.m_dev.data_ptr()), C.byref(t), float(precision),
-                                   int(ncalcE), int(nprint), int(bool(verbose)), C.byref(n)))
+                                   int(ncalcE), int(nprint), int(bool(verbose)), int(state.steps), C.byref(n)))
         state.t = t.value
         state.steps += n.value
         self.accumulated_steps += n.value

@@ -8,14 +8,17 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libmafx.so")
 
 MAFX_F64, MAFX_F32_FIELD = 0, 1
 DEMAG, EXCHANGE, DMI, UNIAXIAL, ZEEMAN, RKKY = 1, 2, 3, 4, 5, 6
-TERM_CACHING, TERM_VERBOSE, TERM_HOST_TENSOR = 1, 2, 4  # mafx_term_desc.flags (include/mafx.h)
+# mafx_term_desc.flags (include/mafx.h)
+TERM_CACHING, TERM_VERBOSE, TERM_HOST_TENSOR, TERM_HOMOGENEOUS_CALLBACK, TERM_HOST_CALLBACK = 1, 2, 4, 8, 16
+TERM_DEVICE_TENSOR = 32
 ATOM_DIPOLE, ATOM_EXCHANGE, ATOM_DMI, ATOM_UNIAXIAL, ATOM_ZEEMAN = 11, 12, 13, 14, 15
 
 # every entry point include/mafx.h declares
 EXPORTS = [
     "mafx_last_error", "mafx_device_count", "mafx_demag_calc_N", "mafx_demag_cache_name", "mafx_ctx_tensor_source",
     "mafx_ctx_create", "mafx_ctx_destroy",
-    "mafx_ctx_set_stream", "mafx_ctx_sync", "mafx_ctx_device_bytes", "mafx_heff", "mafx_energy", "mafx_llg_rhs",
+    "mafx_ctx_set_stream", "mafx_ctx_sync", "mafx_ctx_device_bytes", "mafx_ctx_set_time", "mafx_ctx_update_term",
+    "mafx_ctx_set_material", "mafx_llg_set_alpha_field", "mafx_heff", "mafx_energy", "mafx_llg_rhs",
     "mafx_dot", "mafx_max_abs", "mafx_normalize", "mafx_mean_m", "mafx_llg_create", "mafx_llg_destroy",
     "mafx_llg_set_alpha", "mafx_llg_step", "mafx_llg_run_until", "mafx_llg_relax", "mafx_llg_integrate_dense", "mafx_llg_get_stats",
     "mafx_llg_set_controller_state", "mafx_llg_get_controller_state", "mafx_lbfgs_minimize", "mafx_prof_enable",
@@ -30,11 +33,15 @@ class Mesh(C.Structure):
                 ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double)]
 
 
+# int (*)(void* user, double t, const double* m, double* h_out, void* stream)
+FIELD_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p)
+
+
 class TermDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("flags", C.c_int32), ("value", C.c_double),
                 ("axis", C.c_double * 3), ("field", C.c_double * 3),
                 ("array0", C.POINTER(C.c_double)), ("array1", C.POINTER(C.c_double)),
-                ("index0", C.POINTER(C.c_uint32))]
+                ("index0", C.POINTER(C.c_uint32)), ("callback", FIELD_FN), ("callback_user", C.c_void_p)]
 
 
 class LlgStats(C.Structure):
@@ -63,7 +70,7 @@ def lib():
     L.mafx_last_error.restype = C.c_char_p
     L.mafx_device_count.argtypes = [C.POINTER(C.c_int)]
     L.mafx_demag_calc_N.argtypes = [C.POINTER(Mesh), dp, C.c_int]
-    L.mafx_demag_cache_name.argtypes = [C.POINTER(Mesh), C.c_int, C.c_int, C.c_char_p, i64]
+    L.mafx_demag_cache_name.argtypes = [C.POINTER(Mesh), C.c_int, C.c_int, C.c_int, C.c_char_p, i64]
     L.mafx_ctx_tensor_source.argtypes = [vp, C.POINTER(C.c_int)]
     L.mafx_ctx_create.argtypes = [C.POINTER(Mesh), C.c_double, dp, C.POINTER(TermDesc), C.c_int, C.c_int,
                                   C.POINTER(vp)]
@@ -71,6 +78,10 @@ def lib():
     L.mafx_ctx_set_stream.argtypes = [vp, vp]
     L.mafx_ctx_sync.argtypes = [vp]
     L.mafx_ctx_device_bytes.argtypes = [vp, C.POINTER(i64)]
+    L.mafx_ctx_set_time.argtypes = [vp, C.c_double]
+    L.mafx_ctx_update_term.argtypes = [vp, C.c_int, C.POINTER(TermDesc)]
+    L.mafx_ctx_set_material.argtypes = [vp, C.c_double, dp]
+    L.mafx_llg_set_alpha_field.argtypes = [vp, dp]
     L.mafx_heff.argtypes = [vp, vp, u32, vp, C.c_int]
     L.mafx_energy.argtypes = [vp, vp, u32, dp, dp]
     L.mafx_llg_rhs.argtypes = [vp, C.c_double, C.c_double, vp, vp, vp]
@@ -84,7 +95,7 @@ def lib():
     L.mafx_llg_set_alpha.argtypes = [vp, C.c_double]
     L.mafx_llg_step.argtypes = [vp, vp, dp]
     L.mafx_llg_run_until.argtypes = [vp, vp, dp, C.c_double, i64, C.POINTER(i64)]
-    L.mafx_llg_relax.argtypes = [vp, vp, dp, C.c_double, i64, i64, C.c_int, C.POINTER(i64)]
+    L.mafx_llg_relax.argtypes = [vp, vp, dp, C.c_double, i64, i64, C.c_int, i64, C.POINTER(i64)]
     L.mafx_llg_integrate_dense.argtypes = [vp, vp, dp, C.c_double, C.c_double, dp, i64, C.POINTER(i64)]
     L.mafx_llg_get_stats.argtypes = [vp, C.POINTER(LlgStats)]
     L.mafx_llg_set_controller_state.argtypes = [vp, C.c_double, C.c_double, C.c_int]

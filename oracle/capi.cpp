@@ -34,9 +34,15 @@ std::unique_ptr<Term> build_term(const Mesh& g, const mafx_term_desc& d, int nth
     switch (d.kind) {
     case MAFX_DEMAG: return make_demag(g, nthreads);
     case MAFX_EXCHANGE: return make_exchange(d.value);
-    case MAFX_DMI: return make_dmi(d.value, d.axis);
-    case MAFX_UNIAXIAL: return make_uniaxial(d.value, d.axis);
-    case MAFX_ZEEMAN: return make_zeeman(hext());
+    case MAFX_DMI: return make_dmi(d.value, d.axis, d.array0 ? Vec(d.array0, d.array0 + N) : Vec{});
+    case MAFX_UNIAXIAL:
+        return make_uniaxial(d.value, d.axis, d.array0 ? Vec(d.array0, d.array0 + N) : Vec{},
+                             d.array1 ? Vec(d.array1, d.array1 + 3 * N) : Vec{});
+    case MAFX_ZEEMAN:
+        if (d.callback)
+            return make_zeeman_callback(d.callback, d.callback_user, (d.flags & MAFX_TERM_HOMOGENEOUS_CALLBACK) != 0,
+                                        false);
+        return make_zeeman(hext());
     case MAFX_RKKY: {
         Vec rk(d.array0, d.array0 + N), A(d.array1, d.array1 + N);
         std::vector<uint32_t> idx;
@@ -47,7 +53,11 @@ std::unique_ptr<Term> build_term(const Mesh& g, const mafx_term_desc& d, int nth
     case MAFX_ATOM_EXCHANGE: return make_atom_exchange(d.value);
     case MAFX_ATOM_DMI: return make_atom_dmi(d.value, d.axis);
     case MAFX_ATOM_UNIAXIAL: return make_atom_uniaxial(d.value, d.axis);
-    case MAFX_ATOM_ZEEMAN: return make_atom_zeeman(hext());
+    case MAFX_ATOM_ZEEMAN:
+        if (d.callback)
+            return make_zeeman_callback(d.callback, d.callback_user, (d.flags & MAFX_TERM_HOMOGENEOUS_CALLBACK) != 0,
+                                        true);
+        return make_atom_zeeman(hext());
     default: return nullptr;
     }
 }
@@ -191,6 +201,14 @@ int oracle_llg_set_alpha(oracle_sim* s, double alpha) {
     return 0;
 }
 
+int oracle_llg_set_alpha_field(oracle_sim* s, const double* alpha) {
+    if (alpha)
+        s->llg.alpha_field.assign(alpha, alpha + s->state.mesh.n());
+    else
+        s->llg.alpha_field.clear();
+    return 0;
+}
+
 int oracle_llg_step(oracle_sim* s, double* h_used, double* err, double* h_next, int64_t* accepted,
                     int64_t* rejected, int64_t* heff_evals) {
     if (!s->llg_ready) return 4;
@@ -250,6 +268,36 @@ int oracle_lbfgs(oracle_sim* s, double tol, int64_t maxiter, double torque_tol, 
     } catch (...) {
         return 1;
     }
+}
+
+int oracle_lbfgs_trace(oracle_sim* s, double tol, int64_t maxiter, double* E, double* rows_out, int64_t max_rows,
+                       int64_t* rows) {
+    try {
+        std::vector<LbfgsTraceRow> tr;
+        auto r = lbfgs_minimize(s->terms, s->state, tol, maxiter, 0, 0.0, &tr);
+        if (E) *E = r.E;
+        const int64_t n = std::min<int64_t>(static_cast<int64_t>(tr.size()), max_rows);
+        for (int64_t i = 0; i < n; ++i) {
+            const LbfgsTraceRow& t = tr[i];
+            const double v[7] = {t.finit, t.dginit, t.stp, t.f, t.dg, t.info, t.nfev};
+            for (int c = 0; c < 7; ++c) rows_out[7 * i + c] = v[c];
+        }
+        *rows = n;
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
+int oracle_llg_counters(const oracle_sim* s, int64_t out[6]) {
+    const Controller& c = s->llg.controller;
+    out[0] = c.counter_accepted;
+    out[1] = c.counter_reject;
+    out[2] = c.counter_hmin;
+    out[3] = c.counter_hmax;
+    out[4] = c.counter_maxscale;
+    out[5] = c.counter_minscale;
+    return 0;
 }
 
 int oracle_demag_tensor(const mafx_mesh* mesh, double* N, int nthreads) {

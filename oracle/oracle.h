@@ -37,6 +37,8 @@ int oracle_mean_m(const oracle_sim* s, double out[3]);
 int oracle_llg_init(oracle_sim* s, double alpha, const char* scheme, double hmin, double hmax, double atol,
                     double rtol, int dissipation_term_only, int normalize, int analytic_rhs);
 int oracle_llg_set_alpha(oracle_sim* s, double alpha);
+/* LLGIntegrator(af::array alpha_field, ...) (llg_integrator.cpp:18-38): alpha [N], NULL for the scalar. */
+int oracle_llg_set_alpha_field(oracle_sim* s, const double* alpha);
 /* One AdaptiveRungeKutta::step; outputs the accepted h, its error norm, counters. */
 int oracle_llg_step(oracle_sim* s, double* h_used, double* err, double* h_next, int64_t* accepted,
                     int64_t* rejected, int64_t* heff_evals);
@@ -47,6 +49,13 @@ int oracle_llg_attempt(oracle_sim* s, double dt, double* dm, double* err);
 
 int oracle_lbfgs(oracle_sim* s, double tol, int64_t maxiter, double torque_tol, double* E, int64_t* iters,
                  int64_t* evals, int* reason);
+
+/* oracle_lbfgs without the torque stop, recording one row per line search: (f(0), g(0).s, step,
+ * f(step), g(step).s, cvsrch info code, evaluations), 7 doubles per row. */
+int oracle_lbfgs_trace(oracle_sim* s, double tol, int64_t maxiter, double* E, double* rows_out, int64_t max_rows,
+                       int64_t* rows);
+/* Controller counters: accepted, rejected, hmin, hmax, maxscale, minscale (controller.hpp:43-49). */
+int oracle_llg_counters(const oracle_sim* s, int64_t out[6]);
 
 /* Real-space Newell tensor on the expanded grid, [6][n2e][n1e][n0e] (demag_field.cpp:104-142). */
 int oracle_demag_tensor(const mafx_mesh* mesh, double* N, int nthreads);

@@ -14,6 +14,8 @@
 #include <string>
 #include <vector>
 
+#include "../include/mafx.h"
+
 namespace oracle {
 
 using cplx = std::complex<double>;
@@ -83,9 +85,12 @@ void demag_tensor_real(const Mesh& mesh, Vec& N, int nthreads);
 
 std::unique_ptr<Term> make_demag(const Mesh& mesh, int nthreads);
 std::unique_ptr<Term> make_exchange(double A);
-std::unique_ptr<Term> make_dmi(double D, const double axis[3]);
-std::unique_ptr<Term> make_uniaxial(double Ku1, const double axis[3]);
+std::unique_ptr<Term> make_dmi(double D, const double axis[3], const Vec& Dfield); // Dfield N or empty
+// Kfield N or empty, axis_field 3N or empty
+std::unique_ptr<Term> make_uniaxial(double Ku1, const double axis[3], const Vec& Kfield, const Vec& axis_field);
 std::unique_ptr<Term> make_zeeman(const Vec& h_ext); // 3N
+// time-dependent ExternalField / AtomisticExternalField (external_field.cpp:13-17,28-36)
+std::unique_ptr<Term> make_zeeman_callback(mafx_field_fn cb, void* user, bool homogeneous, bool atomistic);
 std::unique_ptr<Term> make_rkky(const Mesh& mesh, const Vec& rkky, const Vec& A,
                                 const std::vector<uint32_t>& idx);
 std::unique_ptr<Term> make_atom_dipole(const Mesh& mesh);
@@ -118,6 +123,7 @@ struct Controller { // controller.hpp:11-61, controller.cpp:36-95
 struct LLG { // llg_integrator.hpp / adaptive_runge_kutta.hpp
     const Terms* terms{nullptr};
     double alpha{0};
+    Vec alpha_field; // N, empty for the scalar alpha (llg_integrator.hpp:32-40)
     std::string scheme{"RKF45"};
     Controller controller;
     bool normalize{true};
@@ -151,8 +157,14 @@ struct LbfgsResult {
     int64_t iterations{0}, evals{0};
     int stop_reason{0};
 };
-// lbfgs_minimizer.cpp:55-505; torque_tol > 0 adds the build-side torque stop.
+// One line search as cvsrch ends it: f(0), g(0).s, the returned step, f and g.s there, its info
+// code (1 = the sufficient-decrease and curvature tests hold) and the evaluation count.
+struct LbfgsTraceRow {
+    double finit, dginit, stp, f, dg, info, nfev;
+};
+// lbfgs_minimizer.cpp:55-505; torque_tol > 0 adds the build-side torque stop; trace, when given,
+// receives one row per line search.
 LbfgsResult lbfgs_minimize(const Terms& terms, State& s, double tolerance, int64_t maxIter,
-                           int verbose, double torque_tol);
+                           int verbose, double torque_tol, std::vector<LbfgsTraceRow>* trace = nullptr);
 
 } // namespace oracle

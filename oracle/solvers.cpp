@@ -79,6 +79,15 @@ void LLG::f(const State& s, Vec& out) const {
     Vec mxmxh;
     cross4(s.m, mxh, mxmxh, N);
     out.resize(3 * N);
+    if (!alpha_field.empty()) { // llg_integrator.cpp:22-24,30-32: alpha tiled over the components
+        for (int64_t i = 0; i < 3 * N; ++i) {
+            const double al = alpha_field[i % N];
+            const double a2 = std::pow(al, 2.0);
+            const double damp = -al * gamma_ / (1. + a2) * mxmxh[i];
+            out[i] = dissipation_term_only ? damp : -gamma_ / (1. + a2) * mxh[i] + damp;
+        }
+        return;
+    }
     const double a2 = std::pow(alpha, 2.0);
     const double cprec = -gamma_ / (1. + a2);          // equations.hpp:19-21
     const double cdamp = -alpha * gamma_ / (1. + a2);  // equations.hpp:24-27
@@ -549,7 +558,7 @@ int cstep(double& stx, double& fx, double& dx, double& sty, double& fy, double& 
 
 // :213-360
 int cvsrch(EvalCtx& ev, State& state, const Vec& wa, double& f, Vec& g, double& stp, const Vec& s,
-           const double tolf) {
+           const double tolf, LbfgsTraceRow* trace) {
     int info = 0;
     int infoc = 1;
     const double xtol = 1e-15;
@@ -602,7 +611,11 @@ int cvsrch(EvalCtx& ev, State& state, const Vec& wa, double& f, Vec& g, double& 
         if (brackt & (stmax - stmin <= xtol * stmax)) info = 2;
         if ((f <= ftest1) & (std::fabs(dg) <= gtol * (-dginit))) info = 1;
         if (((f <= ftest2) && (ft * dginit >= dg)) && (std::fabs(dg) <= gtol * (-dginit))) info = 1;
-        if (info != 0) return -1;
+        if (info != 0) {
+            if (trace) *trace = LbfgsTraceRow{finit, dginit, stp, f, dg, static_cast<double>(info),
+                                              static_cast<double>(nfev)};
+            return -1;
+        }
         if (stage1 & ((f <= ftest2) && (ft * dginit >= dg)) & (dg >= std::min(ftol, gtol) * dginit)) stage1 = false;
         if (stage1 & (f <= fx) & (!((f <= ftest2) && (ft * dginit >= dg)))) {
             double fm = f - stp * dgtest;
@@ -632,7 +645,7 @@ int cvsrch(EvalCtx& ev, State& state, const Vec& wa, double& f, Vec& g, double& 
 
 // :55-198
 LbfgsResult lbfgs_minimize(const Terms& terms, State& state, double tolerance, int64_t maxIter, int verbose,
-                           double torque_tol) {
+                           double torque_tol, std::vector<LbfgsTraceRow>* trace) {
     (void)verbose;
     LbfgsResult res;
     EvalCtx ev{&terms, 0, {}};
@@ -684,7 +697,9 @@ LbfgsResult lbfgs_minimize(const Terms& terms, State& state, double tolerance, i
         Vec dir(n);
         for (size_t j = 0; j < n; ++j) dir[j] = -q[j];
         double rate = 1.0;
-        cvsrch(ev, state, x_old, f, grad, rate, dir, tolerance);
+        LbfgsTraceRow row{};
+        cvsrch(ev, state, x_old, f, grad, rate, dir, tolerance, trace ? &row : nullptr);
+        if (trace) trace->push_back(row);
         const double f1 = 1 + std::fabs(f);
         gradNorm = max_4d_abs(grad);
         if (gradNorm < (epsr * f1)) {

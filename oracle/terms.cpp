@@ -412,37 +412,101 @@ void dmi_core(const State& s, const std::array<double, 3>& n, bool unit, Vec& ou
         for (int64_t i = 0; i < N; ++i) out[c * N + i] = n[c] * div[i] - second[c * N + i];
 }
 
-struct Dmi : Term { // dmi_field.cpp:41-110 (scalar D branch)
+// Micro coefficient numerator(i) / (mu0 Ms) per cell, with the Ms_field branch that replaces cells
+// of Ms == 0 by 0 (dmi_field.cpp:97-109, uniaxial_anisotropy_field.cpp:78-88): the spatially
+// varying D_constants / Ku1_field variants of scale_by_ms.
+void scale_by_ms_field(const State& s, const Vec& numer, Vec& h) {
+    const int64_t N = s.mesh.n();
+    for (int c = 0; c < 3; ++c)
+        for (int64_t i = 0; i < N; ++i) {
+            if (!s.has_Ms_field()) {
+                h[c * N + i] = numer[i] / (mu0 * s.Ms) * h[c * N + i];
+            } else {
+                const double ms = s.Ms_field[i];
+                h[c * N + i] = ms != 0 ? numer[i] / (mu0 * ms) * h[c * N + i] : 0.0;
+            }
+        }
+}
+
+struct Dmi : Term { // dmi_field.cpp:41-110 (scalar D, or the D_constants field of :17-19)
     double D;
     std::array<double, 3> axis;
-    Dmi(double d, const double a[3]) : D(d), axis{a[0], a[1], a[2]} {}
+    Vec Dfield; // N, empty for the scalar D
+    Dmi(double d, const double a[3], const Vec& df) : D(d), axis{a[0], a[1], a[2]}, Dfield(df) {}
     void H(const State& s, Vec& h) const override {
         const double norm = std::sqrt(std::pow(axis[0], 2) + std::pow(axis[1], 2) + std::pow(axis[2], 2));
         const std::array<double, 3> n{axis[0] / norm, axis[1] / norm, axis[2] / norm};
         dmi_core(s, n, false, h);
-        scale_by_ms(s, 2. * D, h);
+        if (Dfield.empty()) {
+            scale_by_ms(s, 2. * D, h);
+        } else {
+            Vec numer(Dfield.size());
+            for (size_t i = 0; i < numer.size(); ++i) numer[i] = 2. * Dfield[i];
+            scale_by_ms_field(s, numer, h);
+        }
     }
 };
 
-struct Uniaxial : Term { // uniaxial_anisotropy_field.cpp:11-12,58-89 (scalar Ku1, uniform axis)
+// uniaxial_anisotropy_field.cpp:11-25,58-89: scalar Ku1 or Ku1_field, a uniform axis (normalised
+// in the constructor) or an axis field (normalised per cell by util::normalize, a / |a|).
+struct Uniaxial : Term {
     double Ku1;
     std::array<double, 3> eu;
-    Uniaxial(double k, const double a[3]) : Ku1(k), eu(normalized(a)) {}
+    Vec Kfield; // N, empty for the scalar Ku1
+    Vec eufield; // 3N, empty for the uniform axis
+    Uniaxial(double k, const double a[3], const Vec& kf, const Vec& af) : Ku1(k), eu(normalized(a)), Kfield(kf) {
+        if (!af.empty()) {
+            const int64_t N = static_cast<int64_t>(af.size() / 3);
+            eufield.resize(af.size());
+            for (int64_t i = 0; i < N; ++i) {
+                const double n = std::sqrt(af[i] * af[i] + af[N + i] * af[N + i] + af[2 * N + i] * af[2 * N + i]);
+                for (int c = 0; c < 3; ++c) eufield[c * N + i] = af[c * N + i] / n;
+            }
+        }
+    }
     void H(const State& s, Vec& h) const override {
         const int64_t N = s.mesh.n();
         h.resize(3 * N);
         for (int64_t i = 0; i < N; ++i) {
-            const double an = eu[0] * s.m[i] + eu[1] * s.m[N + i] + eu[2] * s.m[2 * N + i];
-            for (int c = 0; c < 3; ++c) h[c * N + i] = eu[c] * an;
+            double e[3];
+            for (int c = 0; c < 3; ++c) e[c] = eufield.empty() ? eu[c] : eufield[c * N + i];
+            const double an = e[0] * s.m[i] + e[1] * s.m[N + i] + e[2] * s.m[2 * N + i];
+            for (int c = 0; c < 3; ++c) h[c * N + i] = e[c] * an;
         }
-        scale_by_ms(s, 2. * Ku1, h);
+        if (Kfield.empty()) {
+            scale_by_ms(s, 2. * Ku1, h);
+        } else {
+            Vec numer(Kfield.size());
+            for (size_t i = 0; i < numer.size(); ++i) numer[i] = 2. * Kfield[i];
+            scale_by_ms_field(s, numer, h);
+        }
     }
 };
 
-struct Zeeman : Term { // external_field.cpp:28-48
+// external_field.cpp:28-48: a constant field, or the callback of :13-17 evaluated at the State's
+// time (here the callback writes the field into a host buffer, 3N values or one homogeneous 3-vector)
+struct Zeeman : Term {
     Vec hext;
+    mafx_field_fn cb{nullptr};
+    void* cb_user{nullptr};
+    bool cb_homogeneous{false};
     explicit Zeeman(const Vec& h) : hext(h) {}
-    void H(const State&, Vec& h) const override { h = hext; }
+    void H(const State& s, Vec& h) const override {
+        if (!cb) {
+            h = hext;
+            return;
+        }
+        const int64_t N = s.mesh.n();
+        h.assign(3 * N, 0.0);
+        if (cb_homogeneous) {
+            double v[3] = {0, 0, 0};
+            if (cb(cb_user, s.t, s.m.data(), v, nullptr) != 0) throw std::runtime_error("field callback failed");
+            for (int c = 0; c < 3; ++c)
+                for (int64_t i = 0; i < N; ++i) h[c * N + i] = v[c];
+        } else if (cb(cb_user, s.t, s.m.data(), h.data(), nullptr) != 0) {
+            throw std::runtime_error("field callback failed");
+        }
+    }
     double E(const State& s, const Vec& h) const override { return micro_energy(s, h, 1.0); }
 };
 
@@ -598,10 +662,8 @@ struct AtomUniaxial : Term { // atomistic_uniaxial_anisotropy_field.cpp:6-34
     double E(const State& s, const Vec& h) const override { return atom_energy(s, h, 0.5); }
 };
 
-struct AtomZeeman : Term { // atomistic_external_field.cpp:5-11
-    Vec hext;
-    explicit AtomZeeman(const Vec& h) : hext(h) {}
-    void H(const State&, Vec& h) const override { h = hext; }
+struct AtomZeeman : Zeeman { // atomistic_external_field.cpp:5-11 (the field of ExternalField)
+    explicit AtomZeeman(const Vec& h) : Zeeman(h) {}
     double E(const State& s, const Vec& h) const override { return atom_energy(s, h, 1.0); }
 };
 
@@ -609,9 +671,25 @@ struct AtomZeeman : Term { // atomistic_external_field.cpp:5-11
 
 std::unique_ptr<Term> make_demag(const Mesh& g, int nthreads) { return std::make_unique<Demag>(g, nthreads); }
 std::unique_ptr<Term> make_exchange(double A) { return std::make_unique<Exchange>(A); }
-std::unique_ptr<Term> make_dmi(double D, const double axis[3]) { return std::make_unique<Dmi>(D, axis); }
-std::unique_ptr<Term> make_uniaxial(double K, const double axis[3]) { return std::make_unique<Uniaxial>(K, axis); }
+std::unique_ptr<Term> make_dmi(double D, const double axis[3], const Vec& Dfield) {
+    return std::make_unique<Dmi>(D, axis, Dfield);
+}
+std::unique_ptr<Term> make_uniaxial(double K, const double axis[3], const Vec& Kfield, const Vec& axis_field) {
+    return std::make_unique<Uniaxial>(K, axis, Kfield, axis_field);
+}
 std::unique_ptr<Term> make_zeeman(const Vec& h) { return std::make_unique<Zeeman>(h); }
+std::unique_ptr<Term> make_zeeman_callback(mafx_field_fn cb, void* user, bool homogeneous, bool atomistic) {
+    auto z = std::make_unique<Zeeman>(Vec{});
+    z->cb = cb;
+    z->cb_user = user;
+    z->cb_homogeneous = homogeneous;
+    if (!atomistic) return z;
+    auto a = std::make_unique<AtomZeeman>(Vec{});
+    a->cb = cb;
+    a->cb_user = user;
+    a->cb_homogeneous = homogeneous;
+    return a;
+}
 std::unique_ptr<Term> make_rkky(const Mesh& g, const Vec& rk, const Vec& A, const std::vector<uint32_t>& idx) {
     return std::make_unique<Rkky>(g, rk, A, idx);
 }

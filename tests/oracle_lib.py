@@ -23,11 +23,17 @@ class Mesh(C.Structure):
                 ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double)]
 
 
+# int (*)(void* user, double t, const double* m, double* h_out, void* stream); the oracle passes
+# host pointers and a null stream
+FIELD_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p)
+HOMOGENEOUS_CALLBACK = 8
+
+
 class TermDesc(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("value", C.c_double),
+    _fields_ = [("kind", C.c_int32), ("flags", C.c_int32), ("value", C.c_double),
                 ("axis", C.c_double * 3), ("field", C.c_double * 3),
                 ("array0", C.POINTER(C.c_double)), ("array1", C.POINTER(C.c_double)),
-                ("index0", C.POINTER(C.c_uint32))]
+                ("index0", C.POINTER(C.c_uint32)), ("callback", FIELD_FN), ("callback_user", C.c_void_p)]
 
 
 _lib = None
@@ -58,12 +64,15 @@ def lib():
         L.oracle_llg_init.argtypes = [C.c_void_p, C.c_double, C.c_char_p, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.c_int, C.c_int, C.c_int]
         L.oracle_llg_set_alpha.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_llg_set_alpha_field.argtypes = [C.c_void_p, dp]
         i64p = C.POINTER(C.c_int64)
         L.oracle_llg_step.argtypes = [C.c_void_p, dp, dp, dp, i64p, i64p, i64p]
         L.oracle_llg_integrate_dense.argtypes = [C.c_void_p, C.c_double, C.c_double, dp, C.c_int64, i64p]
         L.oracle_llg_attempt.argtypes = [C.c_void_p, C.c_double, dp, dp]
         L.oracle_lbfgs.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_double, dp, i64p, i64p,
                                    C.POINTER(C.c_int)]
+        L.oracle_lbfgs_trace.argtypes = [C.c_void_p, C.c_double, C.c_int64, dp, dp, C.c_int64, i64p]
+        L.oracle_llg_counters.argtypes = [C.c_void_p, i64p]
         L.oracle_demag_tensor.argtypes = [C.POINTER(Mesh), dp, C.c_int]
         L.oracle_newell_nxx.argtypes = [C.c_int] * 3 + [C.c_double] * 3
         L.oracle_newell_nxx.restype = C.c_double
@@ -94,14 +103,35 @@ def scalar_flat(s):
     return np.ascontiguousarray(s.transpose(2, 1, 0)).ravel()
 
 
-def term(kind, value=0.0, axis=(0.0, 0.0, 1.0), field=(0.0, 0.0, 0.0), array0=None, array1=None, index0=None):
-    """Build a TermDesc; arrays are kept alive on the returned object."""
+def term(kind, value=0.0, axis=(0.0, 0.0, 1.0), field=(0.0, 0.0, 0.0), array0=None, array1=None, index0=None,
+         callback=None, homogeneous_callback=False):
+    """Build a TermDesc; arrays are kept alive on the returned object.  callback(t, m_flat) returns
+    the external field at time t: a 3-vector with homogeneous_callback, else the flat 3N field."""
     d = TermDesc()
     d.kind = kind
     d.value = value
     d.axis[:] = list(axis)
     d.field[:] = list(field)
     keep = []
+    if callback is not None:
+        d.flags = HOMOGENEOUS_CALLBACK if homogeneous_callback else 0
+
+        def fn(user, t, m_ptr, h_ptr, stream, _cb=callback, _hom=homogeneous_callback):
+            try:
+                if _hom:
+                    v = np.asarray(_cb(t, None), dtype=np.float64)
+                    C.memmove(h_ptr, v.ctypes.data, 24)
+                else:
+                    n3 = d._n3
+                    m = np.ctypeslib.as_array(C.cast(m_ptr, C.POINTER(C.c_double)), shape=(n3,)).copy()
+                    v = np.ascontiguousarray(_cb(t, m), dtype=np.float64)
+                    C.memmove(h_ptr, v.ctypes.data, 8 * n3)
+                return 0
+            except Exception:  # noqa: BLE001 - reported as a failed evaluation
+                return 1
+        f = FIELD_FN(fn)
+        d.callback = f
+        keep.append(f)
     if array0 is not None:
         a = np.ascontiguousarray(array0, dtype=np.float64)
         keep.append(a)
@@ -126,6 +156,8 @@ class Sim:
         self.nx, self.ny, self.nz = self.mesh.nx, self.mesh.ny, self.mesh.nz
         self.N = self.nx * self.ny * self.nz
         self.terms = list(terms)
+        for t in self.terms:
+            t._n3 = 3 * self.N  # full-field callbacks read and write 3N values
         arr = (TermDesc * max(1, len(self.terms)))(*self.terms)
         m_flat = to_flat(m) if np.ndim(m) == 4 else np.ascontiguousarray(m, dtype=np.float64)
         msf = None
@@ -142,6 +174,7 @@ class Sim:
             self.h = None
 
     def add_term(self, d, nthreads=0):
+        d._n3 = 3 * self.N
         self.terms.append(d)
         assert lib().oracle_add_term(self.h, C.byref(d), nthreads) == 0
 
@@ -200,6 +233,15 @@ class Sim:
     def set_alpha(self, a):
         lib().oracle_llg_set_alpha(self.h, a)
 
+    def set_alpha_field(self, alpha):
+        """Cell-wise damping (nx, ny, nz) or flat [N]; None returns to the scalar alpha."""
+        if alpha is None:
+            lib().oracle_llg_set_alpha_field(self.h, None)
+            return
+        a = scalar_flat(alpha) if np.ndim(alpha) == 3 else np.ascontiguousarray(alpha, dtype=np.float64)
+        self._alpha_keep = a
+        lib().oracle_llg_set_alpha_field(self.h, _dp(a))
+
     def step(self):
         h, e, hn = C.c_double(), C.c_double(), C.c_double()
         acc, rej, ev = C.c_int64(), C.c_int64(), C.c_int64()
@@ -219,6 +261,21 @@ class Sim:
         rows = C.c_int64()
         assert lib().oracle_llg_integrate_dense(self.h, time, dt_write, _dp(out), max_rows, C.byref(rows)) == 0
         return out[: 4 * rows.value].reshape(rows.value, 4)
+
+    def counters(self):
+        """Controller counters: accepted, rejected, hmin, hmax, maxscale, minscale."""
+        out = (C.c_int64 * 6)()
+        lib().oracle_llg_counters(self.h, out)
+        return dict(zip(["accepted", "rejected", "hmin", "hmax", "maxscale", "minscale"], list(out)))
+
+    def lbfgs_trace(self, tol=1e-6, maxiter=230, max_rows=100000):
+        """LBFGS without the torque stop; returns (E, rows) with one row per line search:
+        f(0), g(0).s, step, f(step), g(step).s, cvsrch info, evaluations."""
+        E = C.c_double()
+        out = np.zeros(7 * max_rows)
+        rows = C.c_int64()
+        assert lib().oracle_lbfgs_trace(self.h, tol, maxiter, C.byref(E), _dp(out), max_rows, C.byref(rows)) == 0
+        return E.value, out[: 7 * rows.value].reshape(rows.value, 7)
 
     def lbfgs(self, tol=1e-6, maxiter=230, torque_tol=0.0):
         E, it, ev, rs = C.c_double(), C.c_int64(), C.c_int64(), C.c_int()

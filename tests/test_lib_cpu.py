@@ -68,10 +68,12 @@ def test_context_without_gpu_fails_loudly():
     assert b"GPU" in L.mafx_last_error() or b"device" in L.mafx_last_error()
 
 
-@pytest.mark.parametrize("mesh,prec,dipole", [((128, 32, 1, 4e-9, 4e-9, 3e-9), 0, 0),
-                                              ((1024, 1024, 4, 2e-9, 2e-9, 2e-9), 1, 0),
-                                              ((512, 512, 1, 2.7e-10, 2.7e-10, 2.7e-10), 0, 1)])
-def test_demag_cache_name_is_the_reference_mesh_string(mesh, prec, dipole):
+@pytest.mark.parametrize("mesh,prec,flags,dipole", [((128, 32, 1, 4e-9, 4e-9, 3e-9), 0, 0, 0),
+                                                    ((1024, 1024, 4, 2e-9, 2e-9, 2e-9), 1, 0, 0),
+                                                    ((1024, 1024, 4, 2e-9, 2e-9, 2e-9), 1, 4, 0),
+                                                    ((64, 32, 2, 2e-9, 2e-9, 2e-9), 0, 32, 0),
+                                                    ((512, 512, 1, 2.7e-10, 2.7e-10, 2.7e-10), 0, 0, 1)])
+def test_demag_cache_name_is_the_reference_mesh_string(mesh, prec, flags, dipole):
     """The cache file name starts with the reference's to_string(mesh) (demag_field.cpp:185-190):
     expanded lengths and 1e9 * d printed by std::to_string (six decimals); the layout and the exact
     bits of dx, dy, dz follow, so meshes that print alike never share a file."""
@@ -82,13 +84,17 @@ def test_demag_cache_name_is_the_reference_mesh_string(mesh, prec, dipole):
            f"_dx_{1e9 * dx:f}_dy_{1e9 * dy:f}_dz_{1e9 * dz:f}")
     buf = C.create_string_buffer(512)
     m = _abi.Mesh(*mesh)
-    assert _lib().mafx_demag_cache_name(C.byref(m), prec, dipole, buf, len(buf)) == 0
+    assert _lib().mafx_demag_cache_name(C.byref(m), prec, flags, dipole, buf, len(buf)) == 0
     name = buf.value.decode()
-    assert name.startswith(ref + ("_mafx_c64" if prec == 0 else "_mafx_f32sym"))
+    # the layouts built from the other real-space tensor (MAFX_TERM_HOST_TENSOR on the FP32 path,
+    # MAFX_TERM_DEVICE_TENSOR on the FP64 path) have names of their own
+    layout = {(0, 0): "_mafx_c64_", (0, 32): "_mafx_c64_dev_", (1, 0): "_mafx_f32sym_", (1, 4): "_mafx_f32sym_host_"}
+    assert name.startswith(ref + layout[(prec, flags)].rstrip("_"))
+    assert (layout[(prec, flags)] in name) or (dipole and layout[(prec, flags)].rstrip("_") + "_dipole_" in name)
     bits = "".join(f"{struct.unpack('<Q', struct.pack('<d', v))[0]:016x}" for v in (dx, dy, dz))
     assert name.endswith(("_dipole_" if dipole else "_") + bits)
     other = _abi.Mesh(nx, ny, nz, dx * (1 + 1e-12), dy, dz)  # prints the same, differs in its bits
     buf2 = C.create_string_buffer(512)
-    assert _lib().mafx_demag_cache_name(C.byref(other), prec, dipole, buf2, len(buf2)) == 0
+    assert _lib().mafx_demag_cache_name(C.byref(other), prec, flags, dipole, buf2, len(buf2)) == 0
     assert buf2.value.decode().startswith(ref) and buf2.value != buf.value
-    assert _lib().mafx_demag_cache_name(C.byref(m), prec, dipole, buf, 8) == 1  # too small: error
+    assert _lib().mafx_demag_cache_name(C.byref(m), prec, flags, dipole, buf, 8) == 1  # too small: error

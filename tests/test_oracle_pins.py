@@ -457,14 +457,28 @@ def test_state_normalize_and_mean():
 
 # ---------------------------------------------------------------- controller clamp (controller.cpp:77-88)
 def test_controller_hmin_clamp_accepts():
-    # a huge field with a tiny hmax/hmin window forces the clamp-accept branch
+    """A rejected attempt whose shrunk step falls to hmin is accepted (controller.cpp:77-88): t
+    advances by the clamped h = hmin while m advances by the attempt made with the old h
+    (adaptive_runge_kutta.cpp:39-66).  The first attempt runs at h = 1.01e-15; with hmin = 1e-15 and
+    tolerances of 1e-30 its error norm is far above 1, the scale is at least minscale = 0.2, so the
+    shrunk h is below hmin and is clamped."""
     m = np.zeros((2, 1, 1, 3))
     m[..., 0] = 1
     m[1, 0, 0] = [0, 1, 0]
-    s = O.Sim((2, 1, 1, 1e-9, 1e-9, 1e-9), 8e5, m, [O.term(O.ZEEMAN, field=(0, 0, 1e12))])
-    s.llg_init(0.5, "DP45", hmin=1e-13, hmax=2e-13, atol=1e-12, rtol=1e-12)
+    H = (0, 0, 1e9)
+    s = O.Sim((2, 1, 1, 1e-9, 1e-9, 1e-9), 8e5, m, [O.term(O.ZEEMAN, field=H)])
+    s.llg_init(0.5, "DP45", hmin=1e-15, hmax=3.5e-10, atol=1e-30, rtol=1e-30)
+    dm, err = s.attempt(1.01e-15)  # the attempt the step makes first (DP45 recomputes k1, normalize is on)
+    assert err > 1.0
     r = s.step()
-    assert r["h"] in (1e-13, 2e-13) or r["h"] <= 2e-13
+    c = s.counters()
+    assert r["h"] == 1e-15 and s.t == 1e-15
+    assert c["hmin"] == 1 and c["rejected"] == 0 and c["accepted"] == 0 and r["rejected"] == 0
+    mn = s.m_flat()
+    expect = O.to_flat(m) + dm
+    expect = expect.reshape(3, -1)
+    expect = (expect / np.sqrt((expect ** 2).sum(axis=0))).ravel()
+    assert np.max(np.abs(mn - expect)) <= 1e-15
 
 
 # ---------------------------------------------------------------- LBFGS: known minimum
@@ -482,3 +496,236 @@ def test_lbfgs_aligns_with_field():
     assert r["E"] < E0
     assert abs(r["E"] - E_min) < 1e-8 * abs(E_min)
     assert np.min(s.m()[..., 2]) > 1 - 1e-6
+
+
+# ---------------------------------------------------------------- macrospin closed forms (integrate_dense)
+GAMMA = 1.760859644e11 * MU0
+
+
+def macrospin_closed_form(t, theta0, phi0, H, alpha):
+    """One spin in a constant field H along z under dm/dt = -gamma/(1+a^2) [m x H + a m x (m x H)]
+    (equations.hpp:19-33): tan(theta/2) = tan(theta0/2) exp(-gamma a H t/(1+a^2)) and
+    phi = phi0 + gamma H t/(1+a^2)."""
+    c = GAMMA * H / (1 + alpha * alpha)
+    th = 2 * np.arctan(np.tan(theta0 / 2) * np.exp(-alpha * c * t))
+    ph = phi0 + c * t
+    return np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=-1)
+
+
+@pytest.mark.parametrize("scheme", ["RKF45", "DP45", "DP78"])
+def test_integrate_dense_macrospin(scheme):
+    """LLGIntegrator::integrate_dense (llg_integrator.cpp:68-104): every dense sample, taken by the
+    backward RKF45 step from the state past the write time, lies on the closed-form damped
+    precession of a single spin, within the error the tolerances allow.  A wrong sign of the back
+    step, a sample at the overshooting state or a missing stage would leave errors of order
+    |dm/dt| h ~ 1e-2 here."""
+    H, alpha = 8e4, 0.05
+    theta0, phi0 = 2.0, 0.3
+    m0 = macrospin_closed_form(0.0, theta0, phi0, H, alpha).reshape(1, 1, 1, 3)
+    s = O.Sim((1, 1, 1, 1e-9, 1e-9, 1e-9), 8e5, m0, [O.term(O.ZEEMAN, field=(0, 0, H))])
+    s.llg_init(alpha, scheme, atol=1e-10, rtol=1e-10)
+    rows = s.integrate_dense(3e-10, 1e-11)
+    # samples sit on the write grid; a step longer than the write interval skips grid points
+    # (llg_integrator.cpp:95-99), which DP78's long steps do, the final one included
+    k = np.rint(rows[:, 0] / 1e-11)
+    assert np.allclose(rows[:, 0], k * 1e-11, rtol=0, atol=1e-24) and k[0] == 0
+    assert np.all(np.diff(k) >= 1) and (scheme == "DP78" or (rows.shape == (31, 4) and abs(s.t - 3e-10) < 1e-22))
+    ref = macrospin_closed_form(rows[:, 0], theta0, phi0, H, alpha)
+    # the back step is RKF45 over up to one step length: DP78's long steps leave it ~6e-7
+    assert np.max(np.abs(rows[:, 1:] - ref)) < (2e-6 if scheme == "DP78" else 2e-8), np.max(np.abs(rows[:, 1:] - ref))
+
+
+def test_alpha_field_cells_follow_their_own_damping():
+    """LLGIntegrator(alpha_field) (llg_integrator.cpp:18-38): three uncoupled spins (Zeeman only)
+    with damping 0.01, 0.1 and 1 each follow the closed form of their own alpha; a uniform alpha
+    field gives the scalar-alpha run to the last bit."""
+    H = 5e4
+    th0 = np.array([2.5, 1.0, 0.4])
+    ph0 = np.array([0.0, 1.0, 2.0])
+    alphas = np.array([0.01, 0.1, 1.0])
+    m0 = np.stack([macrospin_closed_form(0.0, th0[i], ph0[i], H, alphas[i]) for i in range(3)]).reshape(3, 1, 1, 3)
+    mesh = (3, 1, 1, 1e-9, 1e-9, 1e-9)
+    s = O.Sim(mesh, 8e5, m0, [O.term(O.ZEEMAN, field=(0, 0, H))])
+    s.llg_init(0.0, "DP45", atol=1e-11, rtol=1e-11)
+    s.set_alpha_field(alphas.reshape(3, 1, 1))
+    while s.t < 2e-10:
+        s.step()
+    got = s.m()[:, 0, 0, :]
+    ref = np.stack([macrospin_closed_form(s.t, th0[i], ph0[i], H, alphas[i]) for i in range(3)])
+    assert np.max(np.abs(got - ref)) < 1e-8
+    a = O.Sim(mesh, 8e5, m0, [O.term(O.ZEEMAN, field=(0, 0, H))])
+    b = O.Sim(mesh, 8e5, m0, [O.term(O.ZEEMAN, field=(0, 0, H))])
+    a.llg_init(0.1, "DP45")
+    b.llg_init(0.0, "DP45")
+    b.set_alpha_field(np.full((3, 1, 1), 0.1))
+    for _ in range(20):
+        ra, rb = a.step(), b.step()
+        assert ra["h"] == rb["h"]
+    assert np.array_equal(a.m_flat(), b.m_flat())
+
+
+def test_time_dependent_field_stage_times():
+    """ExternalField(callback) (external_field.cpp:28-36) inside the RK stages: with H(t) = H0 cos(w t)
+    along z and no damping, a spin in the xy plane turns by phi(t) = gamma H0 sin(w t) / w.  The
+    stages must see t + c_i h (adaptive_runge_kutta.cpp:113-173); evaluating every stage at t makes
+    the method first order in h and misses this by ~1e-4 at these tolerances."""
+    H0, w = 1e5, 2 * math.pi * 4e9
+    for scheme in ["DP45", "RKF45", "BS45"]:
+        m0 = np.zeros((1, 1, 1, 3))
+        m0[..., 0] = 1
+        cb = O.term(O.ZEEMAN, callback=lambda t, m: (0.0, 0.0, H0 * math.cos(w * t)), homogeneous_callback=True)
+        s = O.Sim((1, 1, 1, 1e-9, 1e-9, 1e-9), 8e5, m0, [cb])
+        s.llg_init(0.0, scheme, atol=1e-10, rtol=1e-10)
+        while s.t < 4e-10:
+            s.step()
+        ph = GAMMA * H0 * math.sin(w * s.t) / w
+        got = s.m()[0, 0, 0]
+        assert np.max(np.abs(got - [math.cos(ph), math.sin(ph), 0.0])) < 1e-8, (scheme, got, ph)
+    # a full-field callback equal to a static array gives the static run to the last bit
+    rng = np.random.default_rng(2)
+    mesh = (6, 4, 2, 1e-9, 1e-9, 1e-9)
+    m0 = rng.standard_normal((6, 4, 2, 3))
+    Hs = rng.standard_normal((6, 4, 2, 3)) * 1e5
+    a = O.Sim(mesh, 8e5, m0, [O.term(O.EXCHANGE, 1e-11), O.term(O.ZEEMAN, array0=O.to_flat(Hs))])
+    b = O.Sim(mesh, 8e5, m0, [O.term(O.EXCHANGE, 1e-11), O.term(O.ZEEMAN, callback=lambda t, m: O.to_flat(Hs))])
+    for sim in (a, b):
+        sim.llg_init(0.1, "DP45")
+    for _ in range(10):
+        assert a.step()["h"] == b.step()["h"]
+    assert np.array_equal(a.m_flat(), b.m_flat())
+
+
+# ---------------------------------------------------------------- spatial parameter fields (dmi, uniaxial)
+def test_dmi_D_field_uniform_and_chain():
+    """DmiField(D_constants) (dmi_field.cpp:17-19,97-109): a uniform D field equals the scalar D to
+    the last bit, and on a 3-cell chain along x with m = (sin q x, 0, cos q x) each cell's field is
+    2 D_i / (mu0 Ms) [(div m) n - grad(n.m)] with the zero-padded central differences."""
+    rng = np.random.default_rng(4)
+    mesh = (5, 4, 3, 1e-9, 2e-9, 3e-9)
+    m = rng.standard_normal((5, 4, 3, 3))
+    a = O.Sim(mesh, 9e5, m, [O.term(O.DMI, 2e-3, axis=(0, 0, 1))])
+    b = O.Sim(mesh, 9e5, m, [O.term(O.DMI, 0.0, axis=(0, 0, 1), array0=np.full(60, 2e-3))])
+    assert np.array_equal(a.heff_flat(), b.heff_flat())
+    assert a.energy()[0] == b.energy()[0]
+    dx, Ms = 1e-9, 8e5
+    q = 0.4
+    xs = np.arange(3)
+    m = np.zeros((3, 1, 1, 3))
+    m[:, 0, 0, 0] = np.sin(q * xs)
+    m[:, 0, 0, 2] = np.cos(q * xs)
+    D = np.array([1e-3, 2e-3, 4e-3])
+    s = O.Sim((3, 1, 1, dx, dx, dx), Ms, m, [O.term(O.DMI, 0.0, axis=(0, 0, 1), array0=D)])
+    h = s.heff()[:, 0, 0, :]
+    mx, mz = m[:, 0, 0, 0], m[:, 0, 0, 2]
+    pad = lambda v: np.concatenate([[0.0], v, [0.0]])  # noqa: E731
+    dmx = (pad(mx)[2:] - pad(mx)[:-2]) / (2 * dx)
+    dmz = (pad(mz)[2:] - pad(mz)[:-2]) / (2 * dx)
+    c = 2 * D / (MU0 * Ms)
+    ref = np.stack([c * (-dmz), np.zeros(3), c * dmx], axis=-1)
+    assert np.max(np.abs(h - ref)) <= 1e-15 * np.max(np.abs(ref))
+
+
+def test_uniaxial_axis_field_reference_answer():
+    """python/test/micro_anisotropy.py:36-54: Ku1 = 1, Ms = 1, axis field (1,0,0) and (0,-1,0) on a
+    2-cell mesh, m = (1,0,0) and (0,1,0): H = 2/mu0 along the cell's axis direction projected on m."""
+    ax = np.zeros((2, 1, 1, 3))
+    ax[0, 0, 0, 0] = 1.0
+    ax[1, 0, 0, 1] = -1.0
+    m = np.zeros((2, 1, 1, 3))
+    m[0, 0, 0, 0] = 1.0
+    m[1, 0, 0, 1] = 1.0
+    s = O.Sim((2, 1, 1, 1, 1, 1), 1.0, m, [O.term(O.UNIAXIAL, 1.0, array1=O.to_flat(ax))])
+    h = s.heff()
+    assert h[0, 0, 0, 0] == 2 / MU0 and h[0, 0, 0, 1] == 0 and h[0, 0, 0, 2] == 0
+    assert h[1, 0, 0, 0] == 0 and h[1, 0, 0, 1] == 2 / MU0 and h[1, 0, 0, 2] == 0
+    # Ku1 field: E = -sum Ku1_i (u_i . m_i)^2 dV (micro_term.hpp:18-30 with this field)
+    rng = np.random.default_rng(9)
+    K = rng.uniform(1e5, 9e5, (2, 1, 1))
+    m = rng.standard_normal((2, 1, 1, 3))
+    axr = rng.standard_normal((2, 1, 1, 3))
+    s = O.Sim((2, 1, 1, 2e-9, 2e-9, 2e-9), 8e5, m,
+              [O.term(O.UNIAXIAL, 0.0, array0=O.scalar_flat(K), array1=O.to_flat(axr))])
+    mu = m / np.linalg.norm(m, axis=-1, keepdims=True)
+    u = axr / np.linalg.norm(axr, axis=-1, keepdims=True)
+    E_ref = -np.sum(K * np.sum(u * mu, axis=-1) ** 2) * 8e-27
+    assert abs(s.energy()[0] - E_ref) <= 1e-14 * abs(E_ref)
+
+
+# ---------------------------------------------------------------- LBFGS: closed-form minimum and line searches
+def _sw_minimum(K, Ms, H, psi, theta_start):
+    """Stoner-Wohlfarth particle: E(theta)/V = -K cos^2(theta) - mu0 Ms H cos(theta - psi), theta the
+    angle of m from the easy axis z in the xz plane that holds the field.  Returns the minimum of the
+    basin of theta_start by bisection on dE/dtheta."""
+    dE = lambda th: 2 * K * math.sin(th) * math.cos(th) + MU0 * Ms * H * math.sin(th - psi)  # noqa: E731
+    lo, hi = theta_start, theta_start
+    step = 1e-3 if dE(theta_start) < 0 else -1e-3
+    while dE(hi) * dE(lo) > 0:  # walk downhill until dE changes sign
+        lo, hi = hi, hi + step
+    a, b = min(lo, hi), max(lo, hi)
+    for _ in range(200):
+        c = 0.5 * (a + b)
+        if dE(a) * dE(c) <= 0:
+            b = c
+        else:
+            a = c
+    th = 0.5 * (a + b)
+    return th, -K * math.cos(th) ** 2 - MU0 * Ms * H * math.cos(th - psi)
+
+
+@pytest.mark.parametrize("psi_deg,theta0,h", [(150.0, 0.3, 0.4), (45.0, 2.8, 0.3), (100.0, 1.0, 1.5)])
+def test_lbfgs_stoner_wohlfarth(psi_deg, theta0, h):
+    """LBFGS_Minimizer (lbfgs_minimizer.cpp:55-505) on one spin with uniaxial anisotropy and a field at
+    an angle psi to the easy axis: it stops at the closed-form minimum of the starting basin (the
+    metastable one for psi = 150 deg below the switching field), every line search ends with the
+    sufficient-decrease and curvature tests met (cvsrch info 1, :213-360), and the energy falls at
+    every iteration."""
+    K, Ms, d = 5e5, 8e5, 2e-9
+    psi = math.radians(psi_deg)
+    H = h * 2 * K / (MU0 * Ms)
+    m0 = np.array([math.sin(theta0), 0.05, math.cos(theta0)]).reshape(1, 1, 1, 3)
+    terms = [O.term(O.UNIAXIAL, K), O.term(O.ZEEMAN, field=(H * math.sin(psi), 0.0, H * math.cos(psi)))]
+    s = O.Sim((1, 1, 1, d, d, d), Ms, m0, terms)
+    E, rows = s.lbfgs_trace(tol=1e-14, maxiter=500)
+    th_ref, e_ref = _sw_minimum(K, Ms, H, psi, theta0)
+    m = s.m()[0, 0, 0]
+    th = math.atan2(m[0], m[2])
+    assert abs(m[1]) < 1e-6 and abs(th - th_ref) < 1e-6, (th, th_ref, m)
+    assert abs(E - e_ref * d ** 3) <= 1e-12 * abs(e_ref * d ** 3)
+    assert len(rows) >= 2
+    # the first search starts from step 1 on the raw gradient (H0 = 1) and may use all 20
+    # evaluations (info 3); every later one meets its tests
+    assert np.all(rows[1:, 5] == 1), rows[:, 5]
+    _check_line_searches(rows[1:], tol=1e-14)
+    assert np.all(rows[:, 3] <= rows[:, 0] + 1e-14 * np.abs(rows[:, 0])) and np.all(rows[:, 1] < 0)
+
+
+def _check_line_searches(r, tol):
+    """Each returned step of cvsrch with info 1 meets the curvature test |g.s| <= gtol |g(0).s| and
+    either the sufficient-decrease test f <= f(0) + ftol stp g(0).s or its approximate form
+    f <= f(0) + tol |f(0)| with g.s <= (2 ftol - 1) g(0).s (lbfgs_minimizer.cpp:300-315; ftol = 1e-4,
+    gtol = 0.9).  The gradient is m x (m x H) in A/m, not dE/dm in J, so g.s is in (A/m)^2 and the
+    exact test, f(0) + ftol stp g(0).s in J plus (A/m)^2, never holds for energies of 1e-20 J: the
+    reference's searches all end through the approximate form, which these rows show."""
+    finit, dginit, stp, f, dg = r[:, 0], r[:, 1], r[:, 2], r[:, 3], r[:, 4]
+    assert np.all(np.abs(dg) <= 0.9 * np.abs(dginit))
+    armijo = f <= finit + 1e-4 * stp * dginit
+    approx = (f <= finit + tol * np.abs(finit)) & ((2e-4 - 1) * dginit >= dg)
+    assert np.all(armijo | approx), np.nonzero(~(armijo | approx))
+
+
+def test_lbfgs_line_searches_on_a_film():
+    """The same line-search properties on an 8 x 8 film with exchange, DMI, anisotropy and a field
+    (C4's term set without demag): energies fall from line search to line search, and every search
+    that returns info 1 returns a point with f <= f(0) + 1e-4 stp g(0).s and |g.s| <= 0.9 |g(0).s|."""
+    rng = np.random.default_rng(8)
+    mesh = (8, 8, 1, 3e-9, 3e-9, 3e-9)
+    m = rng.standard_normal((8, 8, 1, 3))
+    terms = [O.term(O.EXCHANGE, 1.5e-11), O.term(O.DMI, 2.5e-3), O.term(O.UNIAXIAL, 6e5),
+             O.term(O.ZEEMAN, field=(0, 0, 0.2 / MU0))]
+    s = O.Sim(mesh, 9e5, m, terms)
+    E, rows = s.lbfgs_trace(tol=1e-12, maxiter=300)
+    assert len(rows) > 20
+    assert np.all(np.diff(rows[:, 3]) <= 1e-14 * np.abs(rows[1:, 3]))
+    ok = rows[:, 5] == 1
+    assert ok.mean() > 0.9
+    _check_line_searches(rows[ok], tol=1e-12)
