@@ -41,8 +41,10 @@ typedef struct mafx_mesh {
 /* Arithmetic of the field evaluation.  MAFX_F64 is the plain FP64 path (the reference's
  * arithmetic).  MAFX_F32_FIELD runs the demag FFT, tensor product and stencils in FP32 with
  * FP32 accumulation; the state m, the RK combinations, error norm, renormalisation and all
- * energy / dot-product sums stay FP64. */
-enum mafx_precision { MAFX_F64 = 0, MAFX_F32_FIELD = 1 };
+ * energy / dot-product sums stay FP64.  MAFX_F32_DEMAG runs only the demag (or dipole) FFT
+ * convolution in FP32 and the local terms in FP64; MAFX_F32_STENCILS the reverse (FP64 demag,
+ * FP32 local terms), which exists to separate the two sources of FP32 error. */
+enum mafx_precision { MAFX_F64 = 0, MAFX_F32_FIELD = 1, MAFX_F32_DEMAG = 2, MAFX_F32_STENCILS = 3 };
 
 /* Field terms.  Each kind replaces one reference FieldTerm subclass; the cited file holds
  * the impl_H_in_Apm / impl_E_in_J the GPU evaluation reproduces. */
@@ -94,13 +96,16 @@ typedef int (*mafx_field_fn)(void* user, double t, const double* m, double* h_ou
  *            cache and writes it there when absent, MAFX_TERM_VERBOSE reports cache activity on
  *            stdout (the caching and verbose arguments of DemagField(mesh, verbose, caching),
  *            demag_field.cpp:185-204, cache_manager.cpp:49-114).  MAFX_TERM_HOST_TENSOR (no
- *            counterpart in the reference, whose only tensor is this one): on the MAFX_F32_FIELD
- *            path, build the real-space tensor on the host in the reference's operation order,
- *            as the FP64 path does, instead of on the GPU; the setup takes seconds to minutes,
- *            and the field then carries the reference's far-field rounding noise.
- *            MAFX_TERM_DEVICE_TENSOR (no counterpart in the reference either): on the MAFX_F64
- *            path, use the real-space tensor the MAFX_F32_FIELD path builds on the GPU instead of
- *            the host-built one, so that the two paths can be compared on one tensor.
+ *            counterpart in the reference, whose only tensor is this one) and its companions choose
+ *            the real-space tensor the transform starts from.  By default the FP64 paths use the
+ *            reference's tensor over the whole expanded grid, evaluated on the host in the
+ *            reference's operation order (bit for bit, far-field rounding noise included), and the
+ *            FP32 paths the same evaluation on the non-negative octant extended by the components'
+ *            parities (an eighth of the host work; it differs from the whole tensor only by the
+ *            summation-order noise of the negative offsets).  MAFX_TERM_HOST_TENSOR selects the
+ *            whole host tensor, MAFX_TERM_OCTANT_TENSOR the extended octant, and
+ *            MAFX_TERM_DEVICE_TENSOR the same formulas evaluated on the GPU (milliseconds instead of
+ *            seconds, with the device libm's own far-field noise), on either path.
  *            ZEEMAN / ATOM_ZEEMAN: MAFX_TERM_HOMOGENEOUS_CALLBACK, the callback writes one
  *            homogeneous field into a host double[3]; MAFX_TERM_HOST_CALLBACK, the callback
  *            reads m from and writes the field into host buffers
@@ -111,6 +116,7 @@ typedef int (*mafx_field_fn)(void* user, double t, const double* m, double* h_ou
 #define MAFX_TERM_HOMOGENEOUS_CALLBACK 8
 #define MAFX_TERM_HOST_CALLBACK 16
 #define MAFX_TERM_DEVICE_TENSOR 32
+#define MAFX_TERM_OCTANT_TENSOR 64
 typedef struct mafx_term_desc {
     int32_t kind;
     int32_t flags;
@@ -269,10 +275,20 @@ int mafx_llg_get_controller_state(const mafx_llg* llg, double* h, double* errold
 typedef struct mafx_lbfgs_stats {
     int64_t iterations, energy_evals;
     double final_energy, final_torque_rel;
-    int32_t stop_reason; /* 0 maxiter, 1 gradient, 2 triple test, 3 torque, 4 no descent */
+    int32_t stop_reason; /* 0 maxiter, 1 gradient, 2 triple test, 3 torque, 4 no descent,
+                            5 stalled: five iterations in a row without an energy decrease
+                            (checked only when torque_tol > 0) */
 } mafx_lbfgs_stats;
+/* verbose > 0 prints "f= E" at the start and, when the loop ends through a stop test other than
+ * the gradient one or the iteration bound, the reference's closing "aa> iterations dE max|mxmxH|"
+ * line (lbfgs_minimizer.cpp:181-192); verbose > 1 one "bb>" line per iteration (:142-146). */
 int mafx_lbfgs_minimize(mafx_ctx* ctx, double* m_dev, double tolerance, int64_t max_iter,
                         int verbose, double torque_tol, double* E_out, mafx_lbfgs_stats* stats);
+/* LBFGS_Minimizer::of_convergence_ (lbfgs_minimizer.hpp:18, lbfgs_minimizer.cpp:148-153): while a
+ * file is set, every iteration of mafx_lbfgs_minimize on ctx writes the three stop ratios
+ * (f_old - f) / (tol (1 + |f|)), max|s| / (sqrt(tol) (1 + max|m|)), max|g| / (tol^(1/3) (1 + |f|)),
+ * tab-separated, one line per iteration.  The file is truncated when set; NULL closes it. */
+int mafx_lbfgs_set_convergence_file(mafx_ctx* ctx, const char* path);
 
 /* ------------------------------------------------------------------ learned warm start */
 /* No counterpart in the reference (magnum.af has no learned component).  A 4-level

@@ -197,6 +197,7 @@ void set_term(mafx_ctx* ctx, TermDev& t, const mafx_term_desc& d) {
 
 mafx_ctx::~mafx_ctx() {
     if (host_scal) (void)hipHostFree(host_scal);
+    if (lbfgs_convergence) std::fclose(lbfgs_convergence);
 }
 
 using namespace mafx;
@@ -217,9 +218,10 @@ int mafx_demag_cache_name(const mafx_mesh* mesh, int precision, int term_flags, 
                           int64_t buf_len) {
     return guarded([&] {
         require(mesh && buf && buf_len > 0, "mafx_demag_cache_name: null argument");
-        require(precision == MAFX_F64 || precision == MAFX_F32_FIELD, "unknown precision");
+        require(valid_precision(precision), "unknown precision");
         TensorCacheKey k{2 * mesh->nx, 2 * mesh->ny, mesh->nz == 1 ? 1 : 2 * mesh->nz, (mesh->nx + 1 + 15) / 16 * 16,
-                         tensor_cache_layout(precision, term_flags), dipole ? 1 : 0, mesh->dx, mesh->dy, mesh->dz};
+                         tensor_cache_layout(demag_f64(precision) ? MAFX_F64 : MAFX_F32_FIELD, term_flags),
+                         dipole ? 1 : 0, mesh->dx, mesh->dy, mesh->dz};
         const std::string s = tensor_cache_name(k);
         require(static_cast<int64_t>(s.size()) < buf_len, "mafx_demag_cache_name: buffer too small");
         std::memcpy(buf, s.c_str(), s.size() + 1);
@@ -248,7 +250,7 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
         require(mesh->nx > 0 && mesh->ny > 0 && mesh->nz > 0, "mesh sizes must be positive");
         require(mesh->dx > 0 && mesh->dy > 0 && mesh->dz > 0, "cell sizes must be positive");
         require(nterms >= 1 && nterms <= 8, "between 1 and 8 terms per context");
-        require(precision == MAFX_F64 || precision == MAFX_F32_FIELD, "unknown precision");
+        require(valid_precision(precision), "unknown precision");
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
             throw Error(MAFX_ERR_DEVICE, "no HIP device visible: the MI355X path needs a GPU");
@@ -305,10 +307,10 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
             if (ctx->n2 > 1) ctx->pz.build(static_cast<int>(ctx->n2));
             // the z transform is fused into the tensor product for 2 <= nz <= 8 (nz planes suffice)
             ctx->nzs = (mesh->nz >= 2 && mesh->nz <= 8) ? mesh->nz : ctx->n2;
-            const size_t csz = precision == MAFX_F64 ? sizeof(double2) : sizeof(float2);
+            const size_t csz = demag_f64(precision) ? sizeof(double2) : sizeof(float2);
             ctx->spec.alloc(csz * 3 * ctx->nzs * ctx->n1 * ctx->hxs);
             MAFX_HIP(hipMemset(ctx->spec.p, 0, ctx->spec.bytes));
-            ctx->hdem.alloc((precision == MAFX_F64 ? sizeof(double) : sizeof(float)) * 3 * N);
+            ctx->hdem.alloc((demag_f64(precision) ? sizeof(double) : sizeof(float)) * 3 * N);
             build_tensor(ctx.get());
         }
         ctx->device_bytes = static_cast<int64_t>(ctx->ms_field.bytes + ctx->partials.bytes + ctx->scal.bytes +
@@ -586,6 +588,18 @@ int mafx_lbfgs_minimize(mafx_ctx* ctx, double* m_dev, double tolerance, int64_t 
     int rc = MAFX_OK;
     const int g = guarded([&] { rc = lbfgs_minimize(ctx, m_dev, tolerance, max_iter, verbose, torque_tol, E_out, stats); });
     return g != MAFX_OK ? g : rc;
+}
+
+int mafx_lbfgs_set_convergence_file(mafx_ctx* ctx, const char* path) {
+    return guarded([&] {
+        require(ctx, "null context");
+        if (ctx->lbfgs_convergence) std::fclose(ctx->lbfgs_convergence);
+        ctx->lbfgs_convergence = nullptr;
+        if (path) {
+            ctx->lbfgs_convergence = std::fopen(path, "w");
+            require(ctx->lbfgs_convergence != nullptr, std::string("cannot open ") + path);
+        }
+    });
 }
 
 int mafx_net_create(void* stream, mafx_net** out) {

@@ -450,7 +450,7 @@ template <bool RED> __device__ void finish_block(const EpiParams& P, double* acc
 // tile are read behind the same L2; otherwise cells are taken in memory order.
 // MINW is the waves per SIMD the register budget must allow: 5 (96 VGPRs, no spills) for the
 // FP32 field and LLG evaluations, which run 7 times per RK attempt; 1 (uncapped) otherwise.
-template <class T, bool RED, bool ROWS, int MINW, uint64_t PROG = 0>
+template <class T, class TH, bool RED, bool ROWS, int MINW, uint64_t PROG = 0>
 __global__ void __launch_bounds__(kBlock, MINW) k_epi_cells(EpiParams P, FastDiv nxd, FastDiv nyd, FastDiv nzd,
                                                             FastDiv bprd) {
     double acc[kMaxTerms + 2];
@@ -476,11 +476,11 @@ __global__ void __launch_bounds__(kBlock, MINW) k_epi_cells(EpiParams P, FastDiv
     if (valid) {
         T hd[3] = {0, 0, 0};
         if (P.hd) {
-            const T* h = static_cast<const T*>(P.hd);
-            const uint32_t ob = static_cast<uint32_t>(x + nx * (y + ny * z)) * static_cast<uint32_t>(sizeof(T));
-            hd[0] = ldb(h, ob);
-            hd[1] = ldb(h + P.N, ob);
-            hd[2] = ldb(h + 2 * P.N, ob);
+            const TH* h = static_cast<const TH*>(P.hd);
+            const uint32_t ob = static_cast<uint32_t>(x + nx * (y + ny * z)) * static_cast<uint32_t>(sizeof(TH));
+            hd[0] = static_cast<T>(ldb(h, ob));
+            hd[1] = static_cast<T>(ldb(h + P.N, ob));
+            hd[2] = static_cast<T>(ldb(h + 2 * P.N, ob));
         }
         cell_epilogue<T, RED, PROG>(P, x, y, z, hd, acc);
     }
@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(kBlock, MINW) k_epi_cells(EpiParams P, FastDiv
 // group's cells at the same z (one halo cell each side, loaded by the edge lanes), so only the
 // centre, the next z and the two y neighbours are read from memory, 9 loads per cell instead
 // of 21.  Zero padding as in cell_epilogue.
-template <class T, uint64_t PROG, int MINW>
+template <class T, class TH, uint64_t PROG, int MINW>
 __global__ void __launch_bounds__(kBlock, MINW) k_epi_zcol(EpiParams P, FastDiv bprd) {
     __shared__ V3<T> row[kBlock + 2];
     const int nx = static_cast<int>(P.nx), ny = static_cast<int>(P.ny), nz = static_cast<int>(P.nz);
@@ -526,11 +526,11 @@ __global__ void __launch_bounds__(kBlock, MINW) k_epi_zcol(EpiParams P, FastDiv 
         if (hr) ld3(o + 8u, hv);
         T hd[3] = {0, 0, 0};
         if (P.hd) {
-            const T* h = static_cast<const T*>(P.hd);
-            const uint32_t ob = (o / 8u) * static_cast<uint32_t>(sizeof(T));
-            hd[0] = ldb(h, ob);
-            hd[1] = ldb(h + P.N, ob);
-            hd[2] = ldb(h + 2 * P.N, ob);
+            const TH* h = static_cast<const TH*>(P.hd);
+            const uint32_t ob = (o / 8u) * static_cast<uint32_t>(sizeof(TH));
+            hd[0] = static_cast<T>(ldb(h, ob));
+            hd[1] = static_cast<T>(ldb(h + P.N, ob));
+            hd[2] = static_cast<T>(ldb(h + 2 * P.N, ob));
         }
         __syncthreads(); // the previous z's row readers are done
         row[t + 1] = valid ? to_t(cur) : V3<T>{0, 0, 0};
@@ -640,7 +640,9 @@ EpiParams make_params(mafx_ctx* ctx, const EvalArgs& a, int64_t nblocks) {
     return P;
 }
 
-template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a) {
+// C: complex type of the FFT convolution; TH: its field between the inverse x transform and the
+// per-cell kernel; T: arithmetic of the local terms (stencils) in the per-cell kernel.
+template <class C, class TH, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a) {
     const mafx_mesh& g = ctx->mesh;
     const bool fft = ctx->demag_term >= 0 && (a.mask & (1u << ctx->demag_term));
     // cell and row indices are 32-bit inside the kernels, and the per-cell kernel addresses a
@@ -690,20 +692,20 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
         fft_y_split<C>(ctx, spec, sa, 3, n1 * hxs, static_cast<int>(g.nz), kTile, true);
         const double P_total = static_cast<double>(ctx->n0) * n1 * n2;
         const double hscale = (ctx->atom_dipole ? ctx->Ms : 1.0) / P_total;
-        T* hd = ctx->hdem.as<T>();
-        fft_x_inverse<C, T>(ctx, spec, nzs, hd, hscale);
+        TH* hd = ctx->hdem.as<TH>();
+        fft_x_inverse<C, TH>(ctx, spec, nzs, hd, hscale);
         P.hd = hd;
     }
     {
         // per-cell kernel: local terms, plus the FFT field read back from P.hd
-        const double hd_bytes = P.hd ? 3.0 * sizeof(T) : 0.0;
+        const double hd_bytes = P.hd ? 3.0 * sizeof(TH) : 0.0;
         ProfScope ps(ctx, P.hd ? (sizeof(T) == 4 ? "epilogue_cells_f32" : "epilogue_cells_f64")
                                : (sizeof(T) == 4 ? "local_cells_f32" : "local_cells_f64"),
                      static_cast<double>(ctx->N) * (24.0 + field_bytes + hd_bytes));
         const bool red = P.want_energy || P.mode == static_cast<int>(OutMode::GRAD);
         constexpr int MW = sizeof(T) == 4 ? 5 : 1;
-        auto kern = red ? (rowmap ? k_epi_cells<T, true, true, 1> : k_epi_cells<T, true, false, 1>)
-                        : (rowmap ? k_epi_cells<T, false, true, MW> : k_epi_cells<T, false, false, MW>);
+        auto kern = red ? (rowmap ? k_epi_cells<T, TH, true, true, 1> : k_epi_cells<T, TH, true, false, 1>)
+                        : (rowmap ? k_epi_cells<T, TH, false, true, MW> : k_epi_cells<T, TH, false, false, MW>);
         // compile-time term programs of the common micromagnetic term lists (field and LLG
         // evaluations); any other list runs the run-time term loop
         if (!red) {
@@ -713,10 +715,10 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
             constexpr uint64_t kP2 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_ZEEMAN});
             constexpr uint64_t kP3 = prog_of({MAFX_DEMAG, MAFX_RKKY, MAFX_DMI, MAFX_UNIAXIAL, MAFX_ZEEMAN});
             constexpr uint64_t kP4 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_DMI, MAFX_UNIAXIAL, MAFX_ZEEMAN});
-            if (prog == kP1) kern = rowmap ? k_epi_cells<T, false, true, MW, kP1> : k_epi_cells<T, false, false, MW, kP1>;
-            if (prog == kP2) kern = rowmap ? k_epi_cells<T, false, true, MW, kP2> : k_epi_cells<T, false, false, MW, kP2>;
-            if (prog == kP3) kern = rowmap ? k_epi_cells<T, false, true, MW, kP3> : k_epi_cells<T, false, false, MW, kP3>;
-            if (prog == kP4) kern = rowmap ? k_epi_cells<T, false, true, MW, kP4> : k_epi_cells<T, false, false, MW, kP4>;
+            if (prog == kP1) kern = rowmap ? k_epi_cells<T, TH, false, true, MW, kP1> : k_epi_cells<T, TH, false, false, MW, kP1>;
+            if (prog == kP2) kern = rowmap ? k_epi_cells<T, TH, false, true, MW, kP2> : k_epi_cells<T, TH, false, false, MW, kP2>;
+            if (prog == kP3) kern = rowmap ? k_epi_cells<T, TH, false, true, MW, kP3> : k_epi_cells<T, TH, false, false, MW, kP3>;
+            if (prog == kP4) kern = rowmap ? k_epi_cells<T, TH, false, true, MW, kP4> : k_epi_cells<T, TH, false, false, MW, kP4>;
             // z-column lanes for rows of whole work-groups when a term reads the neighbours
             if (rowmap && P.needs_nb && g.nx % kBlock == 0) {
                 using KZ = void (*)(EpiParams, FastDiv);
@@ -725,11 +727,11 @@ template <class C, class T> void evaluate_impl(mafx_ctx* ctx, const EvalArgs& a)
                 // the other programs fit 80 VGPRs without spills: 6 waves per SIMD, measured
                 // 64.4 -> 61.6 us at 1024 x 1024 x 4 against 5 (8 waves spill)
                 constexpr int MWZ = sizeof(T) == 4 ? 6 : 1;
-                KZ kz = k_epi_zcol<T, 0, MWW>;
-                if (prog == kP1) kz = k_epi_zcol<T, kP1, MWZ>;
-                if (prog == kP2) kz = k_epi_zcol<T, kP2, MWZ>;
-                if (prog == kP3) kz = k_epi_zcol<T, kP3, MWW>;
-                if (prog == kP4) kz = k_epi_zcol<T, kP4, MWZ>;
+                KZ kz = k_epi_zcol<T, TH, 0, MWW>;
+                if (prog == kP1) kz = k_epi_zcol<T, TH, kP1, MWZ>;
+                if (prog == kP2) kz = k_epi_zcol<T, TH, kP2, MWZ>;
+                if (prog == kP3) kz = k_epi_zcol<T, TH, kP3, MWW>;
+                if (prog == kP4) kz = k_epi_zcol<T, TH, kP4, MWZ>;
                 const int64_t ng = g.ny * bpr;
                 hipLaunchKernelGGL(kz, dim3(static_cast<unsigned>(ng)), dim3(kBlock), 0, ctx->stream, P,
                                    make_fastdiv(static_cast<uint32_t>(bpr)));
@@ -776,10 +778,13 @@ void evaluate(mafx_ctx* ctx, const EvalArgs& a) {
         }
         require(rc == 0, "external field callback failed (returned " + std::to_string(rc) + ")");
     }
-    if (ctx->precision == MAFX_F64)
-        evaluate_impl<double2, double>(ctx, a);
-    else
-        evaluate_impl<float2, float>(ctx, a);
+    switch (ctx->precision) {
+    case MAFX_F64: evaluate_impl<double2, double, double>(ctx, a); break;
+    case MAFX_F32_FIELD: evaluate_impl<float2, float, float>(ctx, a); break;
+    case MAFX_F32_DEMAG: evaluate_impl<float2, float, double>(ctx, a); break;
+    case MAFX_F32_STENCILS: evaluate_impl<double2, double, float>(ctx, a); break;
+    default: throw Error(MAFX_ERR_ARG, "unknown precision");
+    }
 }
 
 // Reduces the partials of the last evaluate() (energy slots and, in gradient mode, the two
@@ -956,12 +961,13 @@ void build_tensor(mafx_ctx* ctx) {
     ctx->n2h = n2 / 2 + 1;
     const bool caching = (ctx->demag_flags & MAFX_TERM_CACHING) != 0;
     const bool verbose = (ctx->demag_flags & MAFX_TERM_VERBOSE) != 0;
-    const uint64_t stored = ctx->precision == MAFX_F64
+    const uint64_t stored = demag_f64(ctx->precision)
                                 ? sizeof(double2) * 6 * static_cast<uint64_t>(K)
                                 : sizeof(float) * 6 * static_cast<uint64_t>(ctx->n2h * ctx->n1h * hxs);
     // the FP32 layout from the host-built and the FP64 layout from the GPU-built real-space tensor
     // are stored under keys of their own
-    const TensorCacheKey key{n0, n1, n2, hxs, tensor_cache_layout(ctx->precision, ctx->demag_flags),
+    const TensorCacheKey key{n0, n1, n2, hxs, tensor_cache_layout(demag_f64(ctx->precision) ? MAFX_F64 : MAFX_F32_FIELD,
+                                                                   ctx->demag_flags),
                              ctx->atom_dipole ? 1 : 0, ctx->mesh.dx, ctx->mesh.dy, ctx->mesh.dz};
     if (caching) {
         std::vector<char> host;
@@ -988,18 +994,28 @@ void build_tensor_uncached(mafx_ctx* ctx) {
     const int64_t P = n0 * n1 * n2, K = n2 * n1 * hxs;
     DevBuf Nd, T64;
     Nd.alloc(sizeof(double) * 6 * P);
-    const bool on_host = ctx->precision == MAFX_F64 ? !(ctx->demag_flags & MAFX_TERM_DEVICE_TENSOR)
-                                                    : (ctx->demag_flags & MAFX_TERM_HOST_TENSOR) != 0;
-    if (on_host) {
-        auto Nh = host_tensor(ctx->mesh, ctx->atom_dipole, P);
-        MAFX_HIP(hipMemcpy(Nd.p, Nh->data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
-    } else {
-        const mafx_mesh& g = ctx->mesh;
+    const bool f64 = demag_f64(ctx->precision);
+    const mafx_mesh& g = ctx->mesh;
+    if (ctx->demag_flags & MAFX_TERM_DEVICE_TENSOR) {
+        // the formulas evaluated on the GPU (fast; its own far-field rounding noise)
         hipLaunchKernelGGL(k_real_tensor, dim3(grid_for(P, 256)), dim3(256), 0, ctx->stream, Nd.as<double>(), n0, n1,
                            n2, g.dx, g.dy, g.dz, ctx->atom_dipole ? 1 : 0);
         MAFX_HIP(hipGetLastError());
+    } else if (ctx->atom_dipole || (ctx->demag_flags & MAFX_TERM_HOST_TENSOR) ||
+               (f64 && !(ctx->demag_flags & MAFX_TERM_OCTANT_TENSOR))) {
+        // the reference's tensor, bit for bit, over the whole expanded grid
+        auto Nh = host_tensor(ctx->mesh, ctx->atom_dipole, P);
+        MAFX_HIP(hipMemcpy(Nd.p, Nh->data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
+    } else {
+        // the reference's tensor on the non-negative octant, extended by parity (an eighth of the
+        // host evaluations; the FP32 path's symmetric spectrum needs no more)
+        std::vector<double> oct(static_cast<size_t>(6 * g.nx * g.ny * g.nz));
+        newell_octant_host(g, oct.data(), 0);
+        std::vector<double> full(static_cast<size_t>(6 * P));
+        newell_octant_extend(g, oct.data(), full.data());
+        MAFX_HIP(hipMemcpy(Nd.p, full.data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
     }
-    if (ctx->precision != MAFX_F64) zero_unpaired_planes(ctx, Nd.as<double>(), 6);
+    if (!f64) zero_unpaired_planes(ctx, Nd.as<double>(), 6);
     T64.alloc(sizeof(double2) * 6 * K);
     MAFX_HIP(hipMemset(T64.p, 0, T64.bytes)); // padding columns stay defined
     double2* t = T64.as<double2>();
@@ -1012,7 +1028,7 @@ void build_tensor_uncached(mafx_ctx* ctx) {
                              static_cast<int>(n2), false, "tensor_fft_z");
     MAFX_HIP(hipStreamSynchronize(ctx->stream));
     Nd.release();
-    if (ctx->precision == MAFX_F64) {
+    if (f64) {
         std::swap(ctx->tensor.p, T64.p);
         std::swap(ctx->tensor.bytes, T64.bytes);
     } else {

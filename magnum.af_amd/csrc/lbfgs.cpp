@@ -284,6 +284,7 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
     const double tolf2 = std::sqrt(tolerance);
     const double tolf3 = std::pow(tolerance, 0.3333333333333333333333333);
     int reason = 0;
+    int stalled = 0; // consecutive iterations without an energy decrease (torque_tol > 0 only)
     double f = L.energy_and_gradient(m, grad.as<double>());
     double gradNorm = L.maxabs(grad.as<double>());
     size_t globIter = 0;
@@ -355,8 +356,15 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
             break;
         }
         sub(ctx, s.as<double>(), m, x_old.as<double>(), n3);
-        if (((f_old - f) < (tolerance * f1)) && (L.maxabs(s.as<double>()) < (tolf2 * (1 + L.maxabs(m)))) &&
-            (gradNorm <= (tolf3 * f1))) {
+        const double smax = L.maxabs(s.as<double>());
+        const double mmax = L.maxabs(m);
+        // lbfgs_minimizer.cpp:148-153: the three stop ratios, one line per iteration
+        if (ctx->lbfgs_convergence) {
+            std::fprintf(ctx->lbfgs_convergence, "%.27g\t%.27g\t%.27g\n", (f_old - f) / (tolerance * f1),
+                         smax / (tolf2 * (1 + mmax)), gradNorm / (tolf3 * f1));
+            std::fflush(ctx->lbfgs_convergence);
+        }
+        if (((f_old - f) < (tolerance * f1)) && (smax < (tolf2 * (1 + mmax))) && (gradNorm <= (tolf3 * f1))) {
             globIter++;
             reason = 2;
             break;
@@ -364,6 +372,15 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         if (torque_tol > 0 && L.last_hmax > 0 && L.last_torque / L.last_hmax <= torque_tol) {
             globIter++;
             reason = 3;
+            break;
+        }
+        // with the torque stop, an iteration that cannot lower the energy any more (its change is
+        // below the rounding of the energy sum, as on the FP32-field path near the minimum) five
+        // times in a row ends the run instead of spending the remaining iterations on it
+        stalled = (torque_tol > 0 && f >= f_old) ? stalled + 1 : 0;
+        if (stalled >= 5) {
+            globIter++;
+            reason = 5;
             break;
         }
         sub(ctx, y.as<double>(), grad.as<double>(), grad_old.as<double>(), n3);
@@ -394,6 +411,20 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         }
         globIter++;
     } while (globIter < static_cast<size_t>(max_iter));
+    // lbfgs_minimizer.cpp:181-192: the closing report after the loop (the gradient stop returns
+    // before it); its max |m x (m x H)| is evaluated, as there, on a state whose m is the last gradient
+    if (globIter >= static_cast<size_t>(max_iter) && max_iter > 99 && verbose > 0)
+        std::printf("WARNING : maximum number of iterations exceeded in LBFGS\n");
+    if (verbose > 0 && reason != 1) {
+        EvalArgs ea;
+        ea.m = grad.as<double>();
+        ea.mode = OutMode::GRAD;
+        ea.out = q.as<double>();
+        evaluate(ctx, ea);
+        const double mxh = L.maxabs(q.as<double>());
+        std::printf("aa> %zu %.27g  %.27g\n", globIter, f - f_old, mxh);
+        std::fflush(stdout);
+    }
     MAFX_HIP(hipStreamSynchronize(ctx->stream));
     return finish(reason);
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -121,6 +122,13 @@ void set_term(mafx_ctx* ctx, TermDev& t, const mafx_term_desc& desc);
 
 struct Workspace; // precision-specific demag buffers
 
+// The demag / dipole FFT convolution runs in FP64 (complex reference tensor) for MAFX_F64 and
+// MAFX_F32_STENCILS, in FP32 (symmetric real tensor) for MAFX_F32_FIELD and MAFX_F32_DEMAG.
+inline bool demag_f64(int precision) { return precision == MAFX_F64 || precision == MAFX_F32_STENCILS; }
+inline bool valid_precision(int p) {
+    return p == MAFX_F64 || p == MAFX_F32_FIELD || p == MAFX_F32_DEMAG || p == MAFX_F32_STENCILS;
+}
+
 } // namespace mafx
 
 struct mafx_ctx {
@@ -159,6 +167,7 @@ struct mafx_ctx {
     double last_max_torque{0}, last_max_h{0}; // max|m x H|, max|H| of the last gradient evaluation
     mafx::DevBuf vtmp;        // 3N f64 scratch for vector helpers
     hipStream_t stream{nullptr};
+    std::FILE* lbfgs_convergence{nullptr}; // LBFGS_Minimizer::of_convergence_ (mafx_lbfgs_set_convergence_file)
     mafx::Prof prof;
     int64_t device_bytes{0};
     ~mafx_ctx();
@@ -169,6 +178,10 @@ namespace mafx {
 // ------------------------------------------------------------------ host tensor setup (newell_host.cpp)
 // Real-space Newell tensor [6][n2][n1][n0], reference operation order (demag_field.cpp:20-142).
 void newell_tensor_host(const mafx_mesh& mesh, double* N, int nthreads);
+// Its non-negative octant [6][nz][ny][nx] in the same order, and the octant's extension by parity to
+// the expanded grid [6][n2][n1][n0] (unpaired planes zero), the FP32 path's real-space tensor.
+void newell_octant_host(const mafx_mesh& mesh, double* O, int nthreads);
+void newell_octant_extend(const mafx_mesh& mesh, const double* O, double* N);
 // Point-dipole tensor of AtomisticDipoleDipoleField (atomistic_dipole_dipole_field.cpp:55-113), x 1/(4 pi).
 void dipole_tensor_host(const mafx_mesh& mesh, double* N);
 

@@ -124,6 +124,67 @@ void newell_tensor_host(const mafx_mesh& mesh, double* N, int nthreads) {
     for (auto& t : th) t.join();
 }
 
+// The non-negative octant of the same tensor, [6][m2][m1][m0] with m0 = nx, m1 = ny, m2 = nz (or
+// 1 for a film), entries at offsets j >= 0 in the same operation order; and its extension to the
+// expanded grid by the parities of the components (xx, yy, zz even; xy odd in x and y; xz odd in x
+// and z; yz odd in y and z), with the planes j = -n/2, which have no mirror, set to zero.  The
+// extension differs from newell_tensor_host only at negative offsets, by the summation order of
+// the 27 terms (|delta| ~ 1e-10 of an entry); it costs an eighth of the evaluations.
+void newell_octant_host(const mafx_mesh& mesh, double* O, int nthreads) {
+    const int64_t m0 = mesh.nx, m1 = mesh.ny, m2 = mesh.nz;
+    const int64_t Q = m0 * m1 * m2;
+    const double dx = mesh.dx, dy = mesh.dy, dz = mesh.dz;
+    auto work = [&](int64_t b, int64_t e) {
+        for (int64_t j0 = b; j0 < e; ++j0)
+            for (int64_t j1 = 0; j1 < m1; ++j1)
+                for (int64_t j2 = 0; j2 < m2; ++j2) {
+                    const int a = static_cast<int>(j0), c = static_cast<int>(j1), d = static_cast<int>(j2);
+                    const int64_t at = j0 + m0 * (j1 + m1 * j2);
+                    O[0 * Q + at] = second_diff<nf>(a, c, d, dx, dy, dz);
+                    O[1 * Q + at] = second_diff<ng>(a, c, d, dx, dy, dz);
+                    O[2 * Q + at] = second_diff<ng>(a, d, c, dx, dz, dy);
+                    O[3 * Q + at] = second_diff<nf>(c, d, a, dy, dz, dx);
+                    O[4 * Q + at] = second_diff<ng>(c, d, a, dy, dz, dx);
+                    O[5 * Q + at] = second_diff<nf>(d, a, c, dz, dx, dy);
+                }
+    };
+    if (nthreads < 1) nthreads = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+    nthreads = static_cast<int>(std::min<int64_t>(nthreads, m0));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) th.emplace_back(work, m0 * t / nthreads, m0 * (t + 1) / nthreads);
+    for (auto& t : th) t.join();
+}
+
+void newell_octant_extend(const mafx_mesh& mesh, const double* O, double* N) {
+    const int64_t n0 = exp_xy(mesh.nx), n1 = exp_xy(mesh.ny), n2 = exp_z(mesh.nz);
+    const int64_t m0 = mesh.nx, m1 = mesh.ny, m2 = mesh.nz;
+    const int64_t P = n0 * n1 * n2, Q = m0 * m1 * m2;
+    // odd[c][axis]: component c changes sign with the offset along axis
+    static const bool odd[6][3] = {{false, false, false}, {true, true, false}, {true, false, true},
+                                   {false, false, false}, {false, true, true}, {false, false, false}};
+    for (int64_t i2 = 0; i2 < n2; ++i2) {
+        const int64_t j2 = (i2 + n2 / 2) % n2 - n2 / 2;
+        for (int64_t i1 = 0; i1 < n1; ++i1) {
+            const int64_t j1 = (i1 + n1 / 2) % n1 - n1 / 2;
+            for (int64_t i0 = 0; i0 < n0; ++i0) {
+                const int64_t j0 = (i0 + n0 / 2) % n0 - n0 / 2;
+                const int64_t at = i0 + n0 * (i1 + n1 * i2);
+                const bool unpaired = (n0 > 1 && j0 == -n0 / 2) || (n1 > 1 && j1 == -n1 / 2) || (n2 > 1 && j2 == -n2 / 2);
+                const int64_t src = (j0 < 0 ? -j0 : j0) + m0 * ((j1 < 0 ? -j1 : j1) + m1 * (j2 < 0 ? -j2 : j2));
+                for (int c = 0; c < 6; ++c) {
+                    if (unpaired) {
+                        N[c * P + at] = 0.0;
+                        continue;
+                    }
+                    const bool neg = (odd[c][0] && j0 < 0) != (odd[c][1] && j1 < 0) != (odd[c][2] && j2 < 0);
+                    const double v = O[c * Q + src];
+                    N[c * P + at] = neg ? -v : v;
+                }
+            }
+        }
+    }
+}
+
 // atomistic_dipole_dipole_field.cpp:55-113: N = (3 r r^T / r^5 - I / r^3) / (4 pi), N(0) = 0
 void dipole_tensor_host(const mafx_mesh& mesh, double* N) {
     const int64_t n0 = exp_xy(mesh.nx), n1 = exp_xy(mesh.ny), n2 = exp_z(mesh.nz);

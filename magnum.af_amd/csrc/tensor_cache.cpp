@@ -86,7 +86,9 @@ fs::path tensor_cache_dir() {
 }
 
 int32_t tensor_cache_layout(int precision, int term_flags) {
-    if (precision == MAFX_F64) return (term_flags & MAFX_TERM_DEVICE_TENSOR) ? kCacheDeviceTensor : 0;
+    if (term_flags & MAFX_TERM_DEVICE_TENSOR) return precision | kCacheDeviceTensor;
+    if (precision == MAFX_F64) return (term_flags & MAFX_TERM_OCTANT_TENSOR) && !(term_flags & MAFX_TERM_HOST_TENSOR)
+                                          ? kCacheOctantTensor : 0;
     return precision | ((term_flags & MAFX_TERM_HOST_TENSOR) ? kCacheHostTensor : 0);
 }
 
@@ -96,8 +98,13 @@ std::string tensor_cache_name(const TensorCacheKey& k) {
                     std::to_string(k.n2) + "_dx_" + std::to_string(1e9 * k.dx) + "_dy_" + std::to_string(1e9 * k.dy) +
                     "_dz_" + std::to_string(1e9 * k.dz);
     const int32_t layout = k.precision & 0xff;
-    s += layout == 0 ? ((k.precision & kCacheDeviceTensor) ? "_mafx_c64_dev" : "_mafx_c64")
-                     : ((k.precision & kCacheHostTensor) ? "_mafx_f32sym_host" : "_mafx_f32sym");
+    // FP32 layouts: the parity-extended host octant (default), the whole host tensor, the GPU tensor
+    s += layout == 0 ? ((k.precision & kCacheDeviceTensor)   ? "_mafx_c64_dev"
+                        : (k.precision & kCacheOctantTensor) ? "_mafx_c64_oct"
+                                                             : "_mafx_c64")
+                     : ((k.precision & kCacheHostTensor)     ? "_mafx_f32sym_host"
+                        : (k.precision & kCacheDeviceTensor) ? "_mafx_f32sym_dev"
+                                                             : "_mafx_f32oct");
     if (k.dipole) s += "_dipole";
     s += "_" + hex_bits(k.dx) + hex_bits(k.dy) + hex_bits(k.dz);
     return s;

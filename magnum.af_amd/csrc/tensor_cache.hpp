@@ -13,14 +13,16 @@
 
 namespace mafx {
 
-constexpr int32_t kCacheHostTensor = 0x100;   // FP32 layout from the host-built real-space tensor
-constexpr int32_t kCacheDeviceTensor = 0x200; // FP64 layout from the GPU-built real-space tensor
+constexpr int32_t kCacheHostTensor = 0x100;   // FP32 layout from the whole host-built real-space tensor
+constexpr int32_t kCacheDeviceTensor = 0x200; // either layout from the GPU-built real-space tensor
+constexpr int32_t kCacheOctantTensor = 0x400; // FP64 layout from the parity-extended host octant
 
 struct TensorCacheKey {
     int64_t n0, n1, n2, hxs; // expanded lengths and the padded row stride of the stored planes
     int32_t precision;       // MAFX_F64 (complex half spectrum) or MAFX_F32_FIELD (real quarter),
-                             // | kCacheHostTensor when the FP32 tensor comes from the host tensor,
-                             // | kCacheDeviceTensor when the FP64 tensor comes from the GPU tensor
+                             // | kCacheHostTensor when the FP32 tensor comes from the whole host
+                             // tensor (default: the host octant), | kCacheDeviceTensor when the
+                             // tensor comes from the GPU
     int32_t dipole;          // 1 for the atomistic point-dipole tensor
     double dx, dy, dz;
 };

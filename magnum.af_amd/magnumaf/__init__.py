@@ -106,7 +106,7 @@ class Context:
         msf = None
         if Ms_field is not None:
             msf = np.ascontiguousarray(Ms_field, dtype=np.float64)
-        prec = _abi.MAFX_F64 if precision in ("f64", _abi.MAFX_F64) else _abi.MAFX_F32_FIELD
+        prec = _abi.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         self.precision = prec
         h = C.c_void_p()
         torch = _torch()
@@ -378,22 +378,26 @@ class DemagField(FieldTerm):
     magnumaf.pyx:1086-1121).  caching reads the transformed tensor from the on-disk cache
     (MAFX_CACHE_DIR, else <tmp>/magnumaf.cache) and writes it there when absent; verbose reports
     the cache activity.  nthreads is accepted for the reference's signature (the host tensor uses
-    every hardware thread).  Keyword only, not in the reference: host_tensor makes the FP32-field
-    path build its real-space tensor on the host in the reference's operation order, as the FP64
-    path does, instead of on the GPU; device_tensor makes the FP64 path use the FP32 path's
-    GPU-built tensor instead (to compare the two paths on one tensor)."""
+    every hardware thread).  Keyword only, not in the reference, the real-space tensor the transform
+    starts from: by default the FP64 path uses the reference's whole tensor (host, reference
+    operation order) and the FP32 paths its non-negative octant extended by parity; host_tensor
+    selects the whole host tensor, octant_tensor the extended octant, device_tensor the formulas
+    evaluated on the GPU, on either path."""
 
-    def __init__(self, mesh, verbose=False, caching=False, nthreads=4, *, host_tensor=False, device_tensor=False):
+    def __init__(self, mesh, verbose=False, caching=False, nthreads=4, *, host_tensor=False, device_tensor=False,
+                 octant_tensor=False):
         self.mesh = mesh
         self.verbose, self.caching, self.nthreads = bool(verbose), bool(caching), nthreads
         self.host_tensor = bool(host_tensor)
         self.device_tensor = bool(device_tensor)
+        self.octant_tensor = bool(octant_tensor)
 
     def _desc(self, mesh):
         d = _td(_abi.DEMAG)
         d.flags = ((_abi.TERM_CACHING if self.caching else 0) | (_abi.TERM_VERBOSE if self.verbose else 0)
                    | (_abi.TERM_HOST_TENSOR if self.host_tensor else 0)
-                   | (_abi.TERM_DEVICE_TENSOR if self.device_tensor else 0))
+                   | (_abi.TERM_DEVICE_TENSOR if self.device_tensor else 0)
+                   | (_abi.TERM_OCTANT_TENSOR if self.octant_tensor else 0))
         return d, []
 
 
@@ -619,7 +623,8 @@ class LLGIntegrator:
     alpha is a float or a cell-wise damping field (nx, ny, nz) / (nx, ny, nz, 1)
     (LLGIntegrator(af::array alpha_field, ...), llg_integrator.hpp:32-40, magnumaf.pyx:962-967).
 
-    precision="f32" selects the FP32-field path (demag FFT and stencils in FP32, state and RK in FP64)."""
+    precision="f32" selects the FP32-field path (demag FFT and stencils in FP32, state and RK in FP64);
+    "f32_demag" keeps the stencils in FP64 (FP32 demag only), "f32_stencils" the demag."""
 
     def __init__(self, alpha, terms, mode="RKF45", hmin=1e-15, hmax=3.5e-10, atol=1e-6, rtol=1e-6,
                  dissipation_term_only=False, precision="f64"):
@@ -797,13 +802,24 @@ class LLGIntegrator:
 
 class LBFGS_Minimizer:
     """LBFGS_Minimizer(terms=[], tol=1e-6, maxiter=230, verbose=0) (magnumaf.pyx:1786-1836,
-    lbfgs_minimizer.hpp:9-24).  torque_tol > 0 adds the stop max|m x H| / max|H| <= torque_tol."""
+    lbfgs_minimizer.hpp:9-24).  torque_tol > 0 adds the stop max|m x H| / max|H| <= torque_tol.
 
-    def __init__(self, terms=(), tol=1e-6, maxiter=230, verbose=0, torque_tol=0.0, precision="f64"):
+    precision="f32" (or "f32_demag") minimises on the FP32-field path.  Its energy sums carry the
+    FP32 field's rounding, which near the minimum exceeds the energy change of an iteration; with
+    torque_tol > 0 a run that stops short of the torque tolerance there (iteration bound or five
+    iterations without a decrease) is finished on the FP64 path from where it stopped (fp64_finish,
+    default True).  last_stats then adds the FP64 iterations as "fp64_iterations".
+    convergence_file, when set, receives the reference's of_convergence_ rows
+    (lbfgs_minimizer.cpp:148-153) of every later minimisation."""
+
+    def __init__(self, terms=(), tol=1e-6, maxiter=230, verbose=0, torque_tol=0.0, precision="f64",
+                 fp64_finish=True):
         self.terms = list(terms)
         self.tol, self.maxiter, self.verbose = tol, maxiter, verbose
         self.torque_tol = torque_tol
         self.precision = precision
+        self.fp64_finish = fp64_finish
+        self.convergence_file = None
         self.last_stats = None
 
     def add_terms(self, *args):
@@ -812,11 +828,29 @@ class LBFGS_Minimizer:
     def delete_last_term(self):
         self.terms.pop()
 
-    def minimize(self, state):
-        ctx = state._context(self.terms, self.precision)
+    def _run(self, state, precision, maxiter):
+        ctx = state._context(self.terms, precision)
+        path = None if self.convergence_file is None else str(self.convergence_file).encode()
+        if path is not None and precision != self.precision:
+            path = None  # the FP64 finish appends nothing to the file of the main run
+        check(lib().mafx_lbfgs_set_convergence_file(ctx.h, path))
         E = C.c_double()
         st = _abi.LbfgsStats()
-        check(lib().mafx_lbfgs_minimize(ctx.h, C.c_void_p(state.m_dev.data_ptr()), self.tol, self.maxiter,
-                                        self.verbose, self.torque_tol, C.byref(E), C.byref(st)))
-        self.last_stats = {k: getattr(st, k) for k, _ in _abi.LbfgsStats._fields_}
-        return E.value
+        try:
+            check(lib().mafx_lbfgs_minimize(ctx.h, C.c_void_p(state.m_dev.data_ptr()), self.tol, maxiter,
+                                            self.verbose, self.torque_tol, C.byref(E), C.byref(st)))
+        finally:
+            check(lib().mafx_lbfgs_set_convergence_file(ctx.h, None))
+        return E.value, {k: getattr(st, k) for k, _ in _abi.LbfgsStats._fields_}
+
+    def minimize(self, state):
+        E, st = self._run(state, self.precision, self.maxiter)
+        if (self.precision != "f64" and self.fp64_finish and self.torque_tol > 0 and st["stop_reason"] in (0, 5)
+                and st["final_torque_rel"] > self.torque_tol):
+            E, st64 = self._run(state, "f64", self.maxiter)
+            st = dict(st64, fp32_iterations=st["iterations"], fp32_energy_evals=st["energy_evals"],
+                      fp64_iterations=st64["iterations"],
+                      iterations=st["iterations"] + st64["iterations"],
+                      energy_evals=st["energy_evals"] + st64["energy_evals"])
+        self.last_stats = st
+        return E

@@ -6,11 +6,14 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libmafx.so")
 
-MAFX_F64, MAFX_F32_FIELD = 0, 1
+MAFX_F64, MAFX_F32_FIELD, MAFX_F32_DEMAG, MAFX_F32_STENCILS = 0, 1, 2, 3
+# precision names of the Python layer: "f32" is the FP32-field path (demag FFT and stencils in FP32);
+# "f32_demag" keeps the stencils in FP64; "f32_stencils" keeps the demag in FP64
+PRECISIONS = {"f64": MAFX_F64, "f32": MAFX_F32_FIELD, "f32_demag": MAFX_F32_DEMAG, "f32_stencils": MAFX_F32_STENCILS}
 DEMAG, EXCHANGE, DMI, UNIAXIAL, ZEEMAN, RKKY = 1, 2, 3, 4, 5, 6
 # mafx_term_desc.flags (include/mafx.h)
 TERM_CACHING, TERM_VERBOSE, TERM_HOST_TENSOR, TERM_HOMOGENEOUS_CALLBACK, TERM_HOST_CALLBACK = 1, 2, 4, 8, 16
-TERM_DEVICE_TENSOR = 32
+TERM_DEVICE_TENSOR, TERM_OCTANT_TENSOR = 32, 64
 ATOM_DIPOLE, ATOM_EXCHANGE, ATOM_DMI, ATOM_UNIAXIAL, ATOM_ZEEMAN = 11, 12, 13, 14, 15
 
 # every entry point include/mafx.h declares
@@ -24,7 +27,7 @@ EXPORTS = [
     "mafx_llg_set_controller_state", "mafx_llg_get_controller_state", "mafx_lbfgs_minimize", "mafx_prof_enable",
     "mafx_prof_reset", "mafx_prof_report", "mafx_net_create", "mafx_net_destroy", "mafx_net_param_count",
     "mafx_net_set_params", "mafx_net_get_params", "mafx_net_forward", "mafx_net_gradients", "mafx_net_train_step",
-    "mafx_net_warm_start",
+    "mafx_net_warm_start", "mafx_lbfgs_set_convergence_file",
 ]
 
 
@@ -101,6 +104,7 @@ def lib():
     L.mafx_llg_set_controller_state.argtypes = [vp, C.c_double, C.c_double, C.c_int]
     L.mafx_llg_get_controller_state.argtypes = [vp, dp, dp, C.POINTER(C.c_int)]
     L.mafx_lbfgs_minimize.argtypes = [vp, vp, C.c_double, i64, C.c_int, C.c_double, dp, C.POINTER(LbfgsStats)]
+    L.mafx_lbfgs_set_convergence_file.argtypes = [vp, C.c_char_p]
     L.mafx_prof_enable.argtypes = [vp, C.c_int]
     L.mafx_prof_reset.argtypes = [vp]
     L.mafx_prof_report.argtypes = [vp, C.c_char_p, i64]

@@ -595,7 +595,9 @@ def test_resume_is_step_exact(scheme, tmp_path):
     d = M.State(g, 8e5, np.ones(mesh[:3] + (3,)))
     d.read_vti(fn)
     assert np.array_equal(d.m, c.m)
-    assert str(c).count("\t") == 3
+    # the C++ stream line (state.cpp:26-30) and the Python print (magnumaf.pyx:754-757)
+    assert c.stream_line().count("\t") == 3
+    assert str(c) == " ".join([str(c.t)] + [str(v) for v in c.mean_m()])
 
 
 def test_sp4_relaxation_and_switch_gpu():

@@ -71,6 +71,7 @@ def test_context_without_gpu_fails_loudly():
 @pytest.mark.parametrize("mesh,prec,flags,dipole", [((128, 32, 1, 4e-9, 4e-9, 3e-9), 0, 0, 0),
                                                     ((1024, 1024, 4, 2e-9, 2e-9, 2e-9), 1, 0, 0),
                                                     ((1024, 1024, 4, 2e-9, 2e-9, 2e-9), 1, 4, 0),
+                                                    ((1024, 1024, 4, 2e-9, 2e-9, 2e-9), 1, 32, 0),
                                                     ((64, 32, 2, 2e-9, 2e-9, 2e-9), 0, 32, 0),
                                                     ((512, 512, 1, 2.7e-10, 2.7e-10, 2.7e-10), 0, 0, 1)])
 def test_demag_cache_name_is_the_reference_mesh_string(mesh, prec, flags, dipole):
@@ -86,9 +87,10 @@ def test_demag_cache_name_is_the_reference_mesh_string(mesh, prec, flags, dipole
     m = _abi.Mesh(*mesh)
     assert _lib().mafx_demag_cache_name(C.byref(m), prec, flags, dipole, buf, len(buf)) == 0
     name = buf.value.decode()
-    # the layouts built from the other real-space tensor (MAFX_TERM_HOST_TENSOR on the FP32 path,
-    # MAFX_TERM_DEVICE_TENSOR on the FP64 path) have names of their own
-    layout = {(0, 0): "_mafx_c64_", (0, 32): "_mafx_c64_dev_", (1, 0): "_mafx_f32sym_", (1, 4): "_mafx_f32sym_host_"}
+    # the layouts built from other real-space tensors (the whole host tensor on the FP32 path, whose
+    # default is the host octant; the GPU-built one on either path) have names of their own
+    layout = {(0, 0): "_mafx_c64_", (0, 32): "_mafx_c64_dev_", (1, 0): "_mafx_f32oct_", (1, 4): "_mafx_f32sym_host_",
+              (1, 32): "_mafx_f32sym_dev_"}
     assert name.startswith(ref + layout[(prec, flags)].rstrip("_"))
     assert (layout[(prec, flags)] in name) or (dipole and layout[(prec, flags)].rstrip("_") + "_dipole_" in name)
     bits = "".join(f"{struct.unpack('<Q', struct.pack('<d', v))[0]:016x}" for v in (dx, dy, dz))
