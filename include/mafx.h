@@ -278,6 +278,8 @@ typedef struct mafx_lbfgs_stats {
     int32_t stop_reason; /* 0 maxiter, 1 gradient, 2 triple test, 3 torque, 4 no descent,
                             5 stalled: five iterations in a row without an energy decrease
                             (checked only when torque_tol > 0) */
+    int64_t finish_iterations; /* iterations run on the finish context (see below); included in
+                                  iterations */
 } mafx_lbfgs_stats;
 /* verbose > 0 prints "f= E" at the start and, when the loop ends through a stop test other than
  * the gradient one or the iteration bound, the reference's closing "aa> iterations dE max|mxmxH|"
@@ -289,6 +291,13 @@ int mafx_lbfgs_minimize(mafx_ctx* ctx, double* m_dev, double tolerance, int64_t 
  * (f_old - f) / (tol (1 + |f|)), max|s| / (sqrt(tol) (1 + max|m|)), max|g| / (tol^(1/3) (1 + |f|)),
  * tab-separated, one line per iteration.  The file is truncated when set; NULL closes it. */
 int mafx_lbfgs_set_convergence_file(mafx_ctx* ctx, const char* path);
+/* No counterpart in the reference (its minimizer runs in FP64 only).  With a finish context set
+ * (a context of the same mesh and terms, normally the FP64 one of an FP32-field ctx), a
+ * mafx_lbfgs_minimize on ctx with torque_tol > 0 that stalls (stop reason 5: the FP32 field's
+ * energy rounding hides the decrease of an iteration) goes on from the same iterate on the finish
+ * context, keeping the curvature pairs, H0 scaling and iteration count, until a stop test or
+ * max_iter ends it.  NULL clears it.  The finish context must outlive its use here. */
+int mafx_lbfgs_set_finish_context(mafx_ctx* ctx, mafx_ctx* finish);
 
 /* ------------------------------------------------------------------ learned warm start */
 /* No counterpart in the reference (magnum.af has no learned component).  A 4-level

@@ -602,6 +602,20 @@ int mafx_lbfgs_set_convergence_file(mafx_ctx* ctx, const char* path) {
     });
 }
 
+int mafx_lbfgs_set_finish_context(mafx_ctx* ctx, mafx_ctx* finish) {
+    return guarded([&] {
+        require(ctx, "null context");
+        if (finish) {
+            require(finish->N == ctx->N && finish->mesh.nx == ctx->mesh.nx && finish->mesh.ny == ctx->mesh.ny &&
+                        finish->mesh.nz == ctx->mesh.nz,
+                    "mafx_lbfgs_set_finish_context: the finish context has another mesh");
+            require(finish->terms.size() == ctx->terms.size(),
+                    "mafx_lbfgs_set_finish_context: the finish context has another term count");
+        }
+        ctx->lbfgs_finish = finish == ctx ? nullptr : finish;
+    });
+}
+
 int mafx_net_create(void* stream, mafx_net** out) {
     return guarded([&] {
         require(out != nullptr, "mafx_net_create: null output");

@@ -285,6 +285,11 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
     const double tolf3 = std::pow(tolerance, 0.3333333333333333333333333);
     int reason = 0;
     int stalled = 0; // consecutive iterations without an energy decrease (torque_tol > 0 only)
+    // a stalled run moves to the finish context once (mafx_lbfgs_set_finish_context)
+    mafx_ctx* finish_ctx = torque_tol > 0 ? ctx->lbfgs_finish : nullptr;
+    std::FILE* const conv = ctx->lbfgs_convergence;
+    size_t finish_from = 0; // globIter at the move; 0 = no move
+    bool move_now = false;
     double f = L.energy_and_gradient(m, grad.as<double>());
     double gradNorm = L.maxabs(grad.as<double>());
     size_t globIter = 0;
@@ -296,6 +301,7 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
             stats->final_energy = f;
             stats->final_torque_rel = L.last_hmax > 0 ? L.last_torque / L.last_hmax : 0.0;
             stats->stop_reason = r;
+            stats->finish_iterations = finish_from > 0 ? static_cast<int64_t>(globIter - finish_from) : 0;
         }
         return MAFX_OK;
     };
@@ -359,10 +365,10 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         const double smax = L.maxabs(s.as<double>());
         const double mmax = L.maxabs(m);
         // lbfgs_minimizer.cpp:148-153: the three stop ratios, one line per iteration
-        if (ctx->lbfgs_convergence) {
-            std::fprintf(ctx->lbfgs_convergence, "%.27g\t%.27g\t%.27g\n", (f_old - f) / (tolerance * f1),
+        if (conv) {
+            std::fprintf(conv, "%.27g\t%.27g\t%.27g\n", (f_old - f) / (tolerance * f1),
                          smax / (tolf2 * (1 + mmax)), gradNorm / (tolf3 * f1));
-            std::fflush(ctx->lbfgs_convergence);
+            std::fflush(conv);
         }
         if (((f_old - f) < (tolerance * f1)) && (smax < (tolf2 * (1 + mmax))) && (gradNorm <= (tolf3 * f1))) {
             globIter++;
@@ -379,9 +385,13 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         // times in a row ends the run instead of spending the remaining iterations on it
         stalled = (torque_tol > 0 && f >= f_old) ? stalled + 1 : 0;
         if (stalled >= 5) {
-            globIter++;
-            reason = 5;
-            break;
+            if (finish_ctx && finish_from == 0) {
+                move_now = true; // after this iteration's curvature pair is stored
+            } else {
+                globIter++;
+                reason = 5;
+                break;
+            }
         }
         sub(ctx, y.as<double>(), grad.as<double>(), grad_old.as<double>(), n3);
         // y . s, |y|^2 and |s|^2 (lbfgs_minimizer.cpp:160-170) in one transfer
@@ -410,6 +420,21 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
             iter++;
         }
         globIter++;
+        if (move_now) {
+            // the same iterate, history and H0 scaling on the finish context: its energy and
+            // gradient replace the stalled context's, and the next iteration starts from them
+            move_now = false;
+            MAFX_HIP(hipStreamSynchronize(ctx->stream));
+            ctx = finish_ctx;
+            L.ctx = finish_ctx;
+            f = L.energy_and_gradient(m, grad.as<double>());
+            stalled = 0;
+            finish_from = globIter;
+            if (verbose > 1) {
+                std::printf("bb> finish context from iteration %zu: f= %.17g\n", globIter, f);
+                std::fflush(stdout);
+            }
+        }
     } while (globIter < static_cast<size_t>(max_iter));
     // lbfgs_minimizer.cpp:181-192: the closing report after the loop (the gradient stop returns
     // before it); its max |m x (m x H)| is evaluated, as there, on a state whose m is the last gradient

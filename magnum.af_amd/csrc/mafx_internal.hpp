@@ -168,6 +168,7 @@ struct mafx_ctx {
     mafx::DevBuf vtmp;        // 3N f64 scratch for vector helpers
     hipStream_t stream{nullptr};
     std::FILE* lbfgs_convergence{nullptr}; // LBFGS_Minimizer::of_convergence_ (mafx_lbfgs_set_convergence_file)
+    mafx_ctx* lbfgs_finish{nullptr}; // context an LBFGS run moves to when it stalls (mafx_lbfgs_set_finish_context)
     mafx::Prof prof;
     int64_t device_bytes{0};
     ~mafx_ctx();

@@ -806,9 +806,10 @@ class LBFGS_Minimizer:
 
     precision="f32" (or "f32_demag") minimises on the FP32-field path.  Its energy sums carry the
     FP32 field's rounding, which near the minimum exceeds the energy change of an iteration; with
-    torque_tol > 0 a run that stops short of the torque tolerance there (iteration bound or five
-    iterations without a decrease) is finished on the FP64 path from where it stopped (fp64_finish,
-    default True).  last_stats then adds the FP64 iterations as "fp64_iterations".
+    torque_tol > 0 a run that stalls there (five iterations without a decrease) goes on from the same
+    iterate on the FP64 path, keeping its curvature pairs (fp64_finish, default True;
+    mafx_lbfgs_set_finish_context).  last_stats then splits the iterations into "fp32_iterations"
+    and "fp64_iterations".
     convergence_file, when set, receives the reference's of_convergence_ rows
     (lbfgs_minimizer.cpp:148-153) of every later minimisation."""
 
@@ -828,29 +829,25 @@ class LBFGS_Minimizer:
     def delete_last_term(self):
         self.terms.pop()
 
-    def _run(self, state, precision, maxiter):
-        ctx = state._context(self.terms, precision)
+    def minimize(self, state):
+        ctx = state._context(self.terms, self.precision)
+        finish = None
+        if self.precision != "f64" and self.fp64_finish and self.torque_tol > 0:
+            finish = state._context(self.terms, "f64")
         path = None if self.convergence_file is None else str(self.convergence_file).encode()
-        if path is not None and precision != self.precision:
-            path = None  # the FP64 finish appends nothing to the file of the main run
         check(lib().mafx_lbfgs_set_convergence_file(ctx.h, path))
+        check(lib().mafx_lbfgs_set_finish_context(ctx.h, None if finish is None else finish.h))
         E = C.c_double()
         st = _abi.LbfgsStats()
         try:
-            check(lib().mafx_lbfgs_minimize(ctx.h, C.c_void_p(state.m_dev.data_ptr()), self.tol, maxiter,
+            check(lib().mafx_lbfgs_minimize(ctx.h, C.c_void_p(state.m_dev.data_ptr()), self.tol, self.maxiter,
                                             self.verbose, self.torque_tol, C.byref(E), C.byref(st)))
         finally:
             check(lib().mafx_lbfgs_set_convergence_file(ctx.h, None))
-        return E.value, {k: getattr(st, k) for k, _ in _abi.LbfgsStats._fields_}
-
-    def minimize(self, state):
-        E, st = self._run(state, self.precision, self.maxiter)
-        if (self.precision != "f64" and self.fp64_finish and self.torque_tol > 0 and st["stop_reason"] in (0, 5)
-                and st["final_torque_rel"] > self.torque_tol):
-            E, st64 = self._run(state, "f64", self.maxiter)
-            st = dict(st64, fp32_iterations=st["iterations"], fp32_energy_evals=st["energy_evals"],
-                      fp64_iterations=st64["iterations"],
-                      iterations=st["iterations"] + st64["iterations"],
-                      energy_evals=st["energy_evals"] + st64["energy_evals"])
+            check(lib().mafx_lbfgs_set_finish_context(ctx.h, None))
+        st = {k: getattr(st, k) for k, _ in _abi.LbfgsStats._fields_}
+        if finish is not None:
+            st["fp64_iterations"] = st["finish_iterations"]
+            st["fp32_iterations"] = st["iterations"] - st["finish_iterations"]
         self.last_stats = st
-        return E
+        return E.value
