@@ -218,6 +218,15 @@ int mafx_max_abs(mafx_ctx* ctx, const double* a_dev, double* out_host);
 int mafx_normalize(mafx_ctx* ctx, double* m_dev, int zero_safe);
 /* <m> over cells with |m| != 0 (state.cpp:32-45). */
 int mafx_mean_m(mafx_ctx* ctx, const double* m_dev, double out_host[3]);
+/* No counterpart in the reference.  For callers that cannot allocate (code under HIP graph
+ * capture, tight output loops, harnesses that forbid hipMalloc): mafx_mean_m's result, bit for
+ * bit, without a context.  m_dev is [3][ncells] f64 (SoA, as everywhere); out_dev receives 3
+ * doubles in device memory, in stream order on `stream` (NULL: the null stream); nothing is
+ * synchronised.  The reduction's partial sums live in a static 32 KiB device array (4 x 1024
+ * doubles, one per GPU for the life of the process), so the call allocates nothing.  Calls that
+ * overlap on one GPU share that array and race: serialise them (one stream, or events).
+ * ncells must be in [1, 2^60]; the partial array is the same size for every ncells. */
+int mafx_mean_m_noalloc(const double* m_dev, int64_t ncells, double* out_dev, void* stream);
 
 /* ------------------------------------------------------------------ adaptive RK / LLG */
 /* LLGIntegrator(alpha, terms, scheme, Controller(hmin, hmax, atol, rtol), dissipation_term_only)

@@ -433,6 +433,14 @@ int mafx_mean_m(mafx_ctx* ctx, const double* m_dev, double out_host[3]) {
     return guarded([&] { mean_m(ctx, m_dev, out_host); });
 }
 
+int mafx_mean_m_noalloc(const double* m_dev, int64_t ncells, double* out_dev, void* stream) {
+    return guarded([&] {
+        require(m_dev != nullptr && out_dev != nullptr, "mafx_mean_m_noalloc: null pointer");
+        require(ncells >= 1 && ncells <= (int64_t(1) << 60), "mafx_mean_m_noalloc: ncells must be in [1, 2^60]");
+        mean_m_noalloc(m_dev, ncells, out_dev, static_cast<hipStream_t>(stream));
+    });
+}
+
 int mafx_llg_create(mafx_ctx* ctx, double alpha, const char* scheme, double hmin, double hmax, double atol,
                     double rtol, int dissipation_term_only, mafx_llg** out) {
     return guarded([&] {

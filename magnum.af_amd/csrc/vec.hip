@@ -3,6 +3,7 @@
 // <m>, the LLG right-hand side, RK stage combinations, the embedded error norm and the accept
 // update.  Reference citations are relative to magnumaf/src.
 #include <cmath>
+#include <mutex>
 
 #include "mafx_internal.hpp"
 #include "vec_kernels.hpp"
@@ -292,6 +293,54 @@ void mean_m(mafx_ctx* ctx, const double* m, double out[3]) {
     MAFX_HIP(hipGetLastError());
     read_scalar(ctx, 4);
     for (int c = 0; c < 3; ++c) out[c] = ctx->host_scal[c] / ctx->host_scal[3];
+}
+
+// ------------------------------------------------------------------ <m> without allocation
+// mafx_mean_m_noalloc's block partials [4][kMaxRedBlocks] (component sums, non-zero count) live in
+// this file-scope array, one copy per GPU for the life of the process.  Host code never passes the
+// array's name to a kernel: the host-side symbol is not a device address.  The device address is
+// looked up once per device with hipGetSymbolAddress, with that device current, and kept in
+// g_noalloc_ptr.  Every slot a final pass reads is written by the same call's first pass, so
+// nothing needs clearing; concurrent calls on one GPU share the array and race.
+static __device__ __align__(256) double scratch[4 * kMaxRedBlocks];
+namespace {
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kNoallocMaxDevices] = {};
+
+double* noalloc_scratch() {
+    int dev = -1;
+    MAFX_HIP(hipGetDevice(&dev));
+    require(dev >= 0 && dev < kNoallocMaxDevices, "mafx_mean_m_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[dev] == nullptr) {
+        void* ptr = nullptr;
+        MAFX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        require(ptr != nullptr, "mafx_mean_m_noalloc: no device address for the scratch array");
+        g_noalloc_ptr[dev] = static_cast<double*>(ptr);
+    }
+    return g_noalloc_ptr[dev];
+}
+
+// the four sums of k_final's tree (same partial order and block_sum), then <m> = sum / count
+__global__ void __launch_bounds__(kB) k_mean_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+    double s[4];
+    for (int c = 0; c < 4; ++c) {
+        double v = 0;
+        for (int i = threadIdx.x; i < nb; i += kB) v = v + part[c * nb + i];
+        s[c] = block_sum(v);
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) out[c] = s[c] / s[3];
+}
+} // namespace
+
+void mean_m_noalloc(const double* m, int64_t N, double* out_dev, hipStream_t stream) {
+    double* part = noalloc_scratch();
+    const unsigned nb = red_blocks(N);
+    hipLaunchKernelGGL(k_mean, dim3(nb), dim3(kB), 0, stream, m, N, part);
+    hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(kB), 0, stream, part, static_cast<int>(nb), out_dev);
+    MAFX_HIP(hipGetLastError());
 }
 
 namespace {

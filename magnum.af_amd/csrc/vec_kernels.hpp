@@ -27,5 +27,8 @@ void axpy_dev(mafx_ctx* ctx, double* y, const double* a_dev, const double* x, in
 void lbfgs_alpha(mafx_ctx* ctx, double* alpha_i, double* coef, const double* d, double rho);
 void lbfgs_beta(mafx_ctx* ctx, double* coef, const double* alpha_i, const double* d, double rho);
 void read_device(mafx_ctx* ctx, const double* src_dev, int n, double* dst_host);
+// <m> of mafx_mean_m with its partial sums in a static device array: no allocation, no
+// synchronisation, result in out_dev[0..2] (device memory), launched on `stream`.
+void mean_m_noalloc(const double* m, int64_t N, double* out_dev, hipStream_t stream);
 
 } // namespace mafx

@@ -27,7 +27,7 @@ EXPORTS = [
     "mafx_llg_set_controller_state", "mafx_llg_get_controller_state", "mafx_lbfgs_minimize", "mafx_prof_enable",
     "mafx_prof_reset", "mafx_prof_report", "mafx_net_create", "mafx_net_destroy", "mafx_net_param_count",
     "mafx_net_set_params", "mafx_net_get_params", "mafx_net_forward", "mafx_net_gradients", "mafx_net_train_step",
-    "mafx_net_warm_start", "mafx_lbfgs_set_convergence_file", "mafx_lbfgs_set_finish_context",
+    "mafx_net_warm_start", "mafx_lbfgs_set_convergence_file", "mafx_lbfgs_set_finish_context", "mafx_mean_m_noalloc",
 ]
 
 
@@ -106,6 +106,7 @@ def lib():
     L.mafx_lbfgs_minimize.argtypes = [vp, vp, C.c_double, i64, C.c_int, C.c_double, dp, C.POINTER(LbfgsStats)]
     L.mafx_lbfgs_set_convergence_file.argtypes = [vp, C.c_char_p]
     L.mafx_lbfgs_set_finish_context.argtypes = [vp, vp]
+    L.mafx_mean_m_noalloc.argtypes = [vp, i64, vp, vp]
     L.mafx_prof_enable.argtypes = [vp, C.c_int]
     L.mafx_prof_reset.argtypes = [vp]
     L.mafx_prof_report.argtypes = [vp, C.c_char_p, i64]
