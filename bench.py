@@ -70,7 +70,7 @@ def main():
     ap.add_argument("--config", choices=["C1", "C2traj", "C3", "C4", "C5", "C5full"], default=None,
                     help="run one secondary configuration and print its JSON line instead of the main one")
     ap.add_argument("--c2-seeds", default="2024", help="C2traj: comma-separated seeds of the initial state")
-    ap.add_argument("--c2-paths", default="f32,f32_demag,f32_stencils,f64_dev,f64_oct",
+    ap.add_argument("--c2-paths", default="f32,f32_full,f32_demag,f32_stencils,f64_oct",
                     help="C2traj: paths to compare with the FP64 path")
     ap.add_argument("--c4-samples", type=int, default=4096, help="C4: training films (64x64x1)")
     ap.add_argument("--c4-epochs", type=int, default=20, help="C4: training epochs")
@@ -200,9 +200,9 @@ def main():
     del src, dst
 
     # ---------------------------------------------------------------- DP45 steps/s
-    def dp45_rate(precision, nsteps):
+    def dp45_rate(precision, nsteps, tl=terms):
         st = M.State(mesh, Ms, m0)
-        llg = M.LLGIntegrator(alpha, terms, "DP45", precision=precision)
+        llg = M.LLGIntegrator(alpha, tl, "DP45", precision=precision)
         st._ctx = state._ctx  # reuse the contexts built above
         for _ in range(3):
             llg.step(st)
@@ -222,6 +222,18 @@ def main():
     log(f"[bench] DP45 FP32-field: {dp32:.2f} steps/s ({at32:.2f} attempts/s)")
     dp64, at64, s64 = dp45_rate("f64", max(4, args.dp45_steps // 2))
     log(f"[bench] DP45 FP64: {dp64:.2f} steps/s ({at64:.2f} attempts/s)")
+
+    # ---------------------------------------------------------------- FP32 field on the reference tensor's spectrum
+    # MAFX_TERM_FULL_TENSOR: the FP32 arithmetic on the reference's own (complex) tensor spectrum
+    terms_full = [M.DemagField(mesh, full_tensor=True)] + terms[1:]
+    ctxf = state._context(terms_full, "f32")
+    heff_loop(ctxf, 5)
+    torch.cuda.synchronize()
+    repsf = timed_repeats(lambda: heff_loop(ctxf, args.steps), torch)
+    heff32f = args.steps / float(np.median(repsf))
+    dp32f, at32f, _ = dp45_rate("f32", args.dp45_steps, terms_full)
+    log(f"[bench] FP32 field on the full tensor spectrum: {heff32f:.1f} H_eff/s, DP45 {dp32f:.2f} steps/s")
+    del state._ctx[(tuple(id(t) for t in terms_full), "f32")], ctxf
 
     # ---------------------------------------------------------------- CPU baseline (oracle)
     cpu = None
@@ -272,6 +284,8 @@ def main():
             "dp45_steps_per_s_fp32": dp32, "dp45_steps_per_s_fp64": dp64,
             "dp45_attempts_per_s_fp32": at32, "dp45_attempts_per_s_fp64": at64,
             "dp45_fp32_over_fp64": dp32 / dp64 if dp64 > 0 else None,
+            "heff_per_s_fp32_full_tensor": heff32f, "dp45_steps_per_s_fp32_full_tensor": dp32f,
+            "dp45_fp32_full_tensor_over_fp64": dp32f / dp64 if dp64 > 0 else None,
             "heff_fp32_normwise_error_vs_fp64": heff_err32,
             "field_errors": errors,
             "heff_algorithmic_GB_fp32": bytes_heff32 / 1e9, "heff_algorithmic_GB_fp64": bytes_heff64 / 1e9,
@@ -337,6 +351,7 @@ def field_errors(M, torch, mesh, state, terms, ctx32, ctx64, sim, ref_total):
     d64dev = demag_with("f64", device_tensor=True)
     d32dev = demag_with("f32", device_tensor=True)
     d32host = demag_with("f32", host_tensor=True)
+    d32full = demag_with("f32", full_tensor=True)
     out["demag_split"] = {
         "fp64_octant_tensor_vs_ref": nw(d64oct, ref_demag),
         "fp64_gpu_tensor_vs_ref": nw(d64dev, ref_demag),
@@ -344,7 +359,8 @@ def field_errors(M, torch, mesh, state, terms, ctx32, ctx64, sim, ref_total):
         "fp32_gpu_tensor_vs_fp64_gpu_tensor": nw(d32dev, d64dev),
         "fp32_octant_tensor_vs_ref": out["terms"]["demag"]["fp32_vs_ref"],
         "fp32_whole_host_tensor_vs_ref": nw(d32host, ref_demag),
-        "fp32_gpu_tensor_vs_ref": nw(d32dev, ref_demag)}
+        "fp32_gpu_tensor_vs_ref": nw(d32dev, ref_demag),
+        "fp32_full_tensor_vs_ref": nw(d32full, ref_demag)}  # FP32 arithmetic on the reference's tensor
     log(f"[bench] demag split: {out['demag_split']}")
     return out
 
@@ -394,7 +410,8 @@ def config_c2traj(M, torch, args):
     """1024x1024x4 film, DP45 for 100 ps from seeded random unit vectors on the FP64 path and on the
     FP32 paths; <m> every 1 ps through dense output; max deviation of <m> from the FP64 run of the
     same seed.  Paths: "f32" (FP32 demag and stencils), "f32_demag" (FP32 demag, FP64 stencils),
-    "f32_stencils" (FP64 demag, FP32 stencils), "f32_host" (FP32-field on the whole host tensor),
+    "f32_stencils" (FP64 demag, FP32 stencils), "f32_host" (FP32-field on the whole host tensor,
+    symmetrised), "f32_full" (FP32-field on the reference tensor's own complex spectrum),
     "f64_dev" / "f64_oct" (the FP64 path on another evaluation of the tensor: the floor any path
     that does not reproduce the reference's far-field rounding bit for bit can reach)."""
     nx, ny, nz, d = 1024, 1024, 4, 2e-9
@@ -409,9 +426,9 @@ def config_c2traj(M, torch, args):
         for path in ["f64"] + [p for p in paths if p != "f64"]:
             # f64_dev / f64_oct: the FP64 path on the GPU-built tensor or the parity-extended host
             # octant, equally valid evaluations of the Newell tensor with other far-field noise
-            prec = {"f32_host": "f32", "f64_dev": "f64", "f64_oct": "f64"}.get(path, path)
+            prec = {"f32_host": "f32", "f32_full": "f32", "f64_dev": "f64", "f64_oct": "f64"}.get(path, path)
             terms = [M.DemagField(mesh, host_tensor=path == "f32_host", device_tensor=path == "f64_dev",
-                                  octant_tensor=path == "f64_oct")] + local
+                                  octant_tensor=path == "f64_oct", full_tensor=path == "f32_full")] + local
             state = M.State(mesh, 1.1e6, m0)
             llg = M.LLGIntegrator(0.1, terms, "DP45", precision=prec)
             t0 = time.perf_counter()

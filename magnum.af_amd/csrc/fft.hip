@@ -1270,5 +1270,7 @@ template void fft_y_split<double2>(mafx_ctx*, double2*, int64_t, int, int64_t, i
 template void fft_y_split<float2>(mafx_ctx*, float2*, int64_t, int, int64_t, int, int64_t, bool);
 template void kspace_mul<double2>(mafx_ctx*, double2*, const double2*, int64_t);
 template void zfused<double2>(mafx_ctx*, double2*, const double2*, int);
+template void kspace_mul<float2>(mafx_ctx*, float2*, const float2*, int64_t);
+template void zfused<float2>(mafx_ctx*, float2*, const float2*, int);
 
 } // namespace mafx

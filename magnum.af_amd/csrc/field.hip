@@ -669,17 +669,19 @@ template <class C, class TH, class T> void evaluate_impl(mafx_ctx* ctx, const Ev
         // column tiles: rows ky are kTile elements apart, planes n1 hxs
         fft_y_split<C>(ctx, spec, sa, 3, n1 * hxs, static_cast<int>(g.nz), kTile, false);
         // tensor product: complex reference spectrum (FP64) or its real symmetric quarter (FP32)
+        // (or, MAFX_TERM_FULL_TENSOR, the complex spectrum rounded to FP32)
+        const bool sym = std::is_same<C, float2>::value && !demag_full_c32(ctx->precision, ctx->demag_flags);
         auto kmul = [&](int64_t planes) {
-            if constexpr (std::is_same<C, float2>::value)
-                kspace_mul_sym(ctx, spec, ctx->tensor.as<float>(), planes);
+            if (sym)
+                kspace_mul_sym(ctx, reinterpret_cast<float2*>(spec), ctx->tensor.as<float>(), planes);
             else
                 kspace_mul<C>(ctx, spec, ctx->tensor.as<C>(), sa);
         };
         if (n2 == 1) {
             kmul(1);
         } else if (nzs == g.nz) {
-            if constexpr (std::is_same<C, float2>::value)
-                zfused_sym(ctx, spec, ctx->tensor.as<float>(), static_cast<int>(g.nz));
+            if (sym)
+                zfused_sym(ctx, reinterpret_cast<float2*>(spec), ctx->tensor.as<float>(), static_cast<int>(g.nz));
             else
                 zfused<C>(ctx, spec, ctx->tensor.as<C>(), static_cast<int>(g.nz));
         } else {
@@ -945,6 +947,12 @@ __global__ void __launch_bounds__(256)
     for (int c = 0; c < 6; ++c) N[c * P + at] = v[c] * s;
 }
 
+// the FP64 complex spectrum rounded to complex64 (MAFX_TERM_FULL_TENSOR)
+__global__ void __launch_bounds__(256) k_round_c32(const double2* __restrict__ a, float2* __restrict__ b, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) b[i] = float2{static_cast<float>(a[i].x), static_cast<float>(a[i].y)};
+}
+
 } // namespace
 
 // Demag / dipole tensor: the real-space tensor (FP64 path: on the host in the reference's
@@ -961,8 +969,9 @@ void build_tensor(mafx_ctx* ctx) {
     ctx->n2h = n2 / 2 + 1;
     const bool caching = (ctx->demag_flags & MAFX_TERM_CACHING) != 0;
     const bool verbose = (ctx->demag_flags & MAFX_TERM_VERBOSE) != 0;
-    const uint64_t stored = demag_f64(ctx->precision)
-                                ? sizeof(double2) * 6 * static_cast<uint64_t>(K)
+    const bool full32 = demag_full_c32(ctx->precision, ctx->demag_flags);
+    const uint64_t stored = demag_f64(ctx->precision) ? sizeof(double2) * 6 * static_cast<uint64_t>(K)
+                            : full32                  ? sizeof(float2) * 6 * static_cast<uint64_t>(K)
                                 : sizeof(float) * 6 * static_cast<uint64_t>(ctx->n2h * ctx->n1h * hxs);
     // the FP32 layout from the host-built and the FP64 layout from the GPU-built real-space tensor
     // are stored under keys of their own
@@ -995,13 +1004,14 @@ void build_tensor_uncached(mafx_ctx* ctx) {
     DevBuf Nd, T64;
     Nd.alloc(sizeof(double) * 6 * P);
     const bool f64 = demag_f64(ctx->precision);
+    const bool full32 = demag_full_c32(ctx->precision, ctx->demag_flags);
     const mafx_mesh& g = ctx->mesh;
     if (ctx->demag_flags & MAFX_TERM_DEVICE_TENSOR) {
         // the formulas evaluated on the GPU (fast; its own far-field rounding noise)
         hipLaunchKernelGGL(k_real_tensor, dim3(grid_for(P, 256)), dim3(256), 0, ctx->stream, Nd.as<double>(), n0, n1,
                            n2, g.dx, g.dy, g.dz, ctx->atom_dipole ? 1 : 0);
         MAFX_HIP(hipGetLastError());
-    } else if (ctx->atom_dipole || (ctx->demag_flags & MAFX_TERM_HOST_TENSOR) ||
+    } else if (ctx->atom_dipole || (ctx->demag_flags & MAFX_TERM_HOST_TENSOR) || full32 ||
                (f64 && !(ctx->demag_flags & MAFX_TERM_OCTANT_TENSOR))) {
         // the reference's tensor, bit for bit, over the whole expanded grid
         auto Nh = host_tensor(ctx->mesh, ctx->atom_dipole, P);
@@ -1015,7 +1025,7 @@ void build_tensor_uncached(mafx_ctx* ctx) {
         newell_octant_extend(g, oct.data(), full.data());
         MAFX_HIP(hipMemcpy(Nd.p, full.data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
     }
-    if (!f64) zero_unpaired_planes(ctx, Nd.as<double>(), 6);
+    if (!f64 && !full32) zero_unpaired_planes(ctx, Nd.as<double>(), 6);
     T64.alloc(sizeof(double2) * 6 * K);
     MAFX_HIP(hipMemset(T64.p, 0, T64.bytes)); // padding columns stay defined
     double2* t = T64.as<double2>();
@@ -1031,6 +1041,12 @@ void build_tensor_uncached(mafx_ctx* ctx) {
     if (f64) {
         std::swap(ctx->tensor.p, T64.p);
         std::swap(ctx->tensor.bytes, T64.bytes);
+    } else if (full32) {
+        ctx->tensor.alloc(sizeof(float2) * 6 * K);
+        hipLaunchKernelGGL(k_round_c32, dim3(grid_for(6 * K, 256)), dim3(256), 0, ctx->stream, t,
+                           ctx->tensor.as<float2>(), 6 * K);
+        MAFX_HIP(hipGetLastError());
+        MAFX_HIP(hipStreamSynchronize(ctx->stream));
     } else {
         ctx->tensor.alloc(sizeof(float) * 6 * ctx->n2h * ctx->n1h * hxs);
         tensor_sym_f32(ctx, t, ctx->tensor.as<float>());

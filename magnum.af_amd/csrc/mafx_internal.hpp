@@ -125,6 +125,11 @@ struct Workspace; // precision-specific demag buffers
 // The demag / dipole FFT convolution runs in FP64 (complex reference tensor) for MAFX_F64 and
 // MAFX_F32_STENCILS, in FP32 (symmetric real tensor) for MAFX_F32_FIELD and MAFX_F32_DEMAG.
 inline bool demag_f64(int precision) { return precision == MAFX_F64 || precision == MAFX_F32_STENCILS; }
+// FP32 demag on the complex half spectrum of the whole tensor (MAFX_TERM_FULL_TENSOR) instead of
+// the real symmetric quarter
+inline bool demag_full_c32(int precision, int demag_flags) {
+    return !demag_f64(precision) && (demag_flags & MAFX_TERM_FULL_TENSOR) != 0;
+}
 inline bool valid_precision(int p) {
     return p == MAFX_F64 || p == MAFX_F32_FIELD || p == MAFX_F32_DEMAG || p == MAFX_F32_STENCILS;
 }

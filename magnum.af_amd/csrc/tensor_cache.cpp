@@ -86,6 +86,8 @@ fs::path tensor_cache_dir() {
 }
 
 int32_t tensor_cache_layout(int precision, int term_flags) {
+    if (precision != MAFX_F64 && (term_flags & MAFX_TERM_FULL_TENSOR))
+        return precision | kCacheFullTensor | ((term_flags & MAFX_TERM_DEVICE_TENSOR) ? kCacheDeviceTensor : 0);
     if (term_flags & MAFX_TERM_DEVICE_TENSOR) return precision | kCacheDeviceTensor;
     if (precision == MAFX_F64) return (term_flags & MAFX_TERM_OCTANT_TENSOR) && !(term_flags & MAFX_TERM_HOST_TENSOR)
                                           ? kCacheOctantTensor : 0;
@@ -102,7 +104,9 @@ std::string tensor_cache_name(const TensorCacheKey& k) {
     s += layout == 0 ? ((k.precision & kCacheDeviceTensor)   ? "_mafx_c64_dev"
                         : (k.precision & kCacheOctantTensor) ? "_mafx_c64_oct"
                                                              : "_mafx_c64")
-                     : ((k.precision & kCacheHostTensor)     ? "_mafx_f32sym_host"
+                     : ((k.precision & kCacheFullTensor)     ? ((k.precision & kCacheDeviceTensor) ? "_mafx_c32_dev"
+                                                                                                   : "_mafx_c32_host")
+                        : (k.precision & kCacheHostTensor)     ? "_mafx_f32sym_host"
                         : (k.precision & kCacheDeviceTensor) ? "_mafx_f32sym_dev"
                                                              : "_mafx_f32oct");
     if (k.dipole) s += "_dipole";

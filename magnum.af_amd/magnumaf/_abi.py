@@ -13,7 +13,7 @@ PRECISIONS = {"f64": MAFX_F64, "f32": MAFX_F32_FIELD, "f32_demag": MAFX_F32_DEMA
 DEMAG, EXCHANGE, DMI, UNIAXIAL, ZEEMAN, RKKY = 1, 2, 3, 4, 5, 6
 # mafx_term_desc.flags (include/mafx.h)
 TERM_CACHING, TERM_VERBOSE, TERM_HOST_TENSOR, TERM_HOMOGENEOUS_CALLBACK, TERM_HOST_CALLBACK = 1, 2, 4, 8, 16
-TERM_DEVICE_TENSOR, TERM_OCTANT_TENSOR = 32, 64
+TERM_DEVICE_TENSOR, TERM_OCTANT_TENSOR, TERM_FULL_TENSOR = 32, 64, 128
 ATOM_DIPOLE, ATOM_EXCHANGE, ATOM_DMI, ATOM_UNIAXIAL, ATOM_ZEEMAN = 11, 12, 13, 14, 15
 
 # every entry point include/mafx.h declares

@@ -311,9 +311,13 @@ def test_fp32_demag_far_field_512x512x2():
     h64_oct = field("f64", octant_tensor=True)
     h32 = field("f32")
     h64_dev = field("f64", device_tensor=True)
+    h32_full = field("f32", full_tensor=True)
     e = {"fp64": normwise(h64, ref), "fp64_octant": normwise(h64_oct, ref), "fp64_gpu_tensor": normwise(h64_dev, ref),
-         "fp32_vs_fp64_octant": normwise(h32, h64_oct), "fp32": normwise(h32, ref)}
+         "fp32_vs_fp64_octant": normwise(h32, h64_oct), "fp32": normwise(h32, ref),
+         "fp32_full_tensor": normwise(h32_full, ref)}
     print(f"[fp32 demag far field] {e}", flush=True)
     assert e["fp64"] <= FP64_TOL
     assert e["fp32_vs_fp64_octant"] <= 1e-5          # FP32 arithmetic alone
     assert e["fp32"] <= 1e-3 and e["fp64_octant"] <= 1e-3  # tensor noise of another evaluation order
+    # on the reference's own tensor spectrum (MAFX_TERM_FULL_TENSOR) only the FP32 arithmetic is left
+    assert e["fp32_full_tensor"] <= 1e-5
