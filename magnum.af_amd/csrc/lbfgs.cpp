@@ -260,18 +260,32 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
     const int64_t n3 = L.n3;
     const size_t bytes = sizeof(double) * n3;
     constexpr size_t hist = 5;
-    std::array<DevBuf, hist> sV, yV;
-    DevBuf q, s, y, grad, grad_old, x_old, dir;
-    for (auto* b : {&q, &s, &y, &grad, &grad_old, &x_old, &dir}) b->alloc(bytes);
-    for (size_t i = 0; i < hist; ++i) {
-        sV[i].alloc(bytes); // read only after being written (k <= iter)
-        yV[i].alloc(bytes);
+    // Buffers.  The reference copies x_old = m and grad_old = grad at the top of every iteration
+    // and copies each new pair into its history; here the iterate, the gradient and the history
+    // slots are pointers that change roles, so no vector is copied.  mcur / gcur hold the current
+    // iterate and gradient, mold / gold the previous ones; the history is a ring of hist + 1 slots
+    // whose spare slot receives the next pair.
+    DevBuf xbuf, gbuf0, gbuf1, q, dir;
+    for (auto* b : {&xbuf, &gbuf0, &gbuf1, &q, &dir}) b->alloc(bytes);
+    std::array<DevBuf, hist + 1> sB, yB;
+    for (size_t i = 0; i <= hist; ++i) {
+        sB[i].alloc(bytes); // read only after being written (k <= iter)
+        yB[i].alloc(bytes);
     }
+    std::array<double*, hist + 1> sV, yV; // [0, hist) stored pairs, oldest first; [hist] the spare
+    for (size_t i = 0; i <= hist; ++i) {
+        sV[i] = sB[i].as<double>();
+        yV[i] = yB[i].as<double>();
+    }
+    double* mcur = m;
+    double* mold = xbuf.as<double>();
+    double* gcur = gbuf0.as<double>();
+    double* gold = gbuf1.as<double>();
     // rho_i = 1 / (s_i . y_i) of the stored pairs: the reference recomputes the dot product at every
     // use (lbfgs_minimizer.cpp:88,98); it is the y . s computed when the pair was stored (the dot
     // is commutative and deterministic), kept here.  Device scalars of the two-loop recursion:
-    // [0, hist) alpha_i, [hist] the current dot, [hist + 1] the axpy coefficient, [hist + 2, +3)
-    // y . s, y . y, s . s of a new pair.
+    // [0, hist) alpha_i, [hist] the current dot, [hist + 1] the axpy coefficient, [hist + 2, +5)
+    // y . s, y . y, s . s, max|s|, max|m| of a new pair.
     std::array<double, hist> rho{};
     DevBuf dsc;
     dsc.alloc(sizeof(double) * 16);
@@ -290,10 +304,12 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
     std::FILE* const conv = ctx->lbfgs_convergence;
     size_t finish_from = 0; // globIter at the move; 0 = no move
     bool move_now = false;
-    double f = L.energy_and_gradient(m, grad.as<double>());
-    double gradNorm = L.maxabs(grad.as<double>());
+    double f = L.energy_and_gradient(mcur, gcur);
+    double gradNorm = L.maxabs(gcur);
     size_t globIter = 0;
     auto finish = [&](int r) {
+        if (mcur != m) copy(ctx, m, mcur, n3);
+        MAFX_HIP(hipStreamSynchronize(ctx->stream));
         if (E_out) *E_out = f;
         if (stats) {
             stats->iterations = static_cast<int64_t>(globIter);
@@ -317,40 +333,52 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
     double H0k = 1;
     do {
         f_old = f;
-        copy(ctx, x_old.as<double>(), m, n3);
-        copy(ctx, grad_old.as<double>(), grad.as<double>(), n3);
-        copy(ctx, q.as<double>(), grad.as<double>(), n3);
+        // x_old = m, grad_old = grad: the current iterate and gradient become the old ones, and the
+        // line search writes the new ones into the other buffers
+        std::swap(mcur, mold);
+        std::swap(gcur, gold);
         const int k = static_cast<int>(std::min(hist, iter));
-        // the two-loop recursion with its scalars on the device: alpha_i = rho_i s_i . q,
-        // q -= alpha_i y_i; then beta = rho_i y_i . q, q += (alpha_i - beta) s_i
-        for (int i = k - 1; i >= 0; i--) {
-            dot_dev(ctx, sV[i].as<double>(), q.as<double>(), n3, d_dot);
-            lbfgs_alpha(ctx, ds + i, d_coef, d_dot, rho[i]);
-            axpy_dev(ctx, q.as<double>(), d_coef, yV[i].as<double>(), n3);
+        // The two-loop recursion q = H grad (lbfgs_minimizer.cpp:86-108) with its scalars on the
+        // device: alpha_i = rho_i s_i . q, q -= alpha_i y_i; q *= H0k; beta = rho_i y_i . q,
+        // q += (alpha_i - beta) s_i; then searchDir = -q and phiPrime0 = -grad . q.  Each axpy runs
+        // fused with the dot product that follows it, the first one reads grad in place of the copy
+        // q = grad, and the last one writes -q straight into the search direction.
+        if (k == 0) {
+            lbfgs_fused(ctx, gold, nullptr, nullptr, H0k, true, gold, dir.as<double>(), true, n3, d_dot);
+        } else {
+            dot_dev(ctx, sV[k - 1], gold, n3, d_dot);
+            lbfgs_alpha(ctx, ds + (k - 1), d_coef, d_dot, rho[k - 1]);
+            for (int i = k - 1; i >= 1; i--) {
+                lbfgs_fused(ctx, i == k - 1 ? gold : q.as<double>(), d_coef, yV[i], 1.0, false, sV[i - 1],
+                            q.as<double>(), false, n3, d_dot);
+                lbfgs_alpha(ctx, ds + (i - 1), d_coef, d_dot, rho[i - 1]);
+            }
+            lbfgs_fused(ctx, k == 1 ? gold : q.as<double>(), d_coef, yV[0], H0k, true, yV[0], q.as<double>(), false,
+                        n3, d_dot);
+            lbfgs_beta(ctx, d_coef, ds + 0, d_dot, rho[0]);
+            for (int i = 0; i < k; i++) {
+                const bool last = i == k - 1;
+                lbfgs_fused(ctx, q.as<double>(), d_coef, sV[i], 1.0, false, last ? gold : yV[i + 1],
+                            last ? dir.as<double>() : q.as<double>(), last, n3, d_dot);
+                if (!last) lbfgs_beta(ctx, d_coef, ds + (i + 1), d_dot, rho[i + 1]);
+            }
         }
-        scale(ctx, q.as<double>(), H0k, n3);
-        for (int i = 0; i < k; i++) {
-            dot_dev(ctx, yV[i].as<double>(), q.as<double>(), n3, d_dot);
-            lbfgs_beta(ctx, d_coef, ds + i, d_dot, rho[i]);
-            axpy_dev(ctx, q.as<double>(), d_coef, sV[i].as<double>(), n3);
-        }
-        const double phiPrime0 = -L.mydot(grad.as<double>(), q.as<double>());
+        double phi;
+        read_device(ctx, d_dot, 1, &phi);
+        const double phiPrime0 = -phi;
         // the line search's g . dir for dir = -q is phiPrime0 exactly (every product and partial
         // sum only changes sign); after the reset to q = grad it is computed
         double dginit = phiPrime0;
         if (phiPrime0 > 0) {
-            copy(ctx, q.as<double>(), grad.as<double>(), n3);
+            neg_copy(ctx, dir.as<double>(), gold, n3);
             iter = 0;
+            dginit = L.mydot(gold, dir.as<double>());
         }
-        // searchDir = -q (multiplying by -1 is exact)
-        copy(ctx, dir.as<double>(), q.as<double>(), n3);
-        scale(ctx, dir.as<double>(), -1.0, n3);
-        if (phiPrime0 > 0) dginit = L.mydot(grad.as<double>(), dir.as<double>());
         double rate = 1.0;
         double gmax = -1.0;
-        cvsrch(L, m, x_old.as<double>(), f, grad.as<double>(), rate, dir.as<double>(), tolerance, dginit, gmax);
+        cvsrch(L, mcur, mold, f, gcur, rate, dir.as<double>(), tolerance, dginit, gmax);
         const double f1 = 1 + std::fabs(f);
-        gradNorm = gmax >= 0.0 ? gmax : L.maxabs(grad.as<double>());
+        gradNorm = gmax >= 0.0 ? gmax : L.maxabs(gcur);
         if (verbose > 1) {
             std::printf("bb> %zu %.17g  %.17g %lld %.17g\n", globIter, f, gradNorm, static_cast<long long>(L.evals),
                         rate);
@@ -361,9 +389,13 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
             reason = 1;
             break;
         }
-        sub(ctx, s.as<double>(), m, x_old.as<double>(), n3);
-        const double smax = L.maxabs(s.as<double>());
-        const double mmax = L.maxabs(m);
+        // s = m - x_old and y = grad - grad_old into the spare history slot, with max|s|, max|m| for
+        // the stop test and y . s, |y|^2, |s|^2 for the update (lbfgs_minimizer.cpp:137-170), in one
+        // pass and one transfer
+        double pr[5];
+        lbfgs_pair(ctx, mcur, mold, gcur, gold, sV[hist], yV[hist], n3, ds + hist + 2);
+        read_device(ctx, ds + hist + 2, 5, pr);
+        const double ys = pr[0], yy = pr[1], ss = pr[2], smax = pr[3], mmax = pr[4];
         // lbfgs_minimizer.cpp:148-153: the three stop ratios, one line per iteration
         if (conv) {
             std::fprintf(conv, "%.27g\t%.27g\t%.27g\n", (f_old - f) / (tolerance * f1),
@@ -393,29 +425,21 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
                 break;
             }
         }
-        sub(ctx, y.as<double>(), grad.as<double>(), grad_old.as<double>(), n3);
-        // y . s, |y|^2 and |s|^2 (lbfgs_minimizer.cpp:160-170) in one transfer
-        dot_dev(ctx, y.as<double>(), s.as<double>(), n3, ds + hist + 2);
-        dot_dev(ctx, y.as<double>(), y.as<double>(), n3, ds + hist + 3);
-        dot_dev(ctx, s.as<double>(), s.as<double>(), n3, ds + hist + 4);
-        double pr[3];
-        read_device(ctx, ds + hist + 2, 3, pr);
-        const double ys = pr[0], yy = pr[1], ss = pr[2];
         if (ys <= eps2 * std::sqrt(yy) * std::sqrt(ss)) {
-            // Dennis-Schnabel 9.4.1: skip the update
+            // Dennis-Schnabel 9.4.1: skip the update (the spare slot is simply reused)
         } else {
-            if (iter < hist) {
-                copy(ctx, sV[iter].as<double>(), s.as<double>(), n3);
-                copy(ctx, yV[iter].as<double>(), y.as<double>(), n3);
-                rho[iter] = 1.0 / ys;
-            } else {
-                std::rotate(sV.begin(), sV.begin() + 1, sV.end());
-                std::rotate(yV.begin(), yV.begin() + 1, yV.end());
+            // store the pair: into slot iter while the history fills, else drop the oldest
+            // (lbfgs_minimizer.cpp:160-176; the reference rotates the vectors, here the pointers)
+            size_t slot = iter;
+            if (iter >= hist) {
+                std::rotate(sV.begin(), sV.begin() + 1, sV.begin() + hist);
+                std::rotate(yV.begin(), yV.begin() + 1, yV.begin() + hist);
                 std::rotate(rho.begin(), rho.begin() + 1, rho.end());
-                copy(ctx, sV[hist - 1].as<double>(), s.as<double>(), n3);
-                copy(ctx, yV[hist - 1].as<double>(), y.as<double>(), n3);
-                rho[hist - 1] = 1.0 / ys;
+                slot = hist - 1;
             }
+            std::swap(sV[slot], sV[hist]);
+            std::swap(yV[slot], yV[hist]);
+            rho[slot] = 1.0 / ys;
             H0k = ys / yy;
             iter++;
         }
@@ -427,7 +451,7 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
             MAFX_HIP(hipStreamSynchronize(ctx->stream));
             ctx = finish_ctx;
             L.ctx = finish_ctx;
-            f = L.energy_and_gradient(m, grad.as<double>());
+            f = L.energy_and_gradient(mcur, gcur);
             stalled = 0;
             finish_from = globIter;
             if (verbose > 1) {
@@ -442,7 +466,7 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         std::printf("WARNING : maximum number of iterations exceeded in LBFGS\n");
     if (verbose > 0 && reason != 1) {
         EvalArgs ea;
-        ea.m = grad.as<double>();
+        ea.m = gcur;
         ea.mode = OutMode::GRAD;
         ea.out = q.as<double>();
         evaluate(ctx, ea);
@@ -450,7 +474,6 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         std::printf("aa> %zu %.27g  %.27g\n", globIter, f - f_old, mxh);
         std::fflush(stdout);
     }
-    MAFX_HIP(hipStreamSynchronize(ctx->stream));
     return finish(reason);
 }
 

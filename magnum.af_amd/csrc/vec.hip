@@ -382,6 +382,94 @@ void max_abs_dev(mafx_ctx* ctx, const double* a, int64_t n, double* out_dev) {
     MAFX_HIP(hipGetLastError());
 }
 
+namespace {
+// One step of the LBFGS two-loop recursion with the next step's dot product: qi = q_in + (*coef) v
+// (no axpy when coef is null), then qi = scale qi when do_scale, stored as q_out = qi or, negate,
+// q_out = -qi; and partials of dnext . qi.  Every expression and the element-to-thread mapping are
+// those of k_axpy_dev, k_scale and k_dot, so the results equal the unfused sequence bit for bit.
+__global__ void __launch_bounds__(kB) k_lbfgs_fused(const double* q_in, const double* __restrict__ coef,
+                                                   const double* __restrict__ v, double scale, int do_scale,
+                                                   const double* __restrict__ dnext, double* q_out, int negate,
+                                                   int64_t n, double* __restrict__ part) {
+    double s = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * kB) {
+        double qi = q_in[i];
+        if (coef) qi = qi + (*coef) * v[i];
+        if (do_scale) qi = scale * qi;
+        q_out[i] = negate ? -qi : qi;
+        s += dnext[i] * qi;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// The new curvature pair: s = m - x_old, y = g - g_old, with partials [slot][block] of y.s, y.y,
+// s.s (sums) and max|s|, max|m| (maxima), each as k_sub, k_dot and k_maxabs form them.
+__global__ void __launch_bounds__(kB) k_lbfgs_pair(const double* __restrict__ m, const double* __restrict__ xo,
+                                                  const double* __restrict__ g, const double* __restrict__ go,
+                                                  double* __restrict__ s_out, double* __restrict__ y_out, int64_t n,
+                                                  double* __restrict__ part) {
+    double ys = 0, yy = 0, ss = 0, smax = 0, mmax = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * kB) {
+        const double mi = m[i];
+        const double si = mi - xo[i];
+        const double yi = g[i] - go[i];
+        s_out[i] = si;
+        y_out[i] = yi;
+        ys += yi * si;
+        yy += yi * yi;
+        ss += si * si;
+        smax = fmax(smax, fabs(si));
+        mmax = fmax(mmax, fabs(mi));
+    }
+    ys = block_sum(ys);
+    yy = block_sum(yy);
+    ss = block_sum(ss);
+    smax = block_max(smax);
+    mmax = block_max(mmax);
+    if (threadIdx.x == 0) {
+        const int64_t nb = gridDim.x;
+        part[0 * nb + blockIdx.x] = ys;
+        part[1 * nb + blockIdx.x] = yy;
+        part[2 * nb + blockIdx.x] = ss;
+        part[3 * nb + blockIdx.x] = smax;
+        part[4 * nb + blockIdx.x] = mmax;
+    }
+}
+
+__global__ void __launch_bounds__(kB) k_neg(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    if (i < n) y[i] = -x[i];
+}
+} // namespace
+
+void lbfgs_fused(mafx_ctx* ctx, const double* q_in, const double* coef_dev, const double* v, double scale,
+                 bool do_scale, const double* dnext, double* q_out, bool negate, int64_t n, double* out_dev) {
+    const unsigned nb = red_blocks(n);
+    ProfScope ps(ctx, "lbfgs_twoloop_f64", 8.0 * n * (coef_dev ? 4 : 3));
+    hipLaunchKernelGGL(k_lbfgs_fused, dim3(nb), dim3(kB), 0, ctx->stream, q_in, coef_dev, v, scale, do_scale ? 1 : 0,
+                       dnext, q_out, negate ? 1 : 0, n, ctx->partials.as<double>());
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(), static_cast<int>(nb),
+                       1, 1, out_dev);
+    MAFX_HIP(hipGetLastError());
+}
+
+void lbfgs_pair(mafx_ctx* ctx, const double* m, const double* x_old, const double* g, const double* g_old,
+                double* s, double* y, int64_t n, double* out_dev) {
+    const unsigned nb = red_blocks(n);
+    ProfScope ps(ctx, "lbfgs_pair_f64", 48.0 * n);
+    hipLaunchKernelGGL(k_lbfgs_pair, dim3(nb), dim3(kB), 0, ctx->stream, m, x_old, g, g_old, s, y, n,
+                       ctx->partials.as<double>());
+    hipLaunchKernelGGL(k_final, dim3(5), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(), static_cast<int>(nb),
+                       5, 3, out_dev);
+    MAFX_HIP(hipGetLastError());
+}
+
+void neg_copy(mafx_ctx* ctx, double* y, const double* x, int64_t n) {
+    hipLaunchKernelGGL(k_neg, dim3(grid_for(n, kB)), dim3(kB), 0, ctx->stream, y, x, n);
+    MAFX_HIP(hipGetLastError());
+}
+
 void axpy_dev(mafx_ctx* ctx, double* y, const double* a_dev, const double* x, int64_t n) {
     hipLaunchKernelGGL(k_axpy_dev, dim3(grid_for(n, kB)), dim3(kB), 0, ctx->stream, y, a_dev, x, n);
     MAFX_HIP(hipGetLastError());
@@ -520,6 +608,7 @@ void copy(mafx_ctx* ctx, double* dst, const double* src, int64_t n) {
     MAFX_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
 }
 
-int64_t reduction_partials_needed(int64_t n) { return static_cast<int64_t>(red_blocks(n)) * 4; }
+// the widest reductions over n entries use 5 slots (lbfgs_pair)
+int64_t reduction_partials_needed(int64_t n) { return static_cast<int64_t>(red_blocks(n)) * 5; }
 
 } // namespace mafx

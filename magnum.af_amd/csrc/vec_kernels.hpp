@@ -27,6 +27,16 @@ void axpy_dev(mafx_ctx* ctx, double* y, const double* a_dev, const double* x, in
 void lbfgs_alpha(mafx_ctx* ctx, double* alpha_i, double* coef, const double* d, double rho);
 void lbfgs_beta(mafx_ctx* ctx, double* coef, const double* alpha_i, const double* d, double rho);
 void read_device(mafx_ctx* ctx, const double* src_dev, int n, double* dst_host);
+// LBFGS fused vector work, bit for bit the unfused sequence's results:
+// lbfgs_fused: q_out = (q_in + (*coef_dev) v) [* scale] (negated when `negate`), *out_dev = dnext . q
+//   (before negation), with dot()'s tree; coef_dev null skips the axpy.  q_out may alias q_in.
+// lbfgs_pair: s = m - x_old, y = g - g_old, out_dev[0..4] = y.s, y.y, s.s, max|s|, max|m|.
+// neg_copy: y = -x.
+void lbfgs_fused(mafx_ctx* ctx, const double* q_in, const double* coef_dev, const double* v, double scale,
+                 bool do_scale, const double* dnext, double* q_out, bool negate, int64_t n, double* out_dev);
+void lbfgs_pair(mafx_ctx* ctx, const double* m, const double* x_old, const double* g, const double* g_old,
+                double* s, double* y, int64_t n, double* out_dev);
+void neg_copy(mafx_ctx* ctx, double* y, const double* x, int64_t n);
 // <m> of mafx_mean_m with its partial sums in a static device array: no allocation, no
 // synchronisation, result in out_dev[0..2] (device memory), launched on `stream`.
 void mean_m_noalloc(const double* m, int64_t N, double* out_dev, hipStream_t stream);
