@@ -117,10 +117,13 @@ typedef int (*mafx_field_fn)(void* user, double t, const double* m, double* h_ou
 #define MAFX_TERM_HOST_CALLBACK 16
 #define MAFX_TERM_DEVICE_TENSOR 32
 #define MAFX_TERM_OCTANT_TENSOR 64
-/* DEMAG on the FP32 paths: keep the complex half spectrum of the whole host tensor, rounded to
- * complex64, in place of the real symmetric quarter.  The FP32 arithmetic then runs on the
- * reference's tensor itself, far-field noise and its asymmetry included, for 6.4x the tensor bytes
- * per evaluation (0.81 instead of 0.13 GB at 1024x1024x4).  Ignored by the FP64 paths. */
+/* DEMAG on the FP32 paths: apply the spectrum of the reference's whole tensor (far-field rounding
+ * noise and its asymmetry included) instead of the parity-symmetric one.  It is stored as the
+ * symmetric real quarter (FP32) plus the residual over the whole half spectrum (complex FP16
+ * scaled per component by a power of two; the residual is ~1e-4 of the entries, so its rounding
+ * leaves ~1e-7): 0.13 + 0.40 GB at 1024x1024x4 instead of 0.13 GB.  With
+ * MAFX_TERM_DEVICE_TENSOR the GPU-built whole tensor takes the host tensor's place.  Ignored by
+ * the FP64 paths. */
 #define MAFX_TERM_FULL_TENSOR 128
 typedef struct mafx_term_desc {
     int32_t kind;

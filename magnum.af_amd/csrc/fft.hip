@@ -83,6 +83,13 @@ FftDesc desc_of(const FftPlan& p) {
 
 namespace {
 
+// The reference spectrum's entry c at a k-point: the symmetric quarter's (signed) value plus the
+// stored FP16 residual, unscaled.
+__device__ __forceinline__ float2 res_entry(const TensorRes& res, int c, int64_t idx, float sym) {
+    const float2 r = __half22float2(res.r[idx]);
+    return float2{sym + r.x * res.inv[c], r.y * res.inv[c]};
+}
+
 // 512-thread work-groups for the axis transforms: W L <= 64 KiB of complex data spread over twice
 // the lanes halves the butterflies each lane keeps in registers across a stage.
 constexpr int kAxisBlock = 512;
@@ -826,6 +833,37 @@ __global__ void k_zero_unpaired(double* __restrict__ N, int64_t n0, int64_t n1, 
     }
 }
 
+// R = T - S: the full FP64 spectrum T minus the symmetric quarter's value at the same k-point,
+// in place on T ([6][n2][plane], the spectrum layout); zero in the padding columns kx >= hx.
+__global__ void k_tensor_residual(double2* __restrict__ T, const float* __restrict__ Ts, int64_t n1, int64_t n2,
+                                  int64_t hx, int64_t hxs, int64_t n1h, int64_t n2h, int64_t total) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const int64_t plane = n1 * hxs, K = n2 * plane;
+    const int64_t c = e / K, rem = e - c * K, kz = rem / plane, p = rem - kz * plane;
+    const int64_t kt = p / (kTile * n1), r = p - kt * (kTile * n1), ky = r / kTile, kx = kt * kTile + r % kTile;
+    if (kx >= hx) {
+        T[e] = double2{0.0, 0.0};
+        return;
+    }
+    const bool my = ky >= n1h, mz = kz >= n2h;
+    const int64_t kys = my ? n1 - ky : ky, kzs = mz ? n2 - kz : kz;
+    const double sy = my ? -1.0 : 1.0, sz = mz ? -1.0 : 1.0;
+    const double sg = c == 1 ? sy : c == 2 ? sz : c == 4 ? sy * sz : 1.0;
+    const int64_t Q = n1h * hxs;
+    const double sv = sg * static_cast<double>(Ts[(c * n2h + kzs) * Q + tcol(kx, kTile * n1h) + kTile * kys]);
+    T[e] = double2{T[e].x - sv, T[e].y};
+}
+
+// R scaled by the component's power of two, rounded to FP16 pairs
+__global__ void k_residual_f16(const double2* __restrict__ R, __half2* __restrict__ out, int64_t K, TensorScale sc,
+                               int64_t total) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const double f = sc.s[e / K];
+    out[e] = __floats2half2_rn(static_cast<float>(R[e].x * f), static_cast<float>(R[e].y * f));
+}
+
 // Ts holds, per component and kz <= n2/2, a plane of n1h = n1/2 + 1 rows ky in column tiles:
 // column kx of row kys at tcol(kx, 4 n1h) + 4 kys.  T is the transformed tensor in the spectrum
 // layout (planes of n1 rows).
@@ -840,9 +878,10 @@ __global__ void k_tensor_sym_f32(const double2* __restrict__ T, float* __restric
 
 // H^ = N^ M^ with the symmetric tensor, in place on the spectrum [3][n2][plane]; one thread per
 // plane position e (column tiles, padding columns kx >= hx skipped).
+template <bool RES>
 __global__ void __launch_bounds__(256)
     k_kspace_mul_sym(float2* __restrict__ s, const float* __restrict__ t, int n1, int n2, int hx, FastDiv pld,
-                     FastDiv n1Td, int hxs, int n1h, int n2h, int total) {
+                     FastDiv n1Td, int hxs, int n1h, int n2h, int total, TensorRes res) {
     const int e = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
     if (e >= total) return;
     const int kz = static_cast<int>(fdiv(e, pld)), p = e - kz * static_cast<int>(pld.d);
@@ -858,6 +897,17 @@ __global__ void __launch_bounds__(256)
                 nyz = sy * sz * t[4 * tc + to], nzz = t[5 * tc + to];
     const int64_t K = static_cast<int64_t>(total);
     const float2 m0 = s[e], m1 = s[K + e], m2 = s[2 * K + e];
+    if constexpr (RES) {
+        // the reference spectrum: the symmetric value plus the stored residual (complex)
+        float2 T[6];
+        const float sv[6] = {nxx, nxy, nxz, nyy, nyz, nzz};
+#pragma unroll
+        for (int c = 0; c < 6; ++c) T[c] = res_entry(res, c, c * K + e, sv[c]);
+        s[e] = cadd(cadd(cmul(T[0], m0), cmul(T[1], m1)), cmul(T[2], m2));
+        s[K + e] = cadd(cadd(cmul(T[1], m0), cmul(T[3], m1)), cmul(T[4], m2));
+        s[2 * K + e] = cadd(cadd(cmul(T[2], m0), cmul(T[4], m1)), cmul(T[5], m2));
+        return;
+    }
     s[e] = float2{nxx * m0.x + nxy * m1.x + nxz * m2.x, nxx * m0.y + nxy * m1.y + nxz * m2.y};
     s[K + e] = float2{nxy * m0.x + nyy * m1.x + nyz * m2.x, nxy * m0.y + nyy * m1.y + nyz * m2.y};
     s[2 * K + e] = float2{nxz * m0.x + nyz * m1.x + nzz * m2.x, nxz * m0.y + nyz * m1.y + nzz * m2.y};
@@ -867,10 +917,10 @@ __global__ void __launch_bounds__(256)
 // (kx mod 4, kys, kx / 4) order so neighbouring lanes touch neighbouring addresses of the column
 // tiles: it loads the tensor column once (6 (NZ + 1) reals) and serves both spectrum columns
 // ky = kys and ky = n1 - kys.
-template <int NZ>
+template <int NZ, bool RES>
 __global__ void __launch_bounds__(256)
     k_zfused_sym(float2* __restrict__ s, const float* __restrict__ t, int n1, int hx, int hxs, FastDiv n1hd,
-                 ZTw<float2> tw) {
+                 ZTw<float2> tw, TensorRes res) {
     constexpr int NZ2 = 2 * NZ, NH = NZ + 1;
     const int e = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
     const int r = e >> 2, kt = static_cast<int>(fdiv(r, n1hd)), kys = r - kt * static_cast<int>(n1hd.d);
@@ -913,9 +963,21 @@ __global__ void __launch_bounds__(256)
             const float sz = k <= NZ ? 1.0f : -1.0f;
             const float nxx = tv[0][kzs], nxy = sy * tv[1][kzs], nxz = sz * tv[2][kzs], nyy = tv[3][kzs],
                         nyz = sy * sz * tv[4][kzs], nzz = tv[5][kzs];
-            const float2 H0{nxx * X[0].x + nxy * X[1].x + nxz * X[2].x, nxx * X[0].y + nxy * X[1].y + nxz * X[2].y};
-            const float2 H1{nxy * X[0].x + nyy * X[1].x + nyz * X[2].x, nxy * X[0].y + nyy * X[1].y + nyz * X[2].y};
-            const float2 H2{nxz * X[0].x + nyz * X[1].x + nzz * X[2].x, nxz * X[0].y + nyz * X[1].y + nzz * X[2].y};
+            float2 H0, H1, H2;
+            if constexpr (RES) {
+                // the reference spectrum at (kx, ky, k): symmetric value plus stored residual
+                float2 T[6];
+                const float sv[6] = {nxx, nxy, nxz, nyy, nyz, nzz};
+#pragma unroll
+                for (int c = 0; c < 6; ++c) T[c] = res_entry(res, c, (c * NZ2 + k) * plane + col, sv[c]);
+                H0 = cadd(cadd(cmul(T[0], X[0]), cmul(T[1], X[1])), cmul(T[2], X[2]));
+                H1 = cadd(cadd(cmul(T[1], X[0]), cmul(T[3], X[1])), cmul(T[4], X[2]));
+                H2 = cadd(cadd(cmul(T[2], X[0]), cmul(T[4], X[1])), cmul(T[5], X[2]));
+            } else {
+                H0 = float2{nxx * X[0].x + nxy * X[1].x + nxz * X[2].x, nxx * X[0].y + nxy * X[1].y + nxz * X[2].y};
+                H1 = float2{nxy * X[0].x + nyy * X[1].x + nyz * X[2].x, nxy * X[0].y + nyy * X[1].y + nyz * X[2].y};
+                H2 = float2{nxz * X[0].x + nyz * X[1].x + nzz * X[2].x, nxz * X[0].y + nyz * X[1].y + nzz * X[2].y};
+            }
 #pragma unroll
             for (int z = 0; z < NZ; ++z) {
                 const float2 w = conj_if(tw.w[(k * z) % NZ2], true);
@@ -1211,21 +1273,45 @@ void tensor_sym_f32(mafx_ctx* ctx, const double2* T, float* Ts) {
     MAFX_HIP(hipGetLastError());
 }
 
-void kspace_mul_sym(mafx_ctx* ctx, float2* spec, const float* ts, int64_t n2) {
-    const int64_t plane = ctx->n1 * ctx->hxs, total = n2 * plane;
-    require(total < (int64_t{1} << 31), "spectrum too large for the symmetric tensor product");
-    const double bytes = static_cast<double>(n2 * ctx->n1 * ctx->hx) * (6.0 * sizeof(float2)) +
-                         6.0 * sizeof(float) * static_cast<double>(ctx->n2h * ctx->n1h * ctx->hx);
-    ProfScope ps(ctx, "kspace_mul_sym_f32", bytes);
-    hipLaunchKernelGGL(k_kspace_mul_sym, dim3(grid_for(total, 256)), dim3(256), 0, ctx->stream, spec, ts,
-                       static_cast<int>(ctx->n1), static_cast<int>(n2), static_cast<int>(ctx->hx),
-                       make_fastdiv(static_cast<uint32_t>(plane)), make_fastdiv(static_cast<uint32_t>(kTile * ctx->n1)),
-                       static_cast<int>(ctx->hxs), static_cast<int>(ctx->n1h), static_cast<int>(ctx->n2h),
-                       static_cast<int>(total));
+void tensor_residual_f16(mafx_ctx* ctx, double2* T, const float* Ts, __half2* out, float inv_scale[6]) {
+    const int64_t K = ctx->n2 * ctx->n1 * ctx->hxs, total = 6 * K;
+    hipLaunchKernelGGL(k_tensor_residual, dim3(grid_for(total, 256)), dim3(256), 0, ctx->stream, T, Ts, ctx->n1,
+                       ctx->n2, ctx->hx, ctx->hxs, ctx->n1h, ctx->n2h, total);
+    MAFX_HIP(hipGetLastError());
+    // per component: the power of two that takes max |R| to within [8192, 16384), far from FP16's
+    // overflow (65504), so the largest residuals keep FP16's full 11-bit precision
+    TensorScale sc{};
+    for (int c = 0; c < 6; ++c) {
+        const double mx = max_abs(ctx, reinterpret_cast<const double*>(T + c * K), 2 * K);
+        int ex = 0;
+        if (mx > 0) {
+            std::frexp(mx, &ex); // mx = f 2^ex, f in [0.5, 1)
+        }
+        sc.s[c] = mx > 0 ? std::ldexp(1.0, 14 - ex) : 1.0;
+        inv_scale[c] = static_cast<float>(1.0 / sc.s[c]);
+    }
+    hipLaunchKernelGGL(k_residual_f16, dim3(grid_for(total, 256)), dim3(256), 0, ctx->stream, T, out, K, sc, total);
     MAFX_HIP(hipGetLastError());
 }
 
-void zfused_sym(mafx_ctx* ctx, float2* spec, const float* ts, int nz) {
+void kspace_mul_sym(mafx_ctx* ctx, float2* spec, const float* ts, int64_t n2, const TensorRes* res) {
+    const int64_t plane = ctx->n1 * ctx->hxs, total = n2 * plane;
+    require(total < (int64_t{1} << 31), "spectrum too large for the symmetric tensor product");
+    const double bytes = static_cast<double>(n2 * ctx->n1 * ctx->hx) * (6.0 * sizeof(float2)) +
+                         6.0 * sizeof(float) * static_cast<double>(ctx->n2h * ctx->n1h * ctx->hx) +
+                         (res ? 6.0 * sizeof(__half2) * static_cast<double>(n2 * ctx->n1 * ctx->hx) : 0.0);
+    ProfScope ps(ctx, res ? "kspace_mul_symres_f32" : "kspace_mul_sym_f32", bytes);
+    const TensorRes rv = res ? *res : TensorRes{};
+    auto fn = res ? k_kspace_mul_sym<true> : k_kspace_mul_sym<false>;
+    hipLaunchKernelGGL(fn, dim3(grid_for(total, 256)), dim3(256), 0, ctx->stream, spec, ts,
+                       static_cast<int>(ctx->n1), static_cast<int>(n2), static_cast<int>(ctx->hx),
+                       make_fastdiv(static_cast<uint32_t>(plane)), make_fastdiv(static_cast<uint32_t>(kTile * ctx->n1)),
+                       static_cast<int>(ctx->hxs), static_cast<int>(ctx->n1h), static_cast<int>(ctx->n2h),
+                       static_cast<int>(total), rv);
+    MAFX_HIP(hipGetLastError());
+}
+
+void zfused_sym(mafx_ctx* ctx, float2* spec, const float* ts, int nz, const TensorRes* res) {
     ZTw<float2> tw{};
     const int n2 = 2 * nz;
     const long double two_pi = 6.283185307179586476925286766559L;
@@ -1239,18 +1325,27 @@ void zfused_sym(mafx_ctx* ctx, float2* spec, const float* ts, int nz) {
     // spectrum columns read and written once, the tensor quarter read once
     const double bytes = static_cast<double>(ctx->n1 * ctx->hx) * (2.0 * 3 * nz) * sizeof(float2) +
                          static_cast<double>(ctx->n1h * ctx->hx) * 6.0 * (nz + 1) * sizeof(float);
-    ProfScope ps(ctx, "zfused_mul_f32", bytes);
+    const double res_bytes = res ? static_cast<double>(ctx->n1 * ctx->hx) * 6.0 * (2 * nz) * sizeof(__half2) : 0.0;
+    ProfScope ps(ctx, res ? "zfused_mul_symres_f32" : "zfused_mul_f32", bytes + res_bytes);
     const dim3 g(grid_for(threads, 256)), b(256);
     const int n1 = static_cast<int>(ctx->n1), hxs = static_cast<int>(ctx->hxs), hx = static_cast<int>(ctx->hx);
     const FastDiv n1hd = make_fastdiv(static_cast<uint32_t>(ctx->n1h));
+    const TensorRes rv = res ? *res : TensorRes{};
+    auto launch = [&](auto nzc) {
+        constexpr int NZ = decltype(nzc)::value;
+        if (res)
+            hipLaunchKernelGGL((k_zfused_sym<NZ, true>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw, rv);
+        else
+            hipLaunchKernelGGL((k_zfused_sym<NZ, false>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw, rv);
+    };
     switch (nz) {
-    case 2: hipLaunchKernelGGL((k_zfused_sym<2>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
-    case 3: hipLaunchKernelGGL((k_zfused_sym<3>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
-    case 4: hipLaunchKernelGGL((k_zfused_sym<4>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
-    case 5: hipLaunchKernelGGL((k_zfused_sym<5>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
-    case 6: hipLaunchKernelGGL((k_zfused_sym<6>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
-    case 7: hipLaunchKernelGGL((k_zfused_sym<7>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
-    case 8: hipLaunchKernelGGL((k_zfused_sym<8>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw); break;
+    case 2: launch(std::integral_constant<int, 2>{}); break;
+    case 3: launch(std::integral_constant<int, 3>{}); break;
+    case 4: launch(std::integral_constant<int, 4>{}); break;
+    case 5: launch(std::integral_constant<int, 5>{}); break;
+    case 6: launch(std::integral_constant<int, 6>{}); break;
+    case 7: launch(std::integral_constant<int, 7>{}); break;
+    case 8: launch(std::integral_constant<int, 8>{}); break;
     default: throw Error(MAFX_ERR_ARG, "fused z stage supports 2 <= nz <= 8");
     }
     MAFX_HIP(hipGetLastError());
@@ -1270,7 +1365,5 @@ template void fft_y_split<double2>(mafx_ctx*, double2*, int64_t, int, int64_t, i
 template void fft_y_split<float2>(mafx_ctx*, float2*, int64_t, int, int64_t, int, int64_t, bool);
 template void kspace_mul<double2>(mafx_ctx*, double2*, const double2*, int64_t);
 template void zfused<double2>(mafx_ctx*, double2*, const double2*, int);
-template void kspace_mul<float2>(mafx_ctx*, float2*, const float2*, int64_t);
-template void zfused<float2>(mafx_ctx*, float2*, const float2*, int);
 
 } // namespace mafx

@@ -1,5 +1,7 @@
 // Host launchers of the FFT axis kernels (fft.hip).
 #pragma once
+#include <hip/hip_fp16.h>
+
 #include "fft_device.hpp"
 #include "mafx_internal.hpp"
 
@@ -44,7 +46,19 @@ template <class C> void zfused(mafx_ctx* ctx, C* spec, const C* tensor, int nz);
 // apply it to a spectrum of n2 z-planes; fused z stage for 2 <= nz <= 8
 void zero_unpaired_planes(mafx_ctx* ctx, double* N, int ncomp);
 void tensor_sym_f32(mafx_ctx* ctx, const double2* T, float* Ts);
-void kspace_mul_sym(mafx_ctx* ctx, float2* spec, const float* ts, int64_t n2);
-void zfused_sym(mafx_ctx* ctx, float2* spec, const float* ts, int nz);
+// The reference's own spectrum on the FP32 path (MAFX_TERM_FULL_TENSOR): the symmetric quarter plus
+// the residual R = T - S over the whole spectrum [6][n2][plane], stored as FP16 pairs scaled per
+// component by a power of two (R / S is ~1e-4, so FP16's 2^-11 leaves ~1e-7 of the entry).
+struct TensorRes {
+    const __half2* r; // [6][n2][plane], spectrum layout
+    float inv[6];     // 1 / scale per component
+};
+struct TensorScale {
+    double s[6];
+};
+// R = T - S in place on T, then out = fp16(R scale); inv_scale receives 1 / scale per component
+void tensor_residual_f16(mafx_ctx* ctx, double2* T, const float* Ts, __half2* out, float inv_scale[6]);
+void kspace_mul_sym(mafx_ctx* ctx, float2* spec, const float* ts, int64_t n2, const TensorRes* res = nullptr);
+void zfused_sym(mafx_ctx* ctx, float2* spec, const float* ts, int nz, const TensorRes* res = nullptr);
 
 } // namespace mafx

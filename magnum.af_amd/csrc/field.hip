@@ -669,19 +669,25 @@ template <class C, class TH, class T> void evaluate_impl(mafx_ctx* ctx, const Ev
         // column tiles: rows ky are kTile elements apart, planes n1 hxs
         fft_y_split<C>(ctx, spec, sa, 3, n1 * hxs, static_cast<int>(g.nz), kTile, false);
         // tensor product: complex reference spectrum (FP64) or its real symmetric quarter (FP32)
-        // (or, MAFX_TERM_FULL_TENSOR, the complex spectrum rounded to FP32)
-        const bool sym = std::is_same<C, float2>::value && !demag_full_c32(ctx->precision, ctx->demag_flags);
+        // (plus, MAFX_TERM_FULL_TENSOR, the FP16 residual that makes it the reference's spectrum)
+        TensorRes res{};
+        const TensorRes* resp = nullptr;
+        if (demag_full_c32(ctx->precision, ctx->demag_flags)) {
+            res.r = reinterpret_cast<const __half2*>(static_cast<const char*>(ctx->tensor.p) + ctx->res_off);
+            for (int c = 0; c < 6; ++c) res.inv[c] = ctx->res_inv[c];
+            resp = &res;
+        }
         auto kmul = [&](int64_t planes) {
-            if (sym)
-                kspace_mul_sym(ctx, reinterpret_cast<float2*>(spec), ctx->tensor.as<float>(), planes);
+            if constexpr (std::is_same<C, float2>::value)
+                kspace_mul_sym(ctx, spec, ctx->tensor.as<float>(), planes, resp);
             else
                 kspace_mul<C>(ctx, spec, ctx->tensor.as<C>(), sa);
         };
         if (n2 == 1) {
             kmul(1);
         } else if (nzs == g.nz) {
-            if (sym)
-                zfused_sym(ctx, reinterpret_cast<float2*>(spec), ctx->tensor.as<float>(), static_cast<int>(g.nz));
+            if constexpr (std::is_same<C, float2>::value)
+                zfused_sym(ctx, spec, ctx->tensor.as<float>(), static_cast<int>(g.nz), resp);
             else
                 zfused<C>(ctx, spec, ctx->tensor.as<C>(), static_cast<int>(g.nz));
         } else {
@@ -947,10 +953,11 @@ __global__ void __launch_bounds__(256)
     for (int c = 0; c < 6; ++c) N[c * P + at] = v[c] * s;
 }
 
-// the FP64 complex spectrum rounded to complex64 (MAFX_TERM_FULL_TENSOR)
-__global__ void __launch_bounds__(256) k_round_c32(const double2* __restrict__ a, float2* __restrict__ b, int64_t n) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (i < n) b[i] = float2{static_cast<float>(a[i].x), static_cast<float>(a[i].y)};
+// byte offset of the FP16 residual in the MAFX_TERM_FULL_TENSOR tensor buffer
+// [symmetric quarter f32][pad to 256 B][residual half2 [6][K]][1 / scale f32 x 6, pad to 8]
+int64_t full32_res_offset(const mafx_ctx* ctx) {
+    const int64_t nS = 6 * ctx->n2h * ctx->n1h * ctx->hxs;
+    return (static_cast<int64_t>(sizeof(float)) * nS + 255) / 256 * 256;
 }
 
 } // namespace
@@ -970,9 +977,12 @@ void build_tensor(mafx_ctx* ctx) {
     const bool caching = (ctx->demag_flags & MAFX_TERM_CACHING) != 0;
     const bool verbose = (ctx->demag_flags & MAFX_TERM_VERBOSE) != 0;
     const bool full32 = demag_full_c32(ctx->precision, ctx->demag_flags);
-    const uint64_t stored = demag_f64(ctx->precision) ? sizeof(double2) * 6 * static_cast<uint64_t>(K)
-                            : full32                  ? sizeof(float2) * 6 * static_cast<uint64_t>(K)
-                                : sizeof(float) * 6 * static_cast<uint64_t>(ctx->n2h * ctx->n1h * hxs);
+    const uint64_t stored =
+        demag_f64(ctx->precision)
+            ? sizeof(double2) * 6 * static_cast<uint64_t>(K)
+        : full32 ? static_cast<uint64_t>(full32_res_offset(ctx)) + sizeof(__half2) * 6 * static_cast<uint64_t>(K) +
+                       8 * sizeof(float)
+                 : sizeof(float) * 6 * static_cast<uint64_t>(ctx->n2h * ctx->n1h * hxs);
     // the FP32 layout from the host-built and the FP64 layout from the GPU-built real-space tensor
     // are stored under keys of their own
     const TensorCacheKey key{n0, n1, n2, hxs, tensor_cache_layout(demag_f64(ctx->precision) ? MAFX_F64 : MAFX_F32_FIELD,
@@ -983,6 +993,10 @@ void build_tensor(mafx_ctx* ctx) {
         if (tensor_cache_read(key, stored, host, verbose)) {
             ctx->tensor.alloc(stored);
             MAFX_HIP(hipMemcpy(ctx->tensor.p, host.data(), stored, hipMemcpyHostToDevice));
+            if (full32) {
+                ctx->res_off = full32_res_offset(ctx);
+                std::memcpy(ctx->res_inv, host.data() + ctx->res_off + sizeof(__half2) * 6 * K, sizeof(ctx->res_inv));
+            }
             ctx->tensor_source = 2;
             tensor_cache_shrink(verbose);
             return;
@@ -1006,6 +1020,29 @@ void build_tensor_uncached(mafx_ctx* ctx) {
     const bool f64 = demag_f64(ctx->precision);
     const bool full32 = demag_full_c32(ctx->precision, ctx->demag_flags);
     const mafx_mesh& g = ctx->mesh;
+    // real-space tensor Nd [6][n2][n1][n0] -> its complex half spectrum T [6][K] (FP64, GPU)
+    auto transform = [&](DevBuf& T) {
+        T.alloc(sizeof(double2) * 6 * K);
+        MAFX_HIP(hipMemsetAsync(T.p, 0, T.bytes, ctx->stream)); // padding columns stay defined
+        double2* t = T.as<double2>();
+        fft_x_forward<double2>(ctx, Nd.as<double>(), nullptr, 1.0, n0, n1, n2, 6, n1, n2, t, n2);
+        // column tiles (fft_device.hpp): lines along y step kTile, lines along z one plane
+        fft_strided<double2>(ctx, ctx->py, t, K, 6, n1 * hxs, static_cast<int>(n2), kTile, static_cast<int>(n1),
+                             static_cast<int>(n1), false, "tensor_fft_y");
+        if (n2 > 1)
+            fft_strided<double2>(ctx, ctx->pz, t, K, 6, kTile, static_cast<int>(n1), n1 * hxs, static_cast<int>(n2),
+                                 static_cast<int>(n2), false, "tensor_fft_z");
+        MAFX_HIP(hipStreamSynchronize(ctx->stream));
+    };
+    auto octant = [&] {
+        // the reference's tensor on the non-negative octant, extended by parity (an eighth of the
+        // host evaluations; the FP32 path's symmetric spectrum needs no more)
+        std::vector<double> oct(static_cast<size_t>(6 * g.nx * g.ny * g.nz));
+        newell_octant_host(g, oct.data(), 0);
+        std::vector<double> full(static_cast<size_t>(6 * P));
+        newell_octant_extend(g, oct.data(), full.data());
+        MAFX_HIP(hipMemcpy(Nd.p, full.data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
+    };
     if (ctx->demag_flags & MAFX_TERM_DEVICE_TENSOR) {
         // the formulas evaluated on the GPU (fast; its own far-field rounding noise)
         hipLaunchKernelGGL(k_real_tensor, dim3(grid_for(P, 256)), dim3(256), 0, ctx->stream, Nd.as<double>(), n0, n1,
@@ -1017,39 +1054,48 @@ void build_tensor_uncached(mafx_ctx* ctx) {
         auto Nh = host_tensor(ctx->mesh, ctx->atom_dipole, P);
         MAFX_HIP(hipMemcpy(Nd.p, Nh->data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
     } else {
-        // the reference's tensor on the non-negative octant, extended by parity (an eighth of the
-        // host evaluations; the FP32 path's symmetric spectrum needs no more)
-        std::vector<double> oct(static_cast<size_t>(6 * g.nx * g.ny * g.nz));
-        newell_octant_host(g, oct.data(), 0);
-        std::vector<double> full(static_cast<size_t>(6 * P));
-        newell_octant_extend(g, oct.data(), full.data());
-        MAFX_HIP(hipMemcpy(Nd.p, full.data(), sizeof(double) * 6 * P, hipMemcpyHostToDevice));
+        octant();
     }
-    if (!f64 && !full32) zero_unpaired_planes(ctx, Nd.as<double>(), 6);
-    T64.alloc(sizeof(double2) * 6 * K);
-    MAFX_HIP(hipMemset(T64.p, 0, T64.bytes)); // padding columns stay defined
-    double2* t = T64.as<double2>();
-    fft_x_forward<double2>(ctx, Nd.as<double>(), nullptr, 1.0, n0, n1, n2, 6, n1, n2, t, n2);
-    // column tiles (fft_device.hpp): lines along y step kTile, lines along z one plane
-    fft_strided<double2>(ctx, ctx->py, t, K, 6, n1 * hxs, static_cast<int>(n2), kTile, static_cast<int>(n1),
-                         static_cast<int>(n1), false, "tensor_fft_y");
-    if (n2 > 1)
-        fft_strided<double2>(ctx, ctx->pz, t, K, 6, kTile, static_cast<int>(n1), n1 * hxs, static_cast<int>(n2),
-                             static_cast<int>(n2), false, "tensor_fft_z");
-    MAFX_HIP(hipStreamSynchronize(ctx->stream));
+    if (full32) {
+        // the reference's spectrum T_ref = S + R: S the symmetric quarter of the parity-symmetric
+        // tensor (the octant extension, or the GPU tensor's symmetric form), R the residual.  The
+        // unpaired planes (distance -n/2 along an axis) never reach the cropped field, so they are
+        // zeroed in T_ref as in S: otherwise R would carry their spectrum (at nz = 4 the plane z =
+        // -4 cells is near field), whose FP16 rounding does reach it.
+        zero_unpaired_planes(ctx, Nd.as<double>(), 6);
+        transform(T64); // T_ref from the whole tensor
+        DevBuf Ts;
+        if (ctx->demag_flags & MAFX_TERM_DEVICE_TENSOR) {
+            MAFX_HIP(hipStreamSynchronize(ctx->stream));
+        } else {
+            octant();
+        }
+        zero_unpaired_planes(ctx, Nd.as<double>(), 6);
+        transform(Ts);
+        Nd.release();
+        ctx->res_off = full32_res_offset(ctx);
+        ctx->tensor.alloc(static_cast<size_t>(ctx->res_off) + sizeof(__half2) * 6 * K + 8 * sizeof(float));
+        float* S = ctx->tensor.as<float>();
+        tensor_sym_f32(ctx, Ts.as<double2>(), S);
+        float inv[6];
+        tensor_residual_f16(ctx, T64.as<double2>(), S,
+                            reinterpret_cast<__half2*>(static_cast<char*>(ctx->tensor.p) + ctx->res_off), inv);
+        float* tail = reinterpret_cast<float*>(static_cast<char*>(ctx->tensor.p) + ctx->res_off +
+                                               sizeof(__half2) * 6 * K);
+        MAFX_HIP(hipMemcpyAsync(tail, inv, sizeof(inv), hipMemcpyHostToDevice, ctx->stream));
+        MAFX_HIP(hipStreamSynchronize(ctx->stream));
+        for (int c = 0; c < 6; ++c) ctx->res_inv[c] = inv[c];
+        return;
+    }
+    if (!f64) zero_unpaired_planes(ctx, Nd.as<double>(), 6);
+    transform(T64);
     Nd.release();
     if (f64) {
         std::swap(ctx->tensor.p, T64.p);
         std::swap(ctx->tensor.bytes, T64.bytes);
-    } else if (full32) {
-        ctx->tensor.alloc(sizeof(float2) * 6 * K);
-        hipLaunchKernelGGL(k_round_c32, dim3(grid_for(6 * K, 256)), dim3(256), 0, ctx->stream, t,
-                           ctx->tensor.as<float2>(), 6 * K);
-        MAFX_HIP(hipGetLastError());
-        MAFX_HIP(hipStreamSynchronize(ctx->stream));
     } else {
         ctx->tensor.alloc(sizeof(float) * 6 * ctx->n2h * ctx->n1h * hxs);
-        tensor_sym_f32(ctx, t, ctx->tensor.as<float>());
+        tensor_sym_f32(ctx, T64.as<double2>(), ctx->tensor.as<float>());
         MAFX_HIP(hipStreamSynchronize(ctx->stream));
     }
 }

@@ -161,6 +161,8 @@ struct mafx_ctx {
     // FP32 path: [6][n2h] planes of n1h hxs real f32, the mirror-symmetric spectrum (fft.hip).
     // Planes store columns in tiles of 4 (tcol, fft_device.hpp).
     mafx::DevBuf tensor;
+    int64_t res_off{0};       // MAFX_TERM_FULL_TENSOR: byte offset of the FP16 residual in `tensor`
+    float res_inv[6]{};       // ... and its per-component 1 / scale
     mafx::DevBuf spec;        // [3][nzs] planes of n1 hxs complex values, column tiles
     mafx::DevBuf hdem;        // [3][N] FFT-convolution field (f64 or f32) between the x inverse and the cell pass
     mafx::DevBuf hbuf;        // 3N f64 scratch (field output for helpers)

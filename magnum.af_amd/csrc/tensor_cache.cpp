@@ -104,8 +104,8 @@ std::string tensor_cache_name(const TensorCacheKey& k) {
     s += layout == 0 ? ((k.precision & kCacheDeviceTensor)   ? "_mafx_c64_dev"
                         : (k.precision & kCacheOctantTensor) ? "_mafx_c64_oct"
                                                              : "_mafx_c64")
-                     : ((k.precision & kCacheFullTensor)     ? ((k.precision & kCacheDeviceTensor) ? "_mafx_c32_dev"
-                                                                                                   : "_mafx_c32_host")
+                     : ((k.precision & kCacheFullTensor)     ? ((k.precision & kCacheDeviceTensor) ? "_mafx_f32res_dev"
+                                                                                                   : "_mafx_f32res_host")
                         : (k.precision & kCacheHostTensor)     ? "_mafx_f32sym_host"
                         : (k.precision & kCacheDeviceTensor) ? "_mafx_f32sym_dev"
                                                              : "_mafx_f32oct");

@@ -16,7 +16,7 @@ namespace mafx {
 constexpr int32_t kCacheHostTensor = 0x100;   // FP32 layout from the whole host-built real-space tensor
 constexpr int32_t kCacheDeviceTensor = 0x200; // either layout from the GPU-built real-space tensor
 constexpr int32_t kCacheOctantTensor = 0x400; // FP64 layout from the parity-extended host octant
-constexpr int32_t kCacheFullTensor = 0x800;   // FP32 complex half spectrum of the whole host tensor
+constexpr int32_t kCacheFullTensor = 0x800;   // FP32 symmetric quarter + FP16 residual of the whole tensor's spectrum
 
 struct TensorCacheKey {
     int64_t n0, n1, n2, hxs; // expanded lengths and the padded row stride of the stored planes

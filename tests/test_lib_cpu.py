@@ -92,7 +92,7 @@ def test_demag_cache_name_is_the_reference_mesh_string(mesh, prec, flags, dipole
     # the layouts built from other real-space tensors (the whole host tensor on the FP32 path, whose
     # default is the host octant; the GPU-built one on either path) have names of their own
     layout = {(0, 0): "_mafx_c64_", (0, 32): "_mafx_c64_dev_", (1, 0): "_mafx_f32oct_", (1, 4): "_mafx_f32sym_host_",
-              (1, 32): "_mafx_f32sym_dev_", (1, 128): "_mafx_c32_host_", (1, 160): "_mafx_c32_dev_"}
+              (1, 32): "_mafx_f32sym_dev_", (1, 128): "_mafx_f32res_host_", (1, 160): "_mafx_f32res_dev_"}
     assert name.startswith(ref + layout[(prec, flags)].rstrip("_"))
     assert (layout[(prec, flags)] in name) or (dipole and layout[(prec, flags)].rstrip("_") + "_dipole_" in name)
     bits = "".join(f"{struct.unpack('<Q', struct.pack('<d', v))[0]:016x}" for v in (dx, dy, dz))
