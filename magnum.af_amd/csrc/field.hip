@@ -67,6 +67,13 @@ struct EpiParams {
     double* partials; // [nslots][nblocks]
     int64_t nblocks;
     const void* hd; // FFT-convolution field [3][N] of type T for the per-cell kernel, or null
+    // LLG mode: next RK stage input (EvalArgs::rk_*), rk_out null when not fused
+    const double* rk_base;
+    const double* rk_k[kMaxRkFused];
+    double rk_c[kMaxRkFused]; // weights of rk_k
+    double rk_clast;          // weight of this pass's own k
+    int rk_n;
+    double* rk_out;
 };
 
 namespace {
@@ -408,14 +415,32 @@ __device__ __forceinline__ void cell_terms(const EpiParams& P, int64_t x, int64_
         const double cp = -kGamma / (1. + a2p);
         const double cd = -al * kGamma / (1. + a2p);
         double* o = static_cast<double*>(P.out);
+        double k[3];
         if (P.diss_only) {
-            stb(o, o8, P.dt * (cd * b0));
-            stb(o + N, o8, P.dt * (cd * b1));
-            stb(o + 2 * N, o8, P.dt * (cd * b2));
+            k[0] = P.dt * (cd * b0);
+            k[1] = P.dt * (cd * b1);
+            k[2] = P.dt * (cd * b2);
         } else {
-            stb(o, o8, P.dt * (cp * a0 + cd * b0));
-            stb(o + N, o8, P.dt * (cp * a1 + cd * b1));
-            stb(o + 2 * N, o8, P.dt * (cp * a2 + cd * b2));
+            k[0] = P.dt * (cp * a0 + cd * b0);
+            k[1] = P.dt * (cp * a1 + cd * b1);
+            k[2] = P.dt * (cp * a2 + cd * b2);
+        }
+        stb(o, o8, k[0]);
+        stb(o + N, o8, k[1]);
+        stb(o + 2 * N, o8, k[2]);
+        if (P.rk_out) {
+            // the next stage's input m + sum_j a_j k_j (adaptive_runge_kutta.cpp:138-164), j in
+            // order with this stage's k last, each term added as the stage combination adds it
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int64_t cN = c * N; // component plane (8 N bytes < 2^32 each)
+                double s = ldb(P.rk_base + cN, o8);
+#pragma unroll
+                for (int j = 0; j < kMaxRkFused; ++j)
+                    if (j < P.rk_n && P.rk_c[j] != 0.0) s = s + P.rk_c[j] * ldb(P.rk_k[j] + cN, o8);
+                if (P.rk_clast != 0.0) s = s + P.rk_clast * k[c];
+                stb(P.rk_out + cN, o8, s);
+            }
         }
         break;
     }
@@ -637,6 +662,14 @@ EpiParams make_params(mafx_ctx* ctx, const EvalArgs& a, int64_t nblocks) {
     P.want_energy = (a.want_energy || a.mode == OutMode::ENERGY) ? 1 : 0;
     P.partials = ctx->partials.as<double>();
     P.nblocks = nblocks;
+    P.rk_out = a.mode == OutMode::LLG ? a.rk_out : nullptr;
+    P.rk_base = a.rk_base;
+    P.rk_n = a.rk_n;
+    for (int j = 0; j < kMaxRkFused; ++j) {
+        P.rk_k[j] = a.rk_k[j];
+        P.rk_c[j] = j < a.rk_n ? a.rk_c[j] : 0.0;
+    }
+    P.rk_clast = a.rk_n >= 0 && a.rk_n <= kMaxRkFused ? a.rk_c[a.rk_n] : 0.0;
     return P;
 }
 

@@ -228,23 +228,45 @@ double attempt(mafx_llg* L, const double* m, double t, double dt, const char* sc
     ea.alpha_field = L->has_alpha_field ? L->alpha_field.as<double>() : nullptr;
     ea.dt = dt;
     ea.dissipation_only = L->dissipation_only;
+    // Schemes of up to kMaxRkFused + 1 stages form each stage input m + sum_j a_sj k_j in the
+    // per-cell pass that produces k_(s-1) (the stage combination of adaptive_runge_kutta.cpp:
+    // 138-164 with k_(s-1) still in registers, so no pass re-reads it); the inputs alternate
+    // between two buffers, since a pass still reads its own input's neighbours.  Results are the
+    // separate pass's to the last bit.  DP78, and meshes below 2^20 cells (where the per-cell
+    // pass has too few lanes to hide the extra loads: 256 x 128 x 4 runs 8% slower fused), form
+    // them in a pass of their own.
+    const bool fused = T.stages <= kMaxRkFused + 1 && ctx->N >= (int64_t{1} << 20);
+    double* stage_in[2] = {L->mtmp.as<double>(), L->mtmp2.as<double>()};
+    auto next_input = [&](int s) { // s: the stage whose input the pass being set up writes
+        ea.rk_out = nullptr;
+        if (!fused || s > T.stages) return;
+        ea.rk_base = m;
+        ea.rk_n = s - 2;
+        for (int j = 1; j < s - 1; ++j) ea.rk_k[j - 1] = L->k[j].as<double>();
+        for (int j = 1; j < s; ++j) ea.rk_c[j - 1] = T.a[s][j];
+        ea.rk_out = stage_in[s & 1];
+    };
     // stage 1: k1 = dt f(m) (recomputed every attempt: normalize_ is true, adaptive_runge_kutta.cpp:132)
     ea.m = m;
     ea.out = L->k[1].p;
     ctx->time = t;
+    next_input(2);
     evaluate(ctx, ea);
     L->heff_evals++;
     for (int s = 2; s <= T.stages; ++s) {
-        const double* v[kMaxStages];
-        double c[kMaxStages];
-        for (int j = 1; j < s; ++j) {
-            v[j - 1] = L->k[j].as<double>();
-            c[j - 1] = T.a[s][j];
+        if (!fused) {
+            const double* v[kMaxStages];
+            double c[kMaxStages];
+            for (int j = 1; j < s; ++j) {
+                v[j - 1] = L->k[j].as<double>();
+                c[j - 1] = T.a[s][j];
+            }
+            lincomb(ctx, stage_in[s & 1], m, s - 1, v, c, len);
         }
-        lincomb(ctx, L->mtmp.as<double>(), m, s - 1, v, c, len);
-        ea.m = L->mtmp.as<double>();
+        ea.m = stage_in[s & 1];
         ea.out = L->k[s].p;
         ctx->time = t + T.c[s] * dt;
+        next_input(s + 1);
         evaluate(ctx, ea);
         L->heff_evals++;
     }

@@ -194,6 +194,8 @@ void newell_octant_extend(const mafx_mesh& mesh, const double* O, double* N);
 void dipole_tensor_host(const mafx_mesh& mesh, double* N);
 
 // ------------------------------------------------------------------ field evaluation (field.hip)
+// stage vectors k_j the per-cell pass can combine into the next stage input (schemes of <= 8 stages)
+constexpr int kMaxRkFused = 7;
 enum class OutMode : int {
     H = 0,      // write H (f64 or f32)
     LLG = 1,    // write k = dt * LLG(alpha, m, H) (f64)
@@ -210,6 +212,14 @@ struct EvalArgs {
     const double* alpha_field{nullptr}; // N f64 cell-wise damping, or null for `alpha`
     bool dissipation_only{false};
     bool want_energy{false};
+    // OutMode::LLG: with rk_out set, the same pass also writes the next RK stage's input
+    // rk_out = rk_base + sum_{j < rk_n} rk_c[j] rk_k[j] + rk_c[rk_n] k, k the output, in the order
+    // and rounding of the separate stage combination
+    const double* rk_base{nullptr};
+    const double* rk_k[kMaxRkFused]{};
+    double rk_c[kMaxRkFused + 1]{};
+    int rk_n{0};
+    double* rk_out{nullptr};
 };
 // Runs the selected terms on m; the energy partial sums (one per term) land in ctx->partials and
 // are reduced by finish_energy.
