@@ -65,16 +65,21 @@ def main():
     ap.add_argument("--dp45-steps", type=int, default=12)
     ap.add_argument("--size", default="1024x1024x4")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--heff-only", choices=["f32", "f64"], default=None,
-                    help="run --steps H_eff evaluations of one path and exit (for profiler runs)")
+    ap.add_argument("--heff-only", choices=["f32", "f64", "f32_full"], default=None,
+                    help="run --steps H_eff evaluations of one path and exit (for profiler runs); f32_full: "
+                         "the FP32 field on the reference tensor's spectrum (DemagField(full_tensor=True))")
+    ap.add_argument("--dp45-only", choices=["f32", "f64", "f32_full"], default=None,
+                    help="run --steps DP45 steps of one path and exit (for profiler runs)")
+    ap.add_argument("--c5-precisions", default="f32,f64", help="C5: paths whose H_eff rate is measured")
     ap.add_argument("--config", choices=["C1", "C2traj", "C3", "C4", "C5", "C5full"], default=None,
                     help="run one secondary configuration and print its JSON line instead of the main one")
     ap.add_argument("--c2-seeds", default="2024", help="C2traj: comma-separated seeds of the initial state")
     ap.add_argument("--c2-paths", default="f32,f32_full,f32_demag,f32_stencils,f64_oct",
                     help="C2traj: paths to compare with the FP64 path")
     ap.add_argument("--c4-samples", type=int, default=4096, help="C4: training films (64x64x1)")
-    ap.add_argument("--c4-epochs", type=int, default=20, help="C4: training epochs")
-    ap.add_argument("--c4-eval", type=int, default=4, help="C4: 256x256x1 evaluation films")
+    ap.add_argument("--c4-epochs", type=int, default=200, help="C4: training epochs (cosine step size 1e-3 -> 2e-5)")
+    ap.add_argument("--c4-eval", type=int, default=16, help="C4: 256x256x1 evaluation films")
+    ap.add_argument("--c4-target-torque", type=float, default=1e-6, help="C4: torque of the training targets")
     ap.add_argument("--c4-weights", default=None, help="C4: write the trained parameters here (.npy); "
                     "C5full: read them from here when the file exists")
     ap.add_argument("--c5-samples", type=int, default=256, help="C5full: training films when no weights are given")
@@ -100,11 +105,22 @@ def main():
 
     t0 = time.time()
     state = M.State(mesh, Ms, m0)
-    if args.heff_only:
-        ctx = state._context(terms, args.heff_only)
-        h = torch.empty(3 * mesh.n, dtype=torch.float64, device="cuda")
+    if args.dp45_only:
+        tl = ([M.DemagField(mesh, full_tensor=True)] + terms[1:]) if args.dp45_only == "f32_full" else terms
+        llg = M.LLGIntegrator(alpha, tl, "DP45", precision=args.dp45_only.replace("_full", ""))
         for _ in range(args.warmup + args.steps):
-            ctx.heff(state.m_dev, out=h)
+            llg.step(state)
+        torch.cuda.synchronize()
+        log(f"[bench] {args.steps} {args.dp45_only} DP45 steps done ({time.time() - t0:.1f} s)")
+        return
+    if args.heff_only:
+        if args.heff_only == "f32_full":
+            terms = [M.DemagField(mesh, full_tensor=True)] + terms[1:]
+        ctx = state._context(terms, args.heff_only.replace("_full", ""))
+        f32out = args.heff_only != "f64"  # each path writes its field in its own precision
+        h = torch.empty(3 * mesh.n, dtype=torch.float32 if f32out else torch.float64, device="cuda")
+        for _ in range(args.warmup + args.steps):
+            ctx.heff(state.m_dev, out=h, f32=f32out)
         torch.cuda.synchronize()
         # a plain device-to-device copy of known size: the reference point for HBM byte counters
         src = torch.ones(COPY_CAL_BYTES // 8, dtype=torch.float64, device="cuda")
@@ -121,11 +137,16 @@ def main():
     ctx64 = state._context(terms, "f64")
     log(f"[bench] FP64 context ready in {time.time() - t0:.1f} s")
     N = mesh.n
+    # Each path writes H_eff in its own precision (the FP32-field path 3N float32, the FP64 path 3N
+    # float64), as the reference does for f32 and f64 m (demag_field.cpp:217-221) and as the
+    # algorithmic byte count of the path has it (3N s for the output).
     k32 = torch.empty(3 * N, dtype=torch.float64, device="cuda")
+    h32 = torch.empty(3 * N, dtype=torch.float32, device="cuda")
 
     def heff_loop(ctx, n):
+        f32out = ctx.precision != M._abi.PRECISIONS["f64"]
         for _ in range(n):
-            ctx.heff(state.m_dev, out=k32)
+            ctx.heff(state.m_dev, out=h32 if f32out else k32, f32=f32out)
 
     # ---------------------------------------------------------------- timed region: FP32 H_eff
     # Each repeat times exactly --steps evaluations between two synchronisations; repeats continue
@@ -146,6 +167,14 @@ def main():
     reps64 = timed_repeats(lambda: heff_loop(ctx64, k64n), torch)
     heff64 = k64n / float(np.median(reps64))
     log(f"[bench] FP64 H_eff: {heff64:.1f} /s")
+    # the FP32-field path with its field written as float64 (the form of the FP64 path's output)
+    def heff_loop_f64out(n):
+        for _ in range(n):
+            ctx32.heff(state.m_dev, out=k32)
+    heff_loop_f64out(3)
+    torch.cuda.synchronize()
+    heff32_f64out = args.steps / float(np.median(timed_repeats(lambda: heff_loop_f64out(args.steps), torch)))
+    log(f"[bench] FP32-field H_eff, float64 output: {heff32_f64out:.1f} /s")
 
     # ---------------------------------------------------------------- per-kernel HIP-event timing
     def kernel_table(ctx, n):
@@ -176,9 +205,9 @@ def main():
     # FP32-field H_eff error against the FP64 path (normwise: max|dH| / max|H64|) on the initial state
     h64 = torch.empty_like(k32)
     ctx64.heff(state.m_dev, out=h64)
-    ctx32.heff(state.m_dev, out=k32)
+    ctx32.heff(state.m_dev, out=h32, f32=True)
     torch.cuda.synchronize()
-    heff_err32 = ((k32 - h64).abs().max() / h64.abs().max()).item()
+    heff_err32 = ((h32.double() - h64).abs().max() / h64.abs().max()).item()
     del h64
     per_heff_ms32 = sum(v["avg_ms"] * v["launches"] for v in kt32.values()) / 20
     bytes_heff32 = sum(v["bytes"] * v["launches"] for v in kt32.values()) / 20
@@ -280,7 +309,7 @@ def main():
                      "algorithmic_bytes": dom[1]["bytes"], "avg_ms": dom[1]["avg_ms"]},
         "cpu_baseline": cpu,
         "extra": {
-            "heff_per_s_fp32": heff32, "heff_per_s_fp64": heff64,
+            "heff_per_s_fp32": heff32, "heff_per_s_fp64": heff64, "heff_per_s_fp32_float64_output": heff32_f64out,
             "dp45_steps_per_s_fp32": dp32, "dp45_steps_per_s_fp64": dp64,
             "dp45_attempts_per_s_fp32": at32, "dp45_attempts_per_s_fp64": at64,
             "dp45_fp32_over_fp64": dp32 / dp64 if dp64 > 0 else None,
@@ -494,7 +523,7 @@ def config_c5(M, torch, args):
              M.UniaxialAnisotropyField(7e5, [0, 0, 1]), M.ExternalField(homogeneous=(50e-3 / MU0, 0, 0))]
     state = M.State(mesh, 1.0e6, m0)
     out = {}
-    for prec in ["f32", "f64"]:
+    for prec in args.c5_precisions.split(","):
         t0 = time.time()
         ctx = state._context(terms, prec)
         log(f"[C5] {prec} context in {time.time() - t0:.1f} s")
@@ -590,24 +619,32 @@ def config_c4(M, torch, args):
     the encoder-decoder on the GPU, then warm start + LBFGS vs plain LBFGS at 256x256x1."""
     import magnumaf.warmstart as WS
     t0 = time.perf_counter()
-    data = WS.generate_dataset(args.c4_samples, seed=1, size=64, torque_tol=1e-4, log=log)
+    data = WS.generate_dataset(args.c4_samples, seed=1, size=64, torque_tol=args.c4_target_torque, log=log)
     t_data = time.perf_counter() - t0
     its = [mm["iterations"] for mm in data["meta"]]
     log(f"[C4] dataset: {args.c4_samples} films in {t_data:.1f} s, LBFGS iterations median {np.median(its):.0f}")
+    hold_idx, _ = WS.split(len(its))
+    base = WS.baseline_losses(data, hold_idx)
+    log(f"[C4] held-out loss of predictors without training: {base}")
     net = WS.WarmStartNet()
     t0 = time.perf_counter()
-    hist, hold = WS.train(net, data, epochs=args.c4_epochs, batch=32, lr=1e-3, log=log)
+    hist, hold = WS.train(net, data, epochs=args.c4_epochs, batch=32, lr=1e-3, lr_min=2e-5, log=log)
     t_train = time.perf_counter() - t0
     if args.c4_weights:
         net.save(args.c4_weights)
-    ev = WS.evaluate(net, n=args.c4_eval, size=256, log=log)
+    ev = WS.evaluate(net, n=args.c4_eval, size=256, log=log, perturb=1e-6)
     sp = [e["speedup"] for e in ev]
     same = [e["E_rel_diff"] <= 1e-9 for e in ev]
+    same_pert = [e["E_rel_diff_perturbed"] <= 1e-9 for e in ev]
     return {"config": "C4 learned warm start: 64x64x1 training films, 256x256x1 evaluation, torque <= 1e-6",
-            "samples": args.c4_samples, "dataset_s": t_data, "train_s": t_train, "epochs": args.c4_epochs,
-            "train_loss": hist, "heldout_loss": hold, "eval": ev,
+            "samples": args.c4_samples, "target_torque": args.c4_target_torque, "dataset_s": t_data,
+            "train_s": t_train, "epochs": args.c4_epochs, "train_loss": hist, "heldout_loss": hold,
+            "heldout_loss_without_training": base, "eval": ev,
             "speedup_median": float(np.median(sp)), "speedup_geomean": float(np.exp(np.mean(np.log(sp)))),
-            "equal_final_energy_count": int(sum(same)), "lbfgs_iterations_median_dataset": float(np.median(its))}
+            "equal_final_energy_count": int(sum(same)),
+            "plain_from_perturbed_start_equal_energy_count": int(sum(same_pert)),
+            "warm_lower_energy_count": int(sum(e["E_warm"] < e["E_plain"] and not s for e, s in zip(ev, same))),
+            "lbfgs_iterations_median_dataset": float(np.median(its))}
 
 
 CONFIGS = {"C1": config_c1, "C2traj": config_c2traj, "C3": config_c3, "C4": config_c4, "C5": config_c5,

@@ -217,22 +217,47 @@ def generate_dataset(n, seed=1, size=64, torque_tol=1e-4, maxiter=20000, threads
     return {"x": X, "y": Y, "meta": meta, "seed": seed, "size": size, "torque_tol": torque_tol}
 
 
-def train(net, data, epochs=20, batch=32, lr=1e-3, seed=0, log=None, holdout=0.1):
-    """Adam on shuffled mini-batches; returns (train losses per epoch, held-out loss per epoch)."""
+def split(n, holdout=0.1, seed=0):
+    """(held-out, training) sample indices of train()'s split."""
+    idx = np.random.default_rng(seed).permutation(n)
+    nh = int(n * holdout)
+    return idx[:nh], idx[nh:]
+
+
+def baseline_losses(data, idx):
+    """The network's loss, sum |m - target|^2 / cells, for three predictors that need no training,
+    over the samples idx: m0 itself; the uniform state +z; and, per sample, the uniform state along
+    that sample's own mean target direction (the best any uniform prediction can do)."""
+    x, y = data["x"][idx].astype(np.float64), data["y"][idx].astype(np.float64)
+    cells = y.shape[0] * y.shape[2] * y.shape[3]
+    m0 = x[:, :3]
+    zhat = np.zeros_like(y)
+    zhat[:, 2] = 1.0
+    mean = y.mean(axis=(2, 3), keepdims=True)
+    nrm = np.linalg.norm(mean, axis=1, keepdims=True)
+    # a zero mean leaves every unit vector equally good (loss 2); +z stands for them
+    best = np.where(nrm > 0, mean / np.where(nrm > 0, nrm, 1.0), zhat[:, :, :1, :1])
+    return {"m0": float(((m0 - y) ** 2).sum() / cells), "uniform_z": float(((zhat - y) ** 2).sum() / cells),
+            "uniform_best": float(((np.broadcast_to(best, y.shape) - y) ** 2).sum() / cells)}
+
+
+def train(net, data, epochs=20, batch=32, lr=1e-3, seed=0, log=None, holdout=0.1, lr_min=None):
+    """Adam on shuffled mini-batches; returns (train losses per epoch, held-out loss per epoch).
+    With lr_min the step size follows a cosine from lr down to lr_min over the epochs."""
     torch = _torch()
     X = torch.from_numpy(data["x"]).cuda()
     Y = torch.from_numpy(data["y"]).cuda()
     n = X.shape[0]
-    nh = int(n * holdout)
-    idx = np.random.default_rng(seed).permutation(n)
-    hold, tr = idx[:nh], idx[nh:]
+    hold, tr = split(n, holdout, seed)
+    nh = len(hold)
     hist, hhist = [], []
     for ep in range(epochs):
         order = np.random.default_rng([seed, ep]).permutation(tr)
+        lr_ep = lr if lr_min is None else lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * ep / max(epochs, 1)))
         tot, cnt = 0.0, 0
         for b in range(0, len(order) - batch + 1, batch):
             sel = torch.from_numpy(order[b:b + batch]).cuda()
-            tot += net.train_step(X[sel].contiguous(), Y[sel].contiguous(), lr)
+            tot += net.train_step(X[sel].contiguous(), Y[sel].contiguous(), lr_ep)
             cnt += 1
         hist.append(tot / max(cnt, 1))
         hl = 0.0
@@ -247,23 +272,42 @@ def train(net, data, epochs=20, batch=32, lr=1e-3, seed=0, log=None, holdout=0.1
     return hist, hhist
 
 
-def evaluate(net, n=4, size=256, seed=7, torque_tol=1e-6, maxiter=200000, log=None):
+def texture(state):
+    """|<m>| of a relaxed film: 1 for a uniform state, well below for domains, labyrinths and
+    skyrmion lattices."""
+    return float(np.linalg.norm(state.mean_m()))
+
+
+def evaluate(net, n=4, size=256, seed=7, torque_tol=1e-6, maxiter=200000, log=None, perturb=None):
     """Plain LBFGS from a random state vs network warm start + LBFGS from the same state, both to
-    max|m x H| / max|H| <= torque_tol.  Returns one record per sample."""
+    max|m x H| / max|H| <= torque_tol.  Returns one record per sample.
+
+    With perturb = eps, plain LBFGS also runs from m0 + eps * (unit Gaussian noise), renormalised:
+    the control of how far the method without the network is from reaching one and the same
+    minimum when its start moves by eps (E_rel_diff_perturbed)."""
     mesh = Mesh(size, size, 1, CELL, CELL, CELL)
     out = []
     for i in range(n):
         rng = np.random.default_rng([seed, i])
         p = random_params(rng)
         m0 = random_m(rng, size, size)
-        _, s_plain, t_plain = relax(mesh, m0, p, torque_tol, maxiter)
-        _, s_warm, t_warm = relax(mesh, m0, p, torque_tol, maxiter, net=net)
+        st_plain, s_plain, t_plain = relax(mesh, m0, p, torque_tol, maxiter)
+        st_warm, s_warm, t_warm = relax(mesh, m0, p, torque_tol, maxiter, net=net)
         rel = abs(s_warm["E"] - s_plain["E"]) / max(abs(s_plain["E"]), 1e-300)
         rec = dict(p, t_plain=t_plain, t_warm=t_warm, speedup=t_plain / t_warm, E_plain=s_plain["E"],
                    E_warm=s_warm["E"], E_rel_diff=rel, it_plain=s_plain["iterations"], it_warm=s_warm["iterations"],
-                   stop_plain=s_plain["stop_reason"], stop_warm=s_warm["stop_reason"])
+                   stop_plain=s_plain["stop_reason"], stop_warm=s_warm["stop_reason"],
+                   mean_m_plain=texture(st_plain), mean_m_warm=texture(st_warm))
+        if perturb:
+            mp = m0 + perturb * np.random.default_rng([seed, i, 1]).standard_normal(m0.shape)
+            mp /= np.linalg.norm(mp, axis=-1, keepdims=True)
+            _, s_pert, t_pert = relax(mesh, mp, p, torque_tol, maxiter)
+            rec.update(E_perturbed=s_pert["E"], it_perturbed=s_pert["iterations"],
+                       E_rel_diff_perturbed=abs(s_pert["E"] - s_plain["E"]) / max(abs(s_plain["E"]), 1e-300))
         out.append(rec)
         if log:
-            log(f"[eval] {i}: plain {t_plain:.2f} s ({s_plain['iterations']} it), warm {t_warm:.2f} s "
-                f"({s_warm['iterations']} it), speedup {rec['speedup']:.2f}, dE/E {rel:.2e}")
+            extra = f", perturbed start dE/E {rec['E_rel_diff_perturbed']:.2e}" if perturb else ""
+            log(f"[eval] {i}: plain {t_plain:.2f} s ({s_plain['iterations']} it, |<m>| {rec['mean_m_plain']:.3f}), "
+                f"warm {t_warm:.2f} s ({s_warm['iterations']} it, |<m>| {rec['mean_m_warm']:.3f}), "
+                f"speedup {rec['speedup']:.2f}, dE/E {rel:.2e}{extra}")
     return out

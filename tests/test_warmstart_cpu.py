@@ -87,3 +87,26 @@ def test_adam_matches_torch():
         opt.step()
         w, mom, vel = NR.adam_step(w, g, mom, vel, t, 1e-2)
     assert np.allclose(w, p.detach().numpy(), rtol=1e-12, atol=1e-14)
+
+
+def test_baseline_losses_closed_forms():
+    """The untrained predictors' losses (sum |m - target|^2 / cells) on constructed data: targets all
+    +z give 0 for the uniform +z and the best-uniform predictors; targets +z on half of each film and
+    -z on the other half give 2 for both (the mean direction is 0, and every unit vector u then has
+    loss 2 - 2 u . <target> = 2); m0 = target gives 0 for the m0 predictor."""
+    import magnumaf.warmstart as WS
+    n, s = 3, 8
+    y = np.zeros((n, 3, s, s), np.float32)
+    y[:, 2] = 1.0
+    x = np.zeros((n, 8, s, s), np.float32)
+    x[:, :3] = y
+    b = WS.baseline_losses({"x": x, "y": y}, np.arange(n))
+    assert b == {"m0": 0.0, "uniform_z": 0.0, "uniform_best": 0.0}
+    y2 = y.copy()
+    y2[:, 2, :, : s // 2] = -1.0
+    x2 = x.copy()
+    x2[:, 2] = 0.0
+    x2[:, 0] = 1.0  # m0 along +x: |x - (+-z)|^2 = 2 everywhere
+    b2 = WS.baseline_losses({"x": x2, "y": y2}, np.arange(n))
+    assert abs(b2["m0"] - 2.0) < 1e-12 and abs(b2["uniform_z"] - 2.0) < 1e-12
+    assert abs(b2["uniform_best"] - 2.0) < 1e-6
