@@ -346,21 +346,18 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
         if (k == 0) {
             lbfgs_fused(ctx, gold, nullptr, nullptr, H0k, true, gold, dir.as<double>(), true, n3, d_dot);
         } else {
-            dot_dev(ctx, sV[k - 1], gold, n3, d_dot);
-            lbfgs_alpha(ctx, ds + (k - 1), d_coef, d_dot, rho[k - 1]);
-            for (int i = k - 1; i >= 1; i--) {
+            // each reduction's last stage also forms the next step's alpha_i / coefficient
+            lbfgs_dot_alpha(ctx, sV[k - 1], gold, n3, d_dot, rho[k - 1], ds + (k - 1), d_coef);
+            for (int i = k - 1; i >= 1; i--)
                 lbfgs_fused(ctx, i == k - 1 ? gold : q.as<double>(), d_coef, yV[i], 1.0, false, sV[i - 1],
-                            q.as<double>(), false, n3, d_dot);
-                lbfgs_alpha(ctx, ds + (i - 1), d_coef, d_dot, rho[i - 1]);
-            }
+                            q.as<double>(), false, n3, d_dot, 1, rho[i - 1], ds + (i - 1), d_coef);
             lbfgs_fused(ctx, k == 1 ? gold : q.as<double>(), d_coef, yV[0], H0k, true, yV[0], q.as<double>(), false,
-                        n3, d_dot);
-            lbfgs_beta(ctx, d_coef, ds + 0, d_dot, rho[0]);
+                        n3, d_dot, 2, rho[0], ds + 0, d_coef);
             for (int i = 0; i < k; i++) {
                 const bool last = i == k - 1;
                 lbfgs_fused(ctx, q.as<double>(), d_coef, sV[i], 1.0, false, last ? gold : yV[i + 1],
-                            last ? dir.as<double>() : q.as<double>(), last, n3, d_dot);
-                if (!last) lbfgs_beta(ctx, d_coef, ds + (i + 1), d_dot, rho[i + 1]);
+                            last ? dir.as<double>() : q.as<double>(), last, n3, d_dot, last ? 0 : 2,
+                            last ? 0.0 : rho[i + 1], last ? nullptr : ds + (i + 1), last ? nullptr : d_coef);
             }
         }
         double phi;

@@ -437,20 +437,71 @@ __global__ void __launch_bounds__(kB) k_lbfgs_pair(const double* __restrict__ m,
     }
 }
 
+// k_final's one-slot sum, then (MODE 1) alpha_i = rho d, coef = -alpha_i or (MODE 2) coef =
+// alpha_i - rho d with beta = rho d rounded first, as k_lbfgs_alpha / k_lbfgs_beta do: the two-loop
+// step's scalars without a launch of their own
+template <int MODE>
+__global__ void __launch_bounds__(kB) k_final_coef(const double* __restrict__ part, int nb, double* __restrict__ dot_out,
+                                                  double rho, double* alpha_i, double* coef) {
+    double v = 0;
+    for (int i = threadIdx.x; i < nb; i += kB) v = v + part[i];
+    v = block_sum(v);
+    if (threadIdx.x == 0) {
+        *dot_out = v;
+        if constexpr (MODE == 1) {
+            const double a = rho * v;
+            *alpha_i = a;
+            *coef = -a;
+        }
+        if constexpr (MODE == 2) {
+#pragma clang fp contract(off)
+            const double beta = rho * v;
+            *coef = *alpha_i - beta;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kB) k_neg(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
     if (i < n) y[i] = -x[i];
 }
 } // namespace
 
+namespace {
+void final_coef(mafx_ctx* ctx, unsigned nb, double* dot_out, int mode, double rho, double* alpha_i, double* coef) {
+    const double* part = ctx->partials.as<double>();
+    const int n = static_cast<int>(nb);
+    switch (mode) {
+    case 1:
+        hipLaunchKernelGGL(k_final_coef<1>, dim3(1), dim3(kB), 0, ctx->stream, part, n, dot_out, rho, alpha_i, coef);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_final_coef<2>, dim3(1), dim3(kB), 0, ctx->stream, part, n, dot_out, rho, alpha_i, coef);
+        break;
+    default:
+        hipLaunchKernelGGL(k_final_coef<0>, dim3(1), dim3(kB), 0, ctx->stream, part, n, dot_out, rho, alpha_i, coef);
+        break;
+    }
+}
+} // namespace
+
 void lbfgs_fused(mafx_ctx* ctx, const double* q_in, const double* coef_dev, const double* v, double scale,
-                 bool do_scale, const double* dnext, double* q_out, bool negate, int64_t n, double* out_dev) {
+                 bool do_scale, const double* dnext, double* q_out, bool negate, int64_t n, double* out_dev, int mode,
+                 double rho, double* alpha_i, double* coef_out) {
     const unsigned nb = red_blocks(n);
     ProfScope ps(ctx, "lbfgs_twoloop_f64", 8.0 * n * (coef_dev ? 4 : 3));
     hipLaunchKernelGGL(k_lbfgs_fused, dim3(nb), dim3(kB), 0, ctx->stream, q_in, coef_dev, v, scale, do_scale ? 1 : 0,
                        dnext, q_out, negate ? 1 : 0, n, ctx->partials.as<double>());
-    hipLaunchKernelGGL(k_final, dim3(1), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(), static_cast<int>(nb),
-                       1, 1, out_dev);
+    final_coef(ctx, nb, out_dev, mode, rho, alpha_i, coef_out);
+    MAFX_HIP(hipGetLastError());
+}
+
+void lbfgs_dot_alpha(mafx_ctx* ctx, const double* a, const double* b, int64_t n, double* dot_out, double rho,
+                     double* alpha_i, double* coef) {
+    const unsigned nb = red_blocks(n);
+    ProfScope ps(ctx, "dot_f64", 16.0 * n);
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kB), 0, ctx->stream, a, b, n, ctx->partials.as<double>());
+    final_coef(ctx, nb, dot_out, 1, rho, alpha_i, coef);
     MAFX_HIP(hipGetLastError());
 }
 

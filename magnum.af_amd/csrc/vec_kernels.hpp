@@ -32,8 +32,14 @@ void read_device(mafx_ctx* ctx, const double* src_dev, int n, double* dst_host);
 //   (before negation), with dot()'s tree; coef_dev null skips the axpy.  q_out may alias q_in.
 // lbfgs_pair: s = m - x_old, y = g - g_old, out_dev[0..4] = y.s, y.y, s.s, max|s|, max|m|.
 // neg_copy: y = -x.
+// mode 1 / 2 also forms the next step's scalars from the dot product d = *out_dev, as lbfgs_alpha
+// (alpha_i = rho d, coef = -alpha_i) / lbfgs_beta (coef = alpha_i - rho d) would, in the same launch
+// as the reduction's last stage.  lbfgs_dot_alpha: *dot_out = a . b with dot()'s tree, then mode 1.
 void lbfgs_fused(mafx_ctx* ctx, const double* q_in, const double* coef_dev, const double* v, double scale,
-                 bool do_scale, const double* dnext, double* q_out, bool negate, int64_t n, double* out_dev);
+                 bool do_scale, const double* dnext, double* q_out, bool negate, int64_t n, double* out_dev,
+                 int mode = 0, double rho = 0.0, double* alpha_i = nullptr, double* coef_out = nullptr);
+void lbfgs_dot_alpha(mafx_ctx* ctx, const double* a, const double* b, int64_t n, double* dot_out, double rho,
+                     double* alpha_i, double* coef);
 void lbfgs_pair(mafx_ctx* ctx, const double* m, const double* x_old, const double* g, const double* g_old,
                 double* s, double* y, int64_t n, double* out_dev);
 void neg_copy(mafx_ctx* ctx, double* y, const double* x, int64_t n);
