@@ -77,7 +77,7 @@ def main():
     ap.add_argument("--c2-paths", default="f32,f32_full,f32_demag,f32_stencils,f64_oct",
                     help="C2traj: paths to compare with the FP64 path")
     ap.add_argument("--c4-samples", type=int, default=4096, help="C4: training films (64x64x1)")
-    ap.add_argument("--c4-epochs", type=int, default=200, help="C4: training epochs (cosine step size 1e-3 -> 2e-5)")
+    ap.add_argument("--c4-epochs", type=int, default=100, help="C4: training epochs (cosine step size 1e-3 -> 2e-5; the best held-out epoch is kept)")
     ap.add_argument("--c4-eval", type=int, default=16, help="C4: 256x256x1 evaluation films")
     ap.add_argument("--c4-target-torque", type=float, default=1e-6, help="C4: torque of the training targets")
     ap.add_argument("--c4-weights", default=None, help="C4: write the trained parameters here (.npy); "

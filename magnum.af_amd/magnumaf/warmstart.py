@@ -33,6 +33,13 @@ RANGES = (("Ms", 5e5, 1.2e6), ("A", 1e-11, 2e-11), ("D", 1e-3, 4e-3), ("Ku", 2e5
 CELL = 3e-9
 
 
+def bf16_bits(x):
+    """float32 -> BF16 bit patterns (uint16), round to nearest, ties to even."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return r.astype(np.uint16)
+
+
 def _torch():
     import torch
     if not torch.cuda.is_available():
@@ -93,10 +100,21 @@ class WarmStartNet:
     def save(self, path):
         np.save(path, self.get_params())
 
+    def save_bf16(self, path):
+        """The parameters as BF16 bit patterns (uint16, round to nearest even): half the file of
+        save(), and the same network, since the kernels compute with the BF16 rounding of the
+        weights.  (Biases are rounded too; the kernels keep them in FP32, so a network loaded from
+        this file is the BF16-rounded one.)"""
+        np.save(path, bf16_bits(self.get_params()))
+
     @classmethod
     def load(cls, path):
+        """From save() (float32) or save_bf16() (uint16 BF16 bits)."""
         net = cls()
-        net.set_params(np.load(path, allow_pickle=False))
+        p = np.load(path, allow_pickle=False)
+        if p.dtype == np.uint16:
+            p = (p.astype(np.uint32) << 16).view(np.float32)
+        net.set_params(p)
         return net
 
     def forward(self, x):
@@ -241,9 +259,12 @@ def baseline_losses(data, idx):
             "uniform_best": float(((np.broadcast_to(best, y.shape) - y) ** 2).sum() / cells)}
 
 
-def train(net, data, epochs=20, batch=32, lr=1e-3, seed=0, log=None, holdout=0.1, lr_min=None):
+def train(net, data, epochs=20, batch=32, lr=1e-3, seed=0, log=None, holdout=0.1, lr_min=None, keep_best=True):
     """Adam on shuffled mini-batches; returns (train losses per epoch, held-out loss per epoch).
-    With lr_min the step size follows a cosine from lr down to lr_min over the epochs."""
+    With lr_min the step size follows a cosine from lr down to lr_min over the epochs.  With
+    keep_best (and a held-out set) the network ends with the parameters of its best held-out epoch:
+    on the C4 films the held-out loss is lowest after ~30 epochs and rises after, while the
+    training loss keeps falling (the network memorises the training films)."""
     torch = _torch()
     X = torch.from_numpy(data["x"]).cuda()
     Y = torch.from_numpy(data["y"]).cuda()
@@ -251,6 +272,7 @@ def train(net, data, epochs=20, batch=32, lr=1e-3, seed=0, log=None, holdout=0.1
     hold, tr = split(n, holdout, seed)
     nh = len(hold)
     hist, hhist = [], []
+    best = (math.inf, None)
     for ep in range(epochs):
         order = np.random.default_rng([seed, ep]).permutation(tr)
         lr_ep = lr if lr_min is None else lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * ep / max(epochs, 1)))
@@ -267,8 +289,14 @@ def train(net, data, epochs=20, batch=32, lr=1e-3, seed=0, log=None, holdout=0.1
                 hl += net.gradients(X[sel].contiguous(), Y[sel].contiguous())[0] * len(sel)
             hl /= nh
         hhist.append(hl)
+        if keep_best and nh and hl < best[0]:
+            best = (hl, net.get_params())
         if log:
             log(f"[train] epoch {ep}: train loss {hist[-1]:.4f}, held-out {hl:.4f}")
+    if best[1] is not None:
+        net.set_params(best[1])
+        if log:
+            log(f"[train] kept the parameters of epoch {int(np.argmin(hhist))} (held-out {best[0]:.4f})")
     return hist, hhist
 
 

@@ -308,7 +308,8 @@ FP32_TOL_LONG_ROWS = 3e-5
                                   (6, 512, 2, 2e-9, 1e-9, 2e-9), (12, 256, 1, 2e-9, 2e-9, 3e-9),
                                   (256, 16, 2, 1e-9, 2e-9, 2e-9), (512, 8, 1, 2e-9, 2e-9, 3e-9),
                                   (1024, 4, 2, 2e-9, 1e-9, 2e-9), (2048, 2, 1, 2e-9, 2e-9, 2e-9),
-                                  (1, 1, 1, 2e-9, 2e-9, 2e-9)])
+                                  (1, 1, 1, 2e-9, 2e-9, 2e-9), (16, 16, 7, 2e-9, 2e-9, 2e-9),
+                                  (64, 32, 8, 2e-9, 2e-9, 2e-9)])
 def test_heff_fp32_field_error(mesh):
     print(f"\n[heff fp32] mesh={mesh}", flush=True)
     m = rand_m(mesh[:3], 5)
@@ -325,6 +326,25 @@ def test_heff_fp32_field_error(mesh):
     hd = M.LLGIntegrator(0.1, [terms[0]], precision="f32").H_in_Apm(state)
     err_d = normwise(hd, sim.heff(mask=1))
     assert err_d <= (FP32_TOL if mesh[0] < 2048 else FP32_TOL_LONG_ROWS), err_d
+
+
+@pytest.mark.parametrize("mesh", [(20, 9, 8, 1e-9, 1e-9, 1e-9), (64, 32, 8, 2e-9, 2e-9, 2e-9),
+                                  (16, 16, 7, 2e-9, 2e-9, 2e-9), (16, 8, 5, 2e-9, 1e-9, 1e-9),
+                                  (24, 16, 2, 1e-9, 2e-9, 1e-9), (64, 32, 1, 3e-9, 3e-9, 3e-9),
+                                  (12, 6, 10, 2e-9, 2e-9, 2e-9)])
+def test_fp32_full_tensor(mesh):
+    """DemagField(full_tensor=True): the FP32 field on the reference tensor's spectrum (symmetric
+    quarter + FP16 residual) against the oracle, on the fused z stages (nz = 2, the rolled nz = 5
+    and 7, the paired-lane nz = 8), the nz = 1 product and the strided z passes (nz = 10)."""
+    print(f"\n[fp32 full tensor] mesh={mesh}", flush=True)
+    m = rand_m(mesh[:3], 8)
+    state, terms, sim = build(mesh, 1.1e6, m, MICRO_ALL)
+    M = mf()
+    g = state.mesh
+    hd = M.LLGIntegrator(0.1, [M.DemagField(g, full_tensor=True)], precision="f32").H_in_Apm(state)
+    err = normwise(hd, sim.heff(mask=1))
+    print(f"[fp32 full tensor] mesh={mesh} demag normwise error={err:.3e}", flush=True)
+    assert err <= FP32_TOL
 
 
 def test_fp32_host_tensor():

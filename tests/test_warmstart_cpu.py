@@ -110,3 +110,14 @@ def test_baseline_losses_closed_forms():
     b2 = WS.baseline_losses({"x": x2, "y": y2}, np.arange(n))
     assert abs(b2["m0"] - 2.0) < 1e-12 and abs(b2["uniform_z"] - 2.0) < 1e-12
     assert abs(b2["uniform_best"] - 2.0) < 1e-6
+
+
+def test_bf16_bits_match_torch():
+    """The BF16 weight file's rounding (round to nearest, ties to even) equals torch's float32 ->
+    bfloat16 conversion, including ties and values near the float32 range's ends."""
+    import magnumaf.warmstart as WS
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.standard_normal(20000).astype(np.float32) * np.float32(7.0),
+                        np.array([1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -0.0, 3.0e38, 1e-38], np.float32)])
+    ref = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert np.array_equal(WS.bf16_bits(x), ref)
