@@ -76,3 +76,43 @@ def test_warm_start_entry_gives_unit_vectors():
     m = state.m
     assert np.all(np.isfinite(m))
     assert np.allclose(np.linalg.norm(m, axis=-1), 1.0, atol=1e-12)
+
+
+WEIGHTS = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "warmstart_weights_bf16.npy")
+
+
+def test_trained_warm_start_gain_at_equal_energy():
+    """The committed network (tests/golden/warmstart_weights_bf16.npy, from `bench.py --config C4
+    --c4-weights ...`: trained on 4096 relaxed 64x64 films, the best held-out epoch, stored as BF16)
+    against plain LBFGS on 64x64 films from seeds the training never saw, both to
+    max|m x H| / max|H| <= 1e-6.  Equal final energies decide something only where the film's
+    minimum is defined by its start: plain LBFGS from m0 and from m0 + 1e-6 noise end within 1e-9 of
+    each other.  Where that holds and the warm start reaches the same energy (within 1e-9), it must
+    take at most half the iterations of plain LBFGS (median), on at least three films."""
+    import magnumaf as M
+    import magnumaf.warmstart as WS
+    net = WS.WarmStartNet.load(WEIGHTS)
+    mesh = M.Mesh(64, 64, 1, WS.CELL, WS.CELL, WS.CELL)
+    recs = []
+    for i in range(10):
+        rng = np.random.default_rng([4242, i])
+        p = WS.random_params(rng)
+        m0 = WS.random_m(rng, 64, 64)
+        mp = m0 + 1e-6 * np.random.default_rng([4242, i, 1]).standard_normal(m0.shape)
+        mp /= np.linalg.norm(mp, axis=-1, keepdims=True)
+        _, sp, _ = WS.relax(mesh, m0, p, 1e-6, 20000)
+        _, sq, _ = WS.relax(mesh, mp, p, 1e-6, 20000)
+        _, sw, _ = WS.relax(mesh, m0, p, 1e-6, 20000, net=net)
+        rel = lambda a, b: abs(a - b) / abs(b)
+        r = dict(film=i, it_plain=sp["iterations"], it_warm=sw["iterations"], defined=rel(sq["E"], sp["E"]) <= 1e-9,
+                 equal=rel(sw["E"], sp["E"]) <= 1e-9, dE_warm=rel(sw["E"], sp["E"]), warm_lower=sw["E"] < sp["E"],
+                 stops=(sp["stop_reason"], sw["stop_reason"]))
+        recs.append(r)
+        print(f"\n[warm start 64x64] {r}", flush=True)
+    both = [r for r in recs if r["defined"] and r["equal"]]
+    ratio = [r["it_plain"] / r["it_warm"] for r in both]
+    print(f"[warm start 64x64] {len(both)} films with a defined minimum reached by both; iteration ratios "
+          f"{[round(x, 2) for x in ratio]}", flush=True)
+    assert all(r["stops"] == (3, 3) for r in recs)
+    assert len(both) >= 3
+    assert np.median(ratio) >= 2.0
