@@ -321,6 +321,10 @@ def main():
             "heff_kernel_ms_fp32": per_heff_ms32, "heff_kernel_ms_fp64": per_heff_ms64,
             "heff_frac_of_hbm_peak_fp32": bytes_heff32 / (per_heff_ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "heff_frac_of_hbm_peak_fp64": bytes_heff64 / (per_heff_ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            # the same bytes over the timed loop's wall clock per evaluation (kernels overlap at
+            # their boundaries, so this exceeds the sum of event-timed kernel durations' figure)
+            "heff_wall_frac_of_hbm_peak_fp32": bytes_heff32 * heff32 / 1e9 / HBM_PEAK_GBS,
+            "heff_wall_frac_of_hbm_peak_fp64": bytes_heff64 * heff64 / 1e9 / HBM_PEAK_GBS,
             "copy_kernel_GBps": copy_gbs,
             "kernels_fp32": kt32, "kernels_fp64": kt64,
         },
