@@ -259,6 +259,11 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
         MAFX_HIP(hipGetDevice(&dev));
         MAFX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         if (cus > 0) ctx->num_cus = cus;
+        // MAFX_RK_FUSE_MIN_CELLS: mesh size from which RK stage inputs are formed in the per-cell
+        // pass (tests compare both forms on small meshes); MAFX_RK_K64=1 keeps the FP32-field
+        // integrator's stage derivatives in FP64
+        if (const char* e = std::getenv("MAFX_RK_FUSE_MIN_CELLS")) ctx->rk_fuse_min_cells = std::atoll(e);
+        if (const char* e = std::getenv("MAFX_RK_K64")) ctx->rk_k32 = std::atoi(e) == 0;
         // MAFX_NUM_CUS overrides the CU count that sizes the persistent grids (tuning, and tests
         // that make each resident group take many tiles on small meshes)
         if (const char* e = std::getenv("MAFX_NUM_CUS")) {

@@ -69,7 +69,8 @@ struct EpiParams {
     const void* hd; // FFT-convolution field [3][N] of type T for the per-cell kernel, or null
     // LLG mode: next RK stage input (EvalArgs::rk_*), rk_out null when not fused
     const double* rk_base;
-    const double* rk_k[kMaxRkFused];
+    const void* rk_k[kMaxRkFused]; // double, or float when k_f32
+    int k_f32;                     // k written (and rk_k read) as float32
     double rk_c[kMaxRkFused]; // weights of rk_k
     double rk_clast;          // weight of this pass's own k
     int rk_n;
@@ -414,7 +415,6 @@ __device__ __forceinline__ void cell_terms(const EpiParams& P, int64_t x, int64_
         const double a2p = al * al;
         const double cp = -kGamma / (1. + a2p);
         const double cd = -al * kGamma / (1. + a2p);
-        double* o = static_cast<double*>(P.out);
         double k[3];
         if (P.diss_only) {
             k[0] = P.dt * (cd * b0);
@@ -425,9 +425,22 @@ __device__ __forceinline__ void cell_terms(const EpiParams& P, int64_t x, int64_
             k[1] = P.dt * (cp * a1 + cd * b1);
             k[2] = P.dt * (cp * a2 + cd * b2);
         }
-        stb(o, o8, k[0]);
-        stb(o + N, o8, k[1]);
-        stb(o + 2 * N, o8, k[2]);
+        if (P.k_f32) {
+            // stored as float32; the next stage input below uses the stored value
+            float* o = static_cast<float*>(P.out);
+            const uint32_t o4 = static_cast<uint32_t>(i) * 4u;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float kf = static_cast<float>(k[c]);
+                k[c] = static_cast<double>(kf);
+                stb(o + c * N, o4, kf);
+            }
+        } else {
+            double* o = static_cast<double*>(P.out);
+            stb(o, o8, k[0]);
+            stb(o + N, o8, k[1]);
+            stb(o + 2 * N, o8, k[2]);
+        }
         if (P.rk_out) {
             // the next stage's input m + sum_j a_j k_j (adaptive_runge_kutta.cpp:138-164), j in
             // order with this stage's k last, each term added as the stage combination adds it
@@ -436,8 +449,13 @@ __device__ __forceinline__ void cell_terms(const EpiParams& P, int64_t x, int64_
                 const int64_t cN = c * N; // component plane (8 N bytes < 2^32 each)
                 double s = ldb(P.rk_base + cN, o8);
 #pragma unroll
-                for (int j = 0; j < kMaxRkFused; ++j)
-                    if (j < P.rk_n && P.rk_c[j] != 0.0) s = s + P.rk_c[j] * ldb(P.rk_k[j] + cN, o8);
+                for (int j = 0; j < kMaxRkFused; ++j) {
+                    if (j < P.rk_n && P.rk_c[j] != 0.0) {
+                        const double kj = P.k_f32 ? static_cast<double>(ldb(static_cast<const float*>(P.rk_k[j]) + cN, o8 / 2u))
+                                                  : ldb(static_cast<const double*>(P.rk_k[j]) + cN, o8);
+                        s = s + P.rk_c[j] * kj;
+                    }
+                }
                 if (P.rk_clast != 0.0) s = s + P.rk_clast * k[c];
                 stb(P.rk_out + cN, o8, s);
             }
@@ -748,6 +766,7 @@ EpiParams make_params(mafx_ctx* ctx, const EvalArgs& a, int64_t nblocks) {
     P.nblocks = nblocks;
     P.rk_out = a.mode == OutMode::LLG ? a.rk_out : nullptr;
     P.rk_base = a.rk_base;
+    P.k_f32 = a.k_f32 ? 1 : 0;
     P.rk_n = a.rk_n;
     for (int j = 0; j < kMaxRkFused; ++j) {
         P.rk_k[j] = a.rk_k[j];

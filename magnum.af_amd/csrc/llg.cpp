@@ -228,6 +228,11 @@ double attempt(mafx_llg* L, const double* m, double t, double dt, const char* sc
     ea.alpha_field = L->has_alpha_field ? L->alpha_field.as<double>() : nullptr;
     ea.dt = dt;
     ea.dissipation_only = L->dissipation_only;
+    // The FP32-field path stores its stage derivatives k_j = dt f(m_j) in FP32: they carry the
+    // FP32 field's ~1e-6 relative error already, and every sum over them (stage inputs, sumbk, the
+    // error norm) stays FP64 on the FP64 state.  Halves the k_j traffic of every stage.
+    const bool k32 = ctx->precision == MAFX_F32_FIELD && ctx->rk_k32;
+    ea.k_f32 = k32;
     // Schemes of up to kMaxRkFused + 1 stages form each stage input m + sum_j a_sj k_j in the
     // per-cell pass that produces k_(s-1) (the stage combination of adaptive_runge_kutta.cpp:
     // 138-164 with k_(s-1) still in registers, so no pass re-reads it); the inputs alternate
@@ -235,14 +240,14 @@ double attempt(mafx_llg* L, const double* m, double t, double dt, const char* sc
     // separate pass's to the last bit.  DP78, and meshes below 2^20 cells (where the per-cell
     // pass has too few lanes to hide the extra loads: 256 x 128 x 4 runs 8% slower fused), form
     // them in a pass of their own.
-    const bool fused = T.stages <= kMaxRkFused + 1 && ctx->N >= (int64_t{1} << 20);
+    const bool fused = T.stages <= kMaxRkFused + 1 && ctx->N >= ctx->rk_fuse_min_cells;
     double* stage_in[2] = {L->mtmp.as<double>(), L->mtmp2.as<double>()};
     auto next_input = [&](int s) { // s: the stage whose input the pass being set up writes
         ea.rk_out = nullptr;
         if (!fused || s > T.stages) return;
         ea.rk_base = m;
         ea.rk_n = s - 2;
-        for (int j = 1; j < s - 1; ++j) ea.rk_k[j - 1] = L->k[j].as<double>();
+        for (int j = 1; j < s - 1; ++j) ea.rk_k[j - 1] = L->k[j].p;
         for (int j = 1; j < s; ++j) ea.rk_c[j - 1] = T.a[s][j];
         ea.rk_out = stage_in[s & 1];
     };
@@ -255,13 +260,13 @@ double attempt(mafx_llg* L, const double* m, double t, double dt, const char* sc
     L->heff_evals++;
     for (int s = 2; s <= T.stages; ++s) {
         if (!fused) {
-            const double* v[kMaxStages];
+            const void* v[kMaxStages];
             double c[kMaxStages];
             for (int j = 1; j < s; ++j) {
-                v[j - 1] = L->k[j].as<double>();
+                v[j - 1] = L->k[j].p;
                 c[j - 1] = T.a[s][j];
             }
-            lincomb(ctx, stage_in[s & 1], m, s - 1, v, c, len);
+            lincomb(ctx, stage_in[s & 1], m, s - 1, v, c, len, k32);
         }
         ea.m = stage_in[s & 1];
         ea.out = L->k[s].p;
@@ -270,9 +275,9 @@ double attempt(mafx_llg* L, const double* m, double t, double dt, const char* sc
         evaluate(ctx, ea);
         L->heff_evals++;
     }
-    const double* kp[kMaxStages + 1] = {nullptr};
-    for (int j = 1; j <= T.stages; ++j) kp[j] = L->k[j].as<double>();
-    return rk_error(ctx, m, kp, T.b, T.e, T.stages, T.rkf_form, L->ctrl.atol, L->ctrl.rtol, L->sumbk.as<double>());
+    const void* kp[kMaxStages + 1] = {nullptr};
+    for (int j = 1; j <= T.stages; ++j) kp[j] = L->k[j].p;
+    return rk_error(ctx, m, kp, T.b, T.e, T.stages, T.rkf_form, L->ctrl.atol, L->ctrl.rtol, L->sumbk.as<double>(), k32);
 }
 
 // adaptive_runge_kutta.cpp:39-71

@@ -145,6 +145,11 @@ struct mafx_ctx {
     mafx::DevBuf ms_field; // N f64
     int precision{MAFX_F64};
     int num_cus{256};         // compute units of the current device (persistent grids)
+    // RK stage inputs are formed in the per-cell pass from this many cells on (below it the
+    // pass has too few lanes to hide the extra loads); MAFX_RK_FUSE_MIN_CELLS overrides it
+    int64_t rk_fuse_min_cells{int64_t{1} << 20};
+    // MAFX_F32_FIELD integrators keep their stage derivatives k_j in FP32 (MAFX_RK_K64=1: FP64)
+    bool rk_k32{true};
     std::vector<mafx::TermDev> terms;
     int demag_term{-1};       // index of the (micro or atomistic) FFT term, -1 if none
     bool atom_dipole{false};
@@ -214,9 +219,11 @@ struct EvalArgs {
     bool want_energy{false};
     // OutMode::LLG: with rk_out set, the same pass also writes the next RK stage's input
     // rk_out = rk_base + sum_{j < rk_n} rk_c[j] rk_k[j] + rk_c[rk_n] k, k the output, in the order
-    // and rounding of the separate stage combination
+    // and rounding of the separate stage combination.  With k_f32 the output k and the rk_k are
+    // float32 (the sum is FP64, with this pass's k rounded to float32 as it is stored).
+    bool k_f32{false};
     const double* rk_base{nullptr};
-    const double* rk_k[kMaxRkFused]{};
+    const void* rk_k[kMaxRkFused]{};
     double rk_c[kMaxRkFused + 1]{};
     int rk_n{0};
     double* rk_out{nullptr};
@@ -237,9 +244,10 @@ double max_abs(mafx_ctx* ctx, const double* a, int64_t n);
 void normalize(mafx_ctx* ctx, double* m, bool zero_safe);
 void mean_m(mafx_ctx* ctx, const double* m, double out[3]);
 void llg_rhs(mafx_ctx* ctx, double alpha, double dt, const double* m, const double* h, double* k);
-// out = base + sum_j c_j v_j (j < n, n <= 8), evaluated left to right
-void lincomb(mafx_ctx* ctx, double* out, const double* base, int n, const double* const* v, const double* c,
-             int64_t len);
+// out = base + sum_j c_j v_j (j < n, n <= 12), evaluated left to right in FP64; the v_j are
+// float32 when v_f32 (each converted to double as loaded)
+void lincomb(mafx_ctx* ctx, double* out, const double* base, int n, const void* const* v, const double* c,
+             int64_t len, bool v_f32 = false);
 // x = normalize(a + s * b) with the zero-safe normalisation (lbfgs_minimizer.cpp:277-278)
 void line_point(mafx_ctx* ctx, double* x, const double* a, double s, const double* b);
 

@@ -2,6 +2,7 @@
 // (fixed block count and fixed tree, so results are reproducible run to run), normalisation,
 // <m>, the LLG right-hand side, RK stage combinations, the embedded error norm and the accept
 // update.  Reference citations are relative to magnumaf/src.
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 
@@ -128,22 +129,23 @@ __global__ void __launch_bounds__(kB) k_llg_rhs(const double* __restrict__ m, co
 }
 
 struct LinArgs {
-    const double* v[kMaxStages];
+    const void* v[kMaxStages]; // K* (double, or float for FP32 stage derivatives)
     double c[kMaxStages];
     int n;
 };
 
 // out = ((base + c0 v0) + c1 v1) + ...  (the element-wise evaluation order of the reference's
-// stage inputs, adaptive_runge_kutta.cpp:139-165).  N is the vector count, so the loads of one
-// element are issued together; terms with a zero weight are skipped (x + 0 v == x for finite v).
-template <int N>
+// stage inputs, adaptive_runge_kutta.cpp:139-165), in FP64 whatever the type K of the v_j.  N is
+// the vector count, so the loads of one element are issued together; terms with a zero weight
+// are skipped (x + 0 v == x for finite v).
+template <int N, class K>
 __global__ void __launch_bounds__(kB) k_lincomb(double* __restrict__ out, const double* __restrict__ base, LinArgs a,
                                                int64_t len) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
     if (i >= len) return;
     double v[N > 0 ? N : 1];
 #pragma unroll
-    for (int j = 0; j < N; ++j) v[j] = a.c[j] != 0.0 ? a.v[j][i] : 0.0;
+    for (int j = 0; j < N; ++j) v[j] = a.c[j] != 0.0 ? static_cast<double>(static_cast<const K*>(a.v[j])[i]) : 0.0;
     double s = base[i];
 #pragma unroll
     for (int j = 0; j < N; ++j)
@@ -164,7 +166,7 @@ __global__ void __launch_bounds__(kB) k_line_point(double* __restrict__ x, const
 }
 
 struct ErrArgs {
-    const double* k[kMaxStages + 1]; // k[1..ns]
+    const void* k[kMaxStages + 1]; // k[1..ns] as K* (double, or float for FP32 stage derivatives)
     double b[kMaxStages + 1], e[kMaxStages + 1];
     int ns;
     double atol, rtol;
@@ -174,36 +176,57 @@ struct ErrArgs {
 // err = max |rk_error / (atol + rtol |max(m, m + sumbk)|)| (:108,171) as per-block maxima.
 // For RKF45 the reference forms rk_error = sumbk - (sum c_j k_j); `e` then holds c_j and
 // RKF selects that shape.  NS is the stage count, so one element's stage loads are issued
-// together; stages whose b and e weights are both zero are not read.
-template <int NS, bool RKF>
+// together; stages whose b and e weights are both zero are not read.  The sums are FP64 whatever
+// the type K of the stored k_j.
+template <int NS, bool RKF, class K>
 __global__ void __launch_bounds__(kB) k_rk_error(const double* __restrict__ m, ErrArgs a, double* __restrict__ sumbk,
                                                 int64_t len, double* __restrict__ part) {
+    // U elements per lane and trip, all their loads issued before any is used (the grid is at
+    // most kMaxRedBlocks groups, so each lane walks many elements); a max does not depend on the
+    // order its elements are taken in
+    constexpr int U = NS <= 8 ? 2 : 1;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kB;
     double emax = 0;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < len; i += static_cast<int64_t>(gridDim.x) * kB) {
-        double k[NS + 1];
+    for (int64_t i0 = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i0 < len; i0 += U * stride) {
+        K kv[U][NS + 1]; // held in the stored type until used
+        double mi[U];
 #pragma unroll
-        for (int j = 1; j <= NS; ++j) k[j] = (a.b[j] != 0.0 || a.e[j] != 0.0) ? a.k[j][i] : 0.0;
-        const double mi = m[i];
-        double sb = a.b[1] * k[1];
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + u * stride;
+            const bool in = i < len;
 #pragma unroll
-        for (int j = 2; j <= NS; ++j)
-            if (a.b[j] != 0.0) sb = sb + a.b[j] * k[j];
-        double re;
-        if constexpr (RKF) {
-            double t = a.e[1] * k[1];
-#pragma unroll
-            for (int j = 2; j <= NS; ++j)
-                if (a.e[j] != 0.0) t = t + a.e[j] * k[j];
-            re = sb - t;
-        } else {
-            re = a.e[1] * k[1];
-#pragma unroll
-            for (int j = 2; j <= NS; ++j)
-                if (a.e[j] != 0.0) re = re + a.e[j] * k[j];
+            for (int j = 1; j <= NS; ++j)
+                kv[u][j] = in && (a.b[j] != 0.0 || a.e[j] != 0.0) ? static_cast<const K*>(a.k[j])[i] : K(0);
+            mi[u] = in ? m[i] : 0.0;
         }
-        sumbk[i] = sb;
-        const double mm = fmax(mi, mi + sb);
-        emax = fmax(emax, fabs(re / (a.atol + a.rtol * fabs(mm))));
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + u * stride;
+            if (i >= len) break;
+            double k[NS + 1];
+#pragma unroll
+            for (int j = 1; j <= NS; ++j) k[j] = static_cast<double>(kv[u][j]);
+            double sb = a.b[1] * k[1];
+#pragma unroll
+            for (int j = 2; j <= NS; ++j)
+                if (a.b[j] != 0.0) sb = sb + a.b[j] * k[j];
+            double re;
+            if constexpr (RKF) {
+                double t = a.e[1] * k[1];
+#pragma unroll
+                for (int j = 2; j <= NS; ++j)
+                    if (a.e[j] != 0.0) t = t + a.e[j] * k[j];
+                re = sb - t;
+            } else {
+                re = a.e[1] * k[1];
+#pragma unroll
+                for (int j = 2; j <= NS; ++j)
+                    if (a.e[j] != 0.0) re = re + a.e[j] * k[j];
+            }
+            sumbk[i] = sb;
+            const double mm = fmax(mi[u], mi[u] + sb);
+            emax = fmax(emax, fabs(re / (a.atol + a.rtol * fabs(mm))));
+        }
     }
     emax = block_max(emax);
     if (threadIdx.x == 0) part[blockIdx.x] = emax;
@@ -547,8 +570,28 @@ void llg_rhs(mafx_ctx* ctx, double alpha, double dt, const double* m, const doub
     MAFX_HIP(hipGetLastError());
 }
 
-void lincomb(mafx_ctx* ctx, double* out, const double* base, int n, const double* const* v, const double* c,
-             int64_t len) {
+template <class K>
+void lincomb_k(mafx_ctx* ctx, double* out, const double* base, const LinArgs& a, int n, int64_t len) {
+    const dim3 g(grid_for(len, kB)), b(kB);
+    switch (n) {
+    case 0: hipLaunchKernelGGL((k_lincomb<0, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    case 1: hipLaunchKernelGGL((k_lincomb<1, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    case 2: hipLaunchKernelGGL((k_lincomb<2, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    case 3: hipLaunchKernelGGL((k_lincomb<3, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    case 4: hipLaunchKernelGGL((k_lincomb<4, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    case 5: hipLaunchKernelGGL((k_lincomb<5, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    case 6: hipLaunchKernelGGL((k_lincomb<6, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    case 7: hipLaunchKernelGGL((k_lincomb<7, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    case 8: hipLaunchKernelGGL((k_lincomb<8, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    case 9: hipLaunchKernelGGL((k_lincomb<9, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    case 10: hipLaunchKernelGGL((k_lincomb<10, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    case 11: hipLaunchKernelGGL((k_lincomb<11, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    default: hipLaunchKernelGGL((k_lincomb<12, K>), g, b, 0, ctx->stream, out, base, a, len); break;
+    }
+}
+
+void lincomb(mafx_ctx* ctx, double* out, const double* base, int n, const void* const* v, const double* c,
+             int64_t len, bool v_f32) {
     require(n >= 0 && n <= kMaxStages - 1, "lincomb: at most 12 vectors");
     LinArgs a{};
     a.n = n;
@@ -558,23 +601,11 @@ void lincomb(mafx_ctx* ctx, double* out, const double* base, int n, const double
     }
     int nz = 0;
     for (int j = 0; j < n; ++j) nz += c[j] != 0.0;
-    ProfScope ps(ctx, "rk_stage_f64", 8.0 * len * (nz + 2));
-    const dim3 g(grid_for(len, kB)), b(kB);
-    switch (n) {
-    case 0: hipLaunchKernelGGL(k_lincomb<0>, g, b, 0, ctx->stream, out, base, a, len); break;
-    case 1: hipLaunchKernelGGL(k_lincomb<1>, g, b, 0, ctx->stream, out, base, a, len); break;
-    case 2: hipLaunchKernelGGL(k_lincomb<2>, g, b, 0, ctx->stream, out, base, a, len); break;
-    case 3: hipLaunchKernelGGL(k_lincomb<3>, g, b, 0, ctx->stream, out, base, a, len); break;
-    case 4: hipLaunchKernelGGL(k_lincomb<4>, g, b, 0, ctx->stream, out, base, a, len); break;
-    case 5: hipLaunchKernelGGL(k_lincomb<5>, g, b, 0, ctx->stream, out, base, a, len); break;
-    case 6: hipLaunchKernelGGL(k_lincomb<6>, g, b, 0, ctx->stream, out, base, a, len); break;
-    case 7: hipLaunchKernelGGL(k_lincomb<7>, g, b, 0, ctx->stream, out, base, a, len); break;
-    case 8: hipLaunchKernelGGL(k_lincomb<8>, g, b, 0, ctx->stream, out, base, a, len); break;
-    case 9: hipLaunchKernelGGL(k_lincomb<9>, g, b, 0, ctx->stream, out, base, a, len); break;
-    case 10: hipLaunchKernelGGL(k_lincomb<10>, g, b, 0, ctx->stream, out, base, a, len); break;
-    case 11: hipLaunchKernelGGL(k_lincomb<11>, g, b, 0, ctx->stream, out, base, a, len); break;
-    default: hipLaunchKernelGGL(k_lincomb<12>, g, b, 0, ctx->stream, out, base, a, len); break;
-    }
+    ProfScope ps(ctx, v_f32 ? "rk_stage_k32" : "rk_stage_f64", (v_f32 ? 4.0 * nz + 16.0 : 8.0 * (nz + 2)) * len);
+    if (v_f32)
+        lincomb_k<float>(ctx, out, base, a, n, len);
+    else
+        lincomb_k<double>(ctx, out, base, a, n, len);
     MAFX_HIP(hipGetLastError());
 }
 
@@ -584,8 +615,39 @@ void line_point(mafx_ctx* ctx, double* x, const double* a, double s, const doubl
     MAFX_HIP(hipGetLastError());
 }
 
-double rk_error(mafx_ctx* ctx, const double* m, const double* const* k, const double* b, const double* e, int ns,
-                bool rkf_form, double atol, double rtol, double* sumbk) {
+template <class K>
+void rk_error_k(mafx_ctx* ctx, const double* m, const ErrArgs& a, int ns, bool rkf_form, double* sumbk, int64_t len,
+                unsigned nb) {
+    double* part = ctx->partials.as<double>();
+    const dim3 g(nb), bl(kB);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, bl, 0, ctx->stream, m, a, sumbk, len, part); };
+    if (rkf_form) {
+        switch (ns) {
+        case 1: go(k_rk_error<1, true, K>); break;
+        case 2: go(k_rk_error<2, true, K>); break;
+        case 3: go(k_rk_error<3, true, K>); break;
+        case 4: go(k_rk_error<4, true, K>); break;
+        case 5: go(k_rk_error<5, true, K>); break;
+        case 6: go(k_rk_error<6, true, K>); break;
+        case 7: go(k_rk_error<7, true, K>); break;
+        case 8: go(k_rk_error<8, true, K>); break;
+        default: go(k_rk_error<13, true, K>); break;
+        }
+    } else {
+        switch (ns) {
+        case 1: go(k_rk_error<1, false, K>); break;
+        case 2: go(k_rk_error<2, false, K>); break;
+        case 3: go(k_rk_error<3, false, K>); break;
+        case 4: go(k_rk_error<4, false, K>); break;
+        case 5: go(k_rk_error<5, false, K>); break;
+        case 6: go(k_rk_error<6, false, K>); break;
+        default: go(k_rk_error<7, false, K>); break;
+        }
+    }
+}
+
+double rk_error(mafx_ctx* ctx, const double* m, const void* const* k, const double* b, const double* e, int ns,
+                bool rkf_form, double atol, double rtol, double* sumbk, bool k_f32) {
     ErrArgs a{};
     a.ns = ns;
     for (int j = 1; j <= ns; ++j) {
@@ -596,39 +658,20 @@ double rk_error(mafx_ctx* ctx, const double* m, const double* const* k, const do
     a.atol = atol;
     a.rtol = rtol;
     const int64_t len = 3 * ctx->N;
-    const unsigned nb = red_blocks(len);
+    // one resident round of the kernel (6 groups per CU at its <= 80 VGPRs), each lane taking two
+    // elements per trip; never more partials than the context's buffer holds (50 red_blocks)
+    const unsigned nb = static_cast<unsigned>(std::max<int64_t>(
+        1, std::min<int64_t>({(len + 2 * kB - 1) / (2 * kB), int64_t{6} * ctx->num_cus, int64_t{50} * red_blocks(len)})));
     require((ns >= 1 && ns <= 8) || ns == 13, "rk_error: 1..8 or 13 stages");
     require(ns <= 7 || rkf_form, "rk_error: more than 7 stages only in the sumbk - sum e k form");
     int nread = 0;
     for (int j = 1; j <= ns; ++j) nread += (b[j] != 0.0 || e[j] != 0.0);
     {
-        ProfScope ps(ctx, "rk_error_f64", 8.0 * len * (nread + 2));
-        double* part = ctx->partials.as<double>();
-        const dim3 g(nb), bl(kB);
-        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, bl, 0, ctx->stream, m, a, sumbk, len, part); };
-        if (rkf_form) {
-            switch (ns) {
-            case 1: go(k_rk_error<1, true>); break;
-            case 2: go(k_rk_error<2, true>); break;
-            case 3: go(k_rk_error<3, true>); break;
-            case 4: go(k_rk_error<4, true>); break;
-            case 5: go(k_rk_error<5, true>); break;
-            case 6: go(k_rk_error<6, true>); break;
-            case 7: go(k_rk_error<7, true>); break;
-            case 8: go(k_rk_error<8, true>); break;
-            default: go(k_rk_error<13, true>); break;
-            }
-        } else {
-            switch (ns) {
-            case 1: go(k_rk_error<1, false>); break;
-            case 2: go(k_rk_error<2, false>); break;
-            case 3: go(k_rk_error<3, false>); break;
-            case 4: go(k_rk_error<4, false>); break;
-            case 5: go(k_rk_error<5, false>); break;
-            case 6: go(k_rk_error<6, false>); break;
-            default: go(k_rk_error<7, false>); break;
-            }
-        }
+        ProfScope ps(ctx, k_f32 ? "rk_error_k32" : "rk_error_f64", (k_f32 ? 4.0 * nread + 16.0 : 8.0 * (nread + 2)) * len);
+        if (k_f32)
+            rk_error_k<float>(ctx, m, a, ns, rkf_form, sumbk, len, nb);
+        else
+            rk_error_k<double>(ctx, m, a, ns, rkf_form, sumbk, len, nb);
         hipLaunchKernelGGL(k_final, dim3(1), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(),
                            static_cast<int>(nb), 1, 0, ctx->scal.as<double>());
     }

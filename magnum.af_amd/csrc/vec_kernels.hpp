@@ -9,9 +9,9 @@ constexpr int kMaxStages = 13;
 
 // sumbk = sum_j b_j k_j and the max-norm error of the embedded pair; k, b, e are indexed 1..ns.
 // rkf_form selects rk_error = sumbk - sum_j e_j k_j (adaptive_runge_kutta.cpp:105-106) instead of
-// rk_error = sum_j e_j k_j (:169).
-double rk_error(mafx_ctx* ctx, const double* m, const double* const* k, const double* b, const double* e, int ns,
-                bool rkf_form, double atol, double rtol, double* sumbk);
+// rk_error = sum_j e_j k_j (:169).  The k_j are double*, or float* when k_f32 (sums in FP64).
+double rk_error(mafx_ctx* ctx, const double* m, const void* const* k, const double* b, const double* e, int ns,
+                bool rkf_form, double atol, double rtol, double* sumbk, bool k_f32 = false);
 void accept(mafx_ctx* ctx, double* m, const double* dm, bool normalize_m, bool zero_safe);
 void axpy(mafx_ctx* ctx, double* y, double a, const double* x, int64_t n);
 void scale(mafx_ctx* ctx, double* y, double a, int64_t n);

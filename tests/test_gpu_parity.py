@@ -549,6 +549,87 @@ def test_llg_step_replay(scheme, mesh):
     assert np.max(np.abs(state.m - sim.m())) <= mtol
 
 
+def _film_steps(mesh, precision, scheme, nsteps, seed=7):
+    """nsteps steps of a C2-like film (demag, exchange, DMI, uniaxial, alpha = 0.1) from a
+    near-uniform state; returns the initial and final m, the step sizes and the stats."""
+    M = mf()
+    nx, ny, nz = mesh[:3]
+    rng = np.random.default_rng(seed)
+    m = np.zeros((nx, ny, nz, 3))
+    m[..., 0] = 1
+    m += 0.1 * rng.standard_normal(m.shape)
+    g = M.Mesh(*mesh)
+    terms = [M.DemagField(g), M.ExchangeField(1.5e-11), M.DmiField(3e-3), M.UniaxialAnisotropyField(8e5)]
+    state = M.State(g, 1.1e6, m)
+    m0 = state.m.copy()
+    llg = M.LLGIntegrator(0.1, terms, scheme, precision=precision)
+    hs = []
+    for _ in range(nsteps):
+        llg.step(state)
+        hs.append(llg.stats()["h_last"])
+    return m0, state.m.copy(), np.array(hs), llg.stats(), state.t
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("scheme", ["DP45", "RKF45", "BS23"])
+@pytest.mark.parametrize("mesh", [(256, 32, 4, 2e-9, 2e-9, 2e-9), (64, 16, 2, 2e-9, 2e-9, 2e-9)])
+def test_rk_fused_stage_inputs_bitwise(precision, scheme, mesh, monkeypatch):
+    """Stage inputs m + sum_j a_sj k_j formed in the per-cell pass that produces k_(s-1) (meshes
+    from 2^20 cells) and in a pass of their own (adaptive_runge_kutta.cpp:138-164) give the same
+    steps to the last bit.  MAFX_RK_FUSE_MIN_CELLS moves the threshold, so both forms run here on
+    the z-column kernel's 256-cell rows and on the generic per-cell kernel."""
+    print(f"\n[rk fused] {precision} {scheme} mesh={mesh}", flush=True)
+    runs = []
+    for thr in ["0", str(1 << 40)]:
+        monkeypatch.setenv("MAFX_RK_FUSE_MIN_CELLS", thr)
+        runs.append(_film_steps(mesh, precision, scheme, 12))
+    (_, ma, ha, sa, ta), (_, mb, hb, sb, tb) = runs
+    assert ha.tobytes() == hb.tobytes() and ta == tb
+    assert sa["accepted"] == sb["accepted"] and sa["rejected"] == sb["rejected"]
+    assert ma.tobytes() == mb.tobytes()
+
+
+def test_fp32_stage_derivatives(monkeypatch):
+    """The FP32-field integrator stores its stage derivatives k_j = dt f(m_j) in FP32 and forms
+    every sum over them in FP64.  Against the same path with FP64 k_j (MAFX_RK_K64=1):
+
+    - one DP45 step moves the state by at most 4 FP32 roundings of the step's own increment (the
+      |b_j| of DP45 sum to 1.64);
+    - from h = 1.01e-15 the embedded error estimate is rounding noise of the stage sums until the
+      steps grow, so the controller's step sizes, and the times the steps land on, follow that
+      noise in both forms; compared at the same times (integrate_dense, llg_integrator.cpp:68-104)
+      the trajectories of <m> agree to the bound below, with the step counts within 5%."""
+    mesh = (256, 32, 4, 2e-9, 2e-9, 2e-9)
+    M = mf()
+    res = {}
+    for k64 in ["0", "1"]:
+        monkeypatch.setenv("MAFX_RK_K64", k64)
+        one = _film_steps(mesh, "f32", "DP45", 1)
+        nx, ny, nz = mesh[:3]
+        rng = np.random.default_rng(7)
+        m = np.zeros((nx, ny, nz, 3))
+        m[..., 0] = 1
+        m += 0.1 * rng.standard_normal(m.shape)
+        g = M.Mesh(*mesh)
+        terms = [M.DemagField(g), M.ExchangeField(1.5e-11), M.DmiField(3e-3), M.UniaxialAnisotropyField(8e5)]
+        state = M.State(g, 1.1e6, m)
+        llg = M.LLGIntegrator(0.1, terms, "DP45", precision="f32")
+        rows = llg.integrate_dense(state, 20e-12, 1e-12)
+        res[k64] = (one, rows, llg.stats())
+    (m0, m1_32, _, _, _), rows32, s32 = res["0"]
+    (_, m1_64, _, _, _), rows64, s64 = res["1"]
+    inc = np.max(np.abs(m1_64 - m0))
+    d1 = np.max(np.abs(m1_32 - m1_64))
+    dev = np.max(np.abs(rows32[:, 1:] - rows64[:, 1:]))
+    print(f"\n[k32] one step: increment {inc:.3e}, k32 - k64 {d1:.3e} ({d1 / (inc * 2.0 ** -24):.2f} ulps); "
+          f"20 ps: <m> deviation {dev:.3e}, steps {s32['accepted']} vs {s64['accepted']}, "
+          f"rejected {s32['rejected']} vs {s64['rejected']}", flush=True)
+    assert 0 < d1 <= 4 * 2.0 ** -24 * inc
+    assert np.array_equal(rows32[:, 0], rows64[:, 0]) and len(rows32) == 21
+    assert abs(s32["accepted"] - s64["accepted"]) <= 0.05 * s64["accepted"]
+    assert dev <= 1e-5
+
+
 def test_relax_matches_reference_loop():
     """LLGIntegrator::relax (llg_integrator.cpp:43-66) on the GPU against the same loop driven
     through the oracle: E_prev = 1e20; while |(E_prev - E) / E_prev| > precision: E_prev = E,
