@@ -993,6 +993,130 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// In-place radix-2 DIT transform of length N (power of two, N <= 8) in registers, forward
+// or inverse (conjugate twiddles, no 1/N), unscaled: A[j] = sum_n a[n] w_N^(+-n j).  tw holds
+// the powers of the order-TW root of unity, tw.w[i] = exp(-2 pi i i / TW), TW a multiple of N.
+template <int N, int TW> __device__ __forceinline__ void dft_pow2(float2 (&a)[N], const ZTw<float2>& tw, bool inv) {
+    static_assert(N >= 1 && N <= 8 && (N & (N - 1)) == 0 && TW % N == 0, "length 1, 2, 4 or 8 dividing TW");
+    constexpr int LOG = N == 8 ? 3 : N == 4 ? 2 : N == 2 ? 1 : 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        int r = 0;
+#pragma unroll
+        for (int b = 0; b < LOG; ++b) r |= ((i >> b) & 1) << (LOG - 1 - b);
+        if (r > i) {
+            const float2 t = a[i];
+            a[i] = a[r];
+            a[r] = t;
+        }
+    }
+#pragma unroll
+    for (int len = 2; len <= N; len <<= 1) {
+#pragma unroll
+        for (int i = 0; i < N; i += len) {
+#pragma unroll
+            for (int j = 0; j < len / 2; ++j) {
+                const float2 u = a[i + j];
+                const float2 v = j == 0 ? a[i + j + len / 2] : cmul(a[i + j + len / 2], conj_if(tw.w[j * (TW / len)], inv));
+                a[i + j] = cadd(u, v);
+                a[i + j + len / 2] = float2{u.x - v.x, u.y - v.y};
+            }
+        }
+    }
+}
+
+// Fused z stage for power-of-two NZ as k_zfused_sym, with the length-2 NZ transforms as FFTs done
+// in place: the zero-padded forward is X[2j] = DFT_NZ(x)[j], X[2j+1] = DFT_NZ(x w_2NZ^n)[j]; the
+// tensor product replaces X[k] by H[k]; the cropped inverse is h[z] = IDFT_NZ(H_even)[z] +
+// w_2NZ^-z IDFT_NZ(H_odd)[z], z < NZ.  Only the 3 x 2 NZ spectrum values of a column are live
+// (the direct DFT holds the NZ inputs and the NZ output sums at once), and each tensor plane kz is
+// read when it is used, so NZ = 8 fits 165 VGPRs (3 waves per SIMD) instead of 256 plus AGPRs.
+template <int NZ, bool RES>
+__global__ void __launch_bounds__(256, RES ? 2 : 3)
+    k_zfused_sym_fft(float2* __restrict__ s, const float* __restrict__ t, int n1, int hx, int hxs, FastDiv n1hd,
+                     ZTw<float2> tw, TensorRes res) {
+    constexpr int NZ2 = 2 * NZ, NH = NZ + 1;
+    static_assert(NZ2 <= 16, "twiddle table of 16");
+    const int e = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    const int r = e >> 2, kt = static_cast<int>(fdiv(r, n1hd)), kys = r - kt * static_cast<int>(n1hd.d);
+    const int kx = kt * kTile + (e & (kTile - 1));
+    if (kx >= hx) return;
+    const int n1h = static_cast<int>(n1hd.d);
+    const int64_t plane = static_cast<int64_t>(n1) * hxs;
+    const int64_t Q = static_cast<int64_t>(n1h) * hxs, tc = static_cast<int64_t>(NH) * Q;
+    // every access is (uniform plane base, 32-bit lane byte offset): one offset VGPR per array
+    // instead of a 64-bit address per access (a spectrum plane and a tensor plane are below 4 GiB)
+    const uint32_t tb = static_cast<uint32_t>(tcol(kx, kTile * n1h) + kTile * kys) * 4u;
+    const int64_t scx = tcol(kx, kTile * static_cast<int64_t>(n1));
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1 && (kys == 0 || kys == n1h - 1)) break;
+        const int ky = pass == 0 ? kys : n1 - kys;
+        const float sy = pass == 0 ? 1.0f : -1.0f;
+        const int64_t col = scx + kTile * ky;
+        const uint32_t sb = static_cast<uint32_t>(opaque_int(static_cast<int>(col))) * 8u;
+        const auto sp = [&](int pl) { return reinterpret_cast<float2*>(reinterpret_cast<char*>(s + pl * plane) + sb); };
+        // X[c][0..NZ) even k-points, X[c][NZ..2NZ) odd ones
+        float2 ev[3][NZ], od[3][NZ];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int z = 0; z < NZ; ++z) ev[c][z] = *sp(c * NZ + z);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int z = 0; z < NZ; ++z) od[c][z] = z == 0 ? ev[c][0] : cmul(ev[c][z], tw.w[z]);
+            dft_pow2<NZ, NZ2>(ev[c], tw, false);
+            dft_pow2<NZ, NZ2>(od[c], tw, false);
+        }
+        // H = N X at k-points kz and 2 NZ - kz (the tensor's parity in kz), the tensor plane kz read
+        // once per column through an offset the compiler cannot hoist out of the column loop
+        const uint32_t tbp = static_cast<uint32_t>(opaque_int(static_cast<int>(tb)));
+        const auto product = [&](float2& X0, float2& X1, float2& X2, int k, const float (&tv)[6]) {
+            const float2 x0 = X0, x1 = X1, x2 = X2;
+            const float sz = k <= NZ ? 1.0f : -1.0f;
+            const float nxx = tv[0], nxy = sy * tv[1], nxz = sz * tv[2], nyy = tv[3], nyz = sy * sz * tv[4], nzz = tv[5];
+            if constexpr (RES) {
+                float2 T[6];
+                const float sv[6] = {nxx, nxy, nxz, nyy, nyz, nzz};
+#pragma unroll
+                for (int c = 0; c < 6; ++c) T[c] = res_entry(res, c, (c * NZ2 + k) * plane + col, sv[c]);
+                X0 = cadd(cadd(cmul(T[0], x0), cmul(T[1], x1)), cmul(T[2], x2));
+                X1 = cadd(cadd(cmul(T[1], x0), cmul(T[3], x1)), cmul(T[4], x2));
+                X2 = cadd(cadd(cmul(T[2], x0), cmul(T[4], x1)), cmul(T[5], x2));
+            } else {
+                X0 = float2{nxx * x0.x + nxy * x1.x + nxz * x2.x, nxx * x0.y + nxy * x1.y + nxz * x2.y};
+                X1 = float2{nxy * x0.x + nyy * x1.x + nyz * x2.x, nxy * x0.y + nyy * x1.y + nyz * x2.y};
+                X2 = float2{nxz * x0.x + nyz * x1.x + nzz * x2.x, nxz * x0.y + nyz * x1.y + nzz * x2.y};
+            }
+        };
+        const auto at_k = [&](int k, const float (&tv)[6]) {
+            if (k & 1)
+                product(od[0][k >> 1], od[1][k >> 1], od[2][k >> 1], k, tv);
+            else
+                product(ev[0][k >> 1], ev[1][k >> 1], ev[2][k >> 1], k, tv);
+        };
+#pragma unroll
+        for (int kz = 0; kz <= NZ; ++kz) {
+            float tv[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                tv[c] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(t + c * tc + kz * Q) + tbp);
+            at_k(kz, tv);
+            if (kz != 0 && kz != NZ) at_k(NZ2 - kz, tv);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            dft_pow2<NZ, NZ2>(ev[c], tw, true);
+            dft_pow2<NZ, NZ2>(od[c], tw, true);
+#pragma unroll
+            for (int z = 0; z < NZ; ++z)
+                *sp(c * NZ + z) = z == 0 ? cadd(ev[c][0], od[c][0])
+                                                       : cadd(ev[c][z], cmul(od[c][z], conj_if(tw.w[z], true)));
+        }
+    }
+}
+
 int lines_per_block(int L, int elems) {
     int W = elems / L;
     if (W < 1) W = 1;
@@ -1333,6 +1457,16 @@ void zfused_sym(mafx_ctx* ctx, float2* spec, const float* ts, int nz, const Tens
     const TensorRes rv = res ? *res : TensorRes{};
     auto launch = [&](auto nzc) {
         constexpr int NZ = decltype(nzc)::value;
+        // NZ = 8 through the in-place FFT form (165 VGPRs, 3 waves per SIMD, against 256 + AGPRs
+        // and one wave for the direct DFT: 740 -> 536 us at 2048 x 1024 x 8); at NZ = 4 the direct
+        // DFT is faster (92 against 113 us at 1024 x 1024 x 4)
+        if constexpr (NZ == 8) {
+            if (res)
+                hipLaunchKernelGGL((k_zfused_sym_fft<NZ, true>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw, rv);
+            else
+                hipLaunchKernelGGL((k_zfused_sym_fft<NZ, false>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw, rv);
+            return;
+        }
         if (res)
             hipLaunchKernelGGL((k_zfused_sym<NZ, true>), g, b, 0, ctx->stream, spec, ts, n1, hx, hxs, n1hd, tw, rv);
         else

@@ -334,8 +334,8 @@ def test_heff_fp32_field_error(mesh):
                                   (12, 6, 10, 2e-9, 2e-9, 2e-9)])
 def test_fp32_full_tensor(mesh):
     """DemagField(full_tensor=True): the FP32 field on the reference tensor's spectrum (symmetric
-    quarter + FP16 residual) against the oracle, on the fused z stages (nz = 2, the rolled nz = 5
-    and 7, the paired-lane nz = 8), the nz = 1 product and the strided z passes (nz = 10)."""
+    quarter + FP16 residual) against the oracle, on the fused z stages (the direct DFT at nz = 2, 5
+    and 7, the in-place FFT form at nz = 8), the nz = 1 product and the strided z passes (nz = 10)."""
     print(f"\n[fp32 full tensor] mesh={mesh}", flush=True)
     m = rand_m(mesh[:3], 8)
     state, terms, sim = build(mesh, 1.1e6, m, MICRO_ALL)
