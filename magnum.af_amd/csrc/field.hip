@@ -530,14 +530,15 @@ __global__ void __launch_bounds__(kBlock, MINW) k_epi_cells(EpiParams P, FastDiv
     finish_block<RED>(P, acc);
 }
 
-// Per-cell stage with z-column lanes, for rows of at least one work-group (LLG outputs, and field
-// outputs of films thicker than k_epi_col4 takes; no energy sums).  Work-group g (after the XCD renumbering) takes cells
+// Per-cell stage with z-column lanes, for rows of at least one work-group (LLG outputs, field
+// outputs of films thicker than k_epi_col4 takes, and with RED the energy sums and the LBFGS
+// gradient's maxima, reduced per work-group as in k_epi_cells).  Work-group g (after the XCD renumbering) takes cells
 // x0 .. x0 + kBlock - 1 of row y at every z, and lane t walks its column z = 0 .. nz - 1: the z
 // neighbours are the lane's previous and next cells, the x neighbours come from an LDS row of the
 // group's cells at the same z (one halo cell each side, loaded by the edge lanes), so only the
 // centre, the next z and the two y neighbours are read from memory, 9 loads per cell instead
 // of 21.  Zero padding as in cell_epilogue.
-template <class T, class TH, uint64_t PROG, int MINW>
+template <class T, class TH, bool RED, uint64_t PROG, int MINW>
 __global__ void __launch_bounds__(kBlock, MINW) k_epi_zcol(EpiParams P, FastDiv bprd) {
     __shared__ V3<T> row[kBlock + 2];
     const int nx = static_cast<int>(P.nx), ny = static_cast<int>(P.ny), nz = static_cast<int>(P.nz);
@@ -559,7 +560,7 @@ __global__ void __launch_bounds__(kBlock, MINW) k_epi_zcol(EpiParams P, FastDiv 
     double cur[3];
     ld3(o, cur);
     V3<T> prev{0, 0, 0};
-    double acc[1] = {0};
+    double acc[RED ? kMaxTerms + 2 : 1] = {};
     for (int z = 0; z < nz; ++z) {
         double nxt[3] = {0, 0, 0}, ym[3] = {0, 0, 0}, yp[3] = {0, 0, 0}, hv[3] = {0, 0, 0};
         if (z + 1 < nz) ld3(o + sz, nxt);
@@ -583,13 +584,14 @@ __global__ void __launch_bounds__(kBlock, MINW) k_epi_zcol(EpiParams P, FastDiv 
         __syncthreads();
         if (valid) {
             const V3<T> nb[6] = {row[t], (x + 1 < nx) ? row[t + 2] : V3<T>{0, 0, 0}, to_t(ym), to_t(yp), prev, to_t(nxt)};
-            cell_terms<T, false, PROG>(P, x, y, z, cur[0], cur[1], cur[2], nb, hd, acc);
+            cell_terms<T, RED, PROG>(P, x, y, z, cur[0], cur[1], cur[2], nb, hd, acc);
         }
         prev = to_t(cur);
 #pragma unroll
         for (int c = 0; c < 3; ++c) cur[c] = nxt[c];
         o += sz;
     }
+    finish_block<RED>(P, acc); // every lane of the group gets here
 }
 
 // Per-cell stage for films of at most four layers on 2-D tiles: a work-group of kTileX x kTileY
@@ -850,15 +852,34 @@ template <class C, class TH, class T> void evaluate_impl(mafx_ctx* ctx, const Ev
         constexpr int MW = sizeof(T) == 4 ? 5 : 1;
         auto kern = red ? (rowmap ? k_epi_cells<T, TH, true, true, 1> : k_epi_cells<T, TH, true, false, 1>)
                         : (rowmap ? k_epi_cells<T, TH, false, true, MW> : k_epi_cells<T, TH, false, false, MW>);
-        // compile-time term programs of the common micromagnetic term lists (field and LLG
-        // evaluations); any other list runs the run-time term loop
+        // compile-time term programs of the common micromagnetic term lists; any other list runs
+        // the run-time term loop
+        uint64_t prog = P.has_fields ? 0 : static_cast<uint64_t>(P.nterms);
+        for (int t = 0; t < P.nterms && !P.has_fields; ++t) prog |= static_cast<uint64_t>(P.t[t].kind) << (4 + 4 * t);
+        constexpr uint64_t kP1 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_DMI, MAFX_UNIAXIAL});
+        constexpr uint64_t kP2 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_ZEEMAN});
+        constexpr uint64_t kP3 = prog_of({MAFX_DEMAG, MAFX_RKKY, MAFX_DMI, MAFX_UNIAXIAL, MAFX_ZEEMAN});
+        constexpr uint64_t kP4 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_DMI, MAFX_UNIAXIAL, MAFX_ZEEMAN});
+        if (red && rowmap && P.needs_nb && g.nx % kBlock == 0) {
+            // energy sums / LBFGS gradient on z-column lanes: one partial per work-group and slot
+            using KZ = void (*)(EpiParams, FastDiv);
+            KZ kz = k_epi_zcol<T, TH, true, 0, 1>;
+            if (prog == kP1) kz = k_epi_zcol<T, TH, true, kP1, 1>;
+            if (prog == kP2) kz = k_epi_zcol<T, TH, true, kP2, 1>;
+            if (prog == kP3) kz = k_epi_zcol<T, TH, true, kP3, 1>;
+            if (prog == kP4) kz = k_epi_zcol<T, TH, true, kP4, 1>;
+            const int64_t ng = g.ny * bpr;
+            require(ng <= nblocks, "z-column partials exceed the per-cell partials");
+            EpiParams Pz = P;
+            Pz.nblocks = ng;
+            hipLaunchKernelGGL(kz, dim3(static_cast<unsigned>(ng)), dim3(kBlock), 0, ctx->stream, Pz,
+                               make_fastdiv(static_cast<uint32_t>(bpr)));
+            MAFX_HIP(hipGetLastError());
+            ctx->scal_nblocks = ng;
+            ctx->scal_nterms = P.nterms;
+            return;
+        }
         if (!red) {
-            uint64_t prog = P.has_fields ? 0 : static_cast<uint64_t>(P.nterms);
-            for (int t = 0; t < P.nterms && !P.has_fields; ++t) prog |= static_cast<uint64_t>(P.t[t].kind) << (4 + 4 * t);
-            constexpr uint64_t kP1 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_DMI, MAFX_UNIAXIAL});
-            constexpr uint64_t kP2 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_ZEEMAN});
-            constexpr uint64_t kP3 = prog_of({MAFX_DEMAG, MAFX_RKKY, MAFX_DMI, MAFX_UNIAXIAL, MAFX_ZEEMAN});
-            constexpr uint64_t kP4 = prog_of({MAFX_DEMAG, MAFX_EXCHANGE, MAFX_DMI, MAFX_UNIAXIAL, MAFX_ZEEMAN});
             if (prog == kP1) kern = rowmap ? k_epi_cells<T, TH, false, true, MW, kP1> : k_epi_cells<T, TH, false, false, MW, kP1>;
             if (prog == kP2) kern = rowmap ? k_epi_cells<T, TH, false, true, MW, kP2> : k_epi_cells<T, TH, false, false, MW, kP2>;
             if (prog == kP3) kern = rowmap ? k_epi_cells<T, TH, false, true, MW, kP3> : k_epi_cells<T, TH, false, false, MW, kP3>;
@@ -890,11 +911,11 @@ template <class C, class TH, class T> void evaluate_impl(mafx_ctx* ctx, const Ev
                     ctx->scal_nterms = P.nterms;
                     return;
                 }
-                KZ kz = k_epi_zcol<T, TH, 0, MWW>;
-                if (prog == kP1) kz = k_epi_zcol<T, TH, kP1, MWZ>;
-                if (prog == kP2) kz = k_epi_zcol<T, TH, kP2, MWZ>;
-                if (prog == kP3) kz = k_epi_zcol<T, TH, kP3, MWW>;
-                if (prog == kP4) kz = k_epi_zcol<T, TH, kP4, MWZ>;
+                KZ kz = k_epi_zcol<T, TH, false, 0, MWW>;
+                if (prog == kP1) kz = k_epi_zcol<T, TH, false, kP1, MWZ>;
+                if (prog == kP2) kz = k_epi_zcol<T, TH, false, kP2, MWZ>;
+                if (prog == kP3) kz = k_epi_zcol<T, TH, false, kP3, MWW>;
+                if (prog == kP4) kz = k_epi_zcol<T, TH, false, kP4, MWZ>;
                 const int64_t ng = g.ny * bpr;
                 hipLaunchKernelGGL(kz, dim3(static_cast<unsigned>(ng)), dim3(kBlock), 0, ctx->stream, P,
                                    make_fastdiv(static_cast<uint32_t>(bpr)));
