@@ -118,13 +118,19 @@ def test_heff_fp64_matches_oracle(mesh, which):
                                   (256, 256, 1, 3e-9, 3e-9, 3e-9)])
 def test_heff_fp64_pow2_axes(mesh):
     """Power-of-two nx of 256..2048 or ny of 256..1024: the x or y transforms run the
-    compile-time shaped kernels (row groups of one plane, whole column tiles)."""
+    compile-time shaped kernels (row groups of one plane, whole column tiles).  The energy on rows
+    of whole work-groups (nx >= 256) is summed on the z-column lanes with per-group partials."""
     print(f"\n[heff fp64 pow2 axes] mesh={mesh}", flush=True)
     m = rand_m(mesh[:3], 21)
     state, terms, sim = build(mesh, 1.1e6, m, MICRO_ALL)
     M = mf()
-    h = M.LLGIntegrator(0.1, terms).H_in_Apm(state)
+    llg = M.LLGIntegrator(0.1, terms)
+    h = llg.H_in_Apm(state)
     assert normwise(h, sim.heff()) <= FP64_TOL, normwise(h, sim.heff())
+    E_ref, Et_ref = sim.energy()
+    assert abs(llg.Eeff_in_J(state) - E_ref) <= FP64_TOL * energy_scale(mesh, 1.1e6, Et_ref)
+    for i, t in enumerate(terms):
+        assert abs(t.Energy_in_J(state) - Et_ref[i]) <= FP64_TOL * energy_scale(mesh, 1.1e6, Et_ref), i
 
 
 @pytest.mark.parametrize("precision,tol", [("f64", FP64_TOL), ("f32", 1e-5)])
@@ -767,6 +773,26 @@ def test_integrate_dense_matches_oracle():
     ref = sim.integrate_dense(5e-11, 1e-11)
     assert rows.shape == ref.shape
     assert np.max(np.abs(rows - ref)) <= 1e-10
+
+
+def test_lbfgs_rows_matches_oracle():
+    """LBFGS on rows of 256 cells, where the gradient, its maxima and the energies come from the
+    z-column lanes with per-group partials: the final energy of 60 iterations against the oracle's
+    (lbfgs_minimizer.cpp:55-198)."""
+    mesh = (256, 4, 2, 3e-9, 3e-9, 3e-9)
+    m = rand_m(mesh[:3], 23)
+    specs = [("demag", {}), ("exchange", {"A": 1.5e-11}), ("dmi", {"D": 3e-3}), ("uniaxial", {"K": 8e5}),
+             ("zeeman_h", {"h": (0, 0, 0.2 / MU0)})]
+    state, terms, sim = build(mesh, 1.1e6, m, specs)
+    M = mf()
+    E0 = M.LLGIntegrator(0.1, terms).Eeff_in_J(state)
+    mini = M.LBFGS_Minimizer(terms, tol=1e-6, maxiter=60)
+    E = mini.minimize(state)
+    r = sim.lbfgs(tol=1e-6, maxiter=60)
+    print(f"\n[lbfgs rows] E0={E0:.6e} gpu E={E:.12e} it={mini.last_stats['iterations']} "
+          f"oracle E={r['E']:.12e} it={r['iterations']}", flush=True)
+    assert E < E0
+    assert abs(E - r["E"]) <= 1e-9 * abs(r["E"])
 
 
 def test_lbfgs_matches_oracle():
