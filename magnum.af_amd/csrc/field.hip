@@ -23,6 +23,7 @@
 #include "fft_device.hpp"
 #include "fft_kernels.hpp"
 #include "mafx_internal.hpp"
+#include "vec_kernels.hpp"
 #include "tensor_cache.hpp"
 
 namespace mafx {
@@ -937,6 +938,7 @@ template <class C, class TH, class T> void evaluate_impl(mafx_ctx* ctx, const Ev
 } // namespace
 
 void evaluate(mafx_ctx* ctx, const EvalArgs& a) {
+    lbfgs_flush(ctx); // the per-cell pass overwrites the reduction partials
     require(ctx->terms.size() <= static_cast<size_t>(kMaxTerms), "at most 8 terms per context");
     // time-dependent external fields (external_field.cpp:28-36): the callback fills the term's
     // field for this evaluation's time and magnetisation

@@ -292,6 +292,17 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
     double* ds = dsc.as<double>();
     double* d_dot = ds + hist;
     double* d_coef = ds + hist + 1;
+    // A two-loop step's final reduction may be left pending on a context (lbfgs_flush); it points
+    // into dsc, so none may outlive this call: discarded on every exit.
+    struct PendingGuard {
+        mafx_ctx* a;
+        mafx_ctx* b;
+        ~PendingGuard() {
+            a->lbfgs_pend.mode = -1;
+            if (b) b->lbfgs_pend.mode = -1;
+        }
+    } pending_guard{ctx, ctx->lbfgs_finish};
+    ctx->lbfgs_pend.mode = -1;
     const double eps = 2.22e-16;
     const double eps2 = std::sqrt(eps);
     const double epsr = std::pow(eps, 0.9);

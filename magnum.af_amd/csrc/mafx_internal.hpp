@@ -173,6 +173,17 @@ struct mafx_ctx {
     mafx::DevBuf hbuf;        // 3N f64 scratch (field output for helpers)
     mafx::DevBuf partials;    // reduction partials (f64)
     mafx::DevBuf scal;        // small device scalars
+    // LBFGS two-loop step whose final reduction (its dot product, alpha_i and coefficient) is
+    // still pending in partials: the next two-loop pass forms it, or lbfgs_flush launches it
+    struct LbfgsPending {
+        int mode{-1}; // -1 none; 0 dot only; 1 alpha_i = rho d, coef = -alpha_i; 2 coef = alpha_i - rho d
+        const double* part{nullptr};
+        int nb{0};
+        double rho{0};
+        double* alpha_i{nullptr};
+        double* coef{nullptr};
+        double* dot{nullptr};
+    } lbfgs_pend;
     double* host_scal{nullptr}; // pinned host scalars (hipHostMalloc)
     int64_t scal_nblocks{0};  // partial-sum blocks of the last evaluation
     int scal_nterms{0};       // terms selected in the last evaluation

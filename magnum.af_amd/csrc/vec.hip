@@ -278,6 +278,7 @@ double read_scalar(mafx_ctx* ctx, int nslots) {
 } // namespace
 
 double dot(mafx_ctx* ctx, const double* a, const double* b, int64_t n) {
+    lbfgs_flush(ctx);
     const unsigned nb = red_blocks(n);
     {
         ProfScope ps(ctx, "dot_f64", 16.0 * n);
@@ -290,6 +291,7 @@ double dot(mafx_ctx* ctx, const double* a, const double* b, int64_t n) {
 }
 
 double max_abs(mafx_ctx* ctx, const double* a, int64_t n) {
+    lbfgs_flush(ctx);
     const unsigned nb = red_blocks(n);
     {
         ProfScope ps(ctx, "maxabs_f64", 8.0 * n);
@@ -388,6 +390,7 @@ __global__ void k_lbfgs_beta(double* coef, const double* alpha_i, const double* 
 } // namespace
 
 void dot_dev(mafx_ctx* ctx, const double* a, const double* b, int64_t n, double* out_dev) {
+    lbfgs_flush(ctx);
     const unsigned nb = red_blocks(n);
     ProfScope ps(ctx, "dot_f64", 16.0 * n);
     hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kB), 0, ctx->stream, a, b, n, ctx->partials.as<double>());
@@ -397,6 +400,7 @@ void dot_dev(mafx_ctx* ctx, const double* a, const double* b, int64_t n, double*
 }
 
 void max_abs_dev(mafx_ctx* ctx, const double* a, int64_t n, double* out_dev) {
+    lbfgs_flush(ctx);
     const unsigned nb = red_blocks(n);
     ProfScope ps(ctx, "maxabs_f64", 8.0 * n);
     hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(kB), 0, ctx->stream, a, n, ctx->partials.as<double>());
@@ -410,14 +414,45 @@ namespace {
 // (no axpy when coef is null), then qi = scale qi when do_scale, stored as q_out = qi or, negate,
 // q_out = -qi; and partials of dnext . qi.  Every expression and the element-to-thread mapping are
 // those of k_axpy_dev, k_scale and k_dot, so the results equal the unfused sequence bit for bit.
+// PMODE >= 1: the coefficient is the previous step's, formed here from its partials ppart[0..pnb)
+// with k_final_coef<PMODE>'s tree and expressions (every work-group forms the same value; group 0
+// also stores the dot product, alpha_i and the coefficient as k_final_coef would).  PMODE = -1:
+// the coefficient is read from coef (none when coef is null).
+template <int PMODE>
 __global__ void __launch_bounds__(kB) k_lbfgs_fused(const double* q_in, const double* __restrict__ coef,
                                                    const double* __restrict__ v, double scale, int do_scale,
                                                    const double* __restrict__ dnext, double* q_out, int negate,
-                                                   int64_t n, double* __restrict__ part) {
+                                                   int64_t n, double* __restrict__ part, const double* ppart, int pnb,
+                                                   double prho, double* palpha, double* pcoef, double* pdot) {
+    double c = 0;
+    bool axpy = coef != nullptr;
+    if constexpr (PMODE >= 1) {
+        axpy = true;
+        double d = 0;
+        for (int i = threadIdx.x; i < pnb; i += kB) d = d + ppart[i];
+        d = block_sum(d);
+        const bool g0 = blockIdx.x == 0 && threadIdx.x == 0;
+        if constexpr (PMODE == 1) {
+            const double a = prho * d;
+            c = -a;
+            if (g0) *palpha = a;
+        }
+        if constexpr (PMODE == 2) {
+#pragma clang fp contract(off)
+            const double beta = prho * d;
+            c = *palpha - beta;
+        }
+        if (g0) {
+            *pdot = d;
+            *pcoef = c;
+        }
+    } else {
+        if (axpy) c = *coef;
+    }
     double s = 0;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * kB) {
         double qi = q_in[i];
-        if (coef) qi = qi + (*coef) * v[i];
+        if (axpy) qi = qi + c * v[i];
         if (do_scale) qi = scale * qi;
         q_out[i] = negate ? -qi : qi;
         s += dnext[i] * qi;
@@ -490,21 +525,36 @@ __global__ void __launch_bounds__(kB) k_neg(double* __restrict__ y, const double
 }
 } // namespace
 
-namespace {
-void final_coef(mafx_ctx* ctx, unsigned nb, double* dot_out, int mode, double rho, double* alpha_i, double* coef) {
-    const double* part = ctx->partials.as<double>();
-    const int n = static_cast<int>(nb);
+void lbfgs_flush(mafx_ctx* ctx) {
+    auto& p = ctx->lbfgs_pend;
+    if (p.mode < 0) return;
+    const int mode = p.mode;
+    p.mode = -1;
+    double* part = const_cast<double*>(p.part);
     switch (mode) {
     case 1:
-        hipLaunchKernelGGL(k_final_coef<1>, dim3(1), dim3(kB), 0, ctx->stream, part, n, dot_out, rho, alpha_i, coef);
+        hipLaunchKernelGGL(k_final_coef<1>, dim3(1), dim3(kB), 0, ctx->stream, part, p.nb, p.dot, p.rho, p.alpha_i, p.coef);
         break;
     case 2:
-        hipLaunchKernelGGL(k_final_coef<2>, dim3(1), dim3(kB), 0, ctx->stream, part, n, dot_out, rho, alpha_i, coef);
+        hipLaunchKernelGGL(k_final_coef<2>, dim3(1), dim3(kB), 0, ctx->stream, part, p.nb, p.dot, p.rho, p.alpha_i, p.coef);
         break;
     default:
-        hipLaunchKernelGGL(k_final_coef<0>, dim3(1), dim3(kB), 0, ctx->stream, part, n, dot_out, rho, alpha_i, coef);
+        hipLaunchKernelGGL(k_final_coef<0>, dim3(1), dim3(kB), 0, ctx->stream, part, p.nb, p.dot, p.rho, p.alpha_i, p.coef);
         break;
     }
+    MAFX_HIP(hipGetLastError());
+}
+
+namespace {
+// The half of the partials buffer the pending step does not hold ([0, nb) or [nb, 2 nb); the
+// buffer holds at least 10 red_blocks(3 N) doubles).
+double* free_half(mafx_ctx* ctx, int nb) {
+    double* base = ctx->partials.as<double>();
+    return ctx->lbfgs_pend.mode >= 0 && ctx->lbfgs_pend.part == base ? base + nb : base;
+}
+void set_pending(mafx_ctx* ctx, int mode, const double* part, int nb, double rho, double* alpha_i, double* coef,
+                 double* dot) {
+    ctx->lbfgs_pend = mafx_ctx::LbfgsPending{mode, part, nb, rho, alpha_i, coef, dot};
 }
 } // namespace
 
@@ -513,23 +563,41 @@ void lbfgs_fused(mafx_ctx* ctx, const double* q_in, const double* coef_dev, cons
                  double rho, double* alpha_i, double* coef_out) {
     const unsigned nb = red_blocks(n);
     ProfScope ps(ctx, "lbfgs_twoloop_f64", 8.0 * n * (coef_dev ? 4 : 3));
-    hipLaunchKernelGGL(k_lbfgs_fused, dim3(nb), dim3(kB), 0, ctx->stream, q_in, coef_dev, v, scale, do_scale ? 1 : 0,
-                       dnext, q_out, negate ? 1 : 0, n, ctx->partials.as<double>());
-    final_coef(ctx, nb, out_dev, mode, rho, alpha_i, coef_out);
+    const auto pend = ctx->lbfgs_pend;
+    // the previous step's coefficient is formed in this pass when it is the one this step uses
+    const bool chain = pend.mode >= 1 && coef_dev != nullptr && coef_dev == pend.coef;
+    if (!chain) lbfgs_flush(ctx);
+    require(2 * static_cast<int64_t>(nb) <= static_cast<int64_t>(ctx->partials.bytes / sizeof(double)),
+            "lbfgs: partials buffer too small");
+    double* part = free_half(ctx, static_cast<int>(nb));
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(kB), 0, ctx->stream, q_in, coef_dev, v, scale, do_scale ? 1 : 0, dnext,
+                           q_out, negate ? 1 : 0, n, part, pend.part, pend.nb, pend.rho, pend.alpha_i, pend.coef, pend.dot);
+    };
+    if (chain && pend.mode == 1)
+        go(k_lbfgs_fused<1>);
+    else if (chain && pend.mode == 2)
+        go(k_lbfgs_fused<2>);
+    else
+        go(k_lbfgs_fused<-1>);
     MAFX_HIP(hipGetLastError());
+    set_pending(ctx, mode, part, static_cast<int>(nb), rho, alpha_i, coef_out, out_dev);
 }
 
 void lbfgs_dot_alpha(mafx_ctx* ctx, const double* a, const double* b, int64_t n, double* dot_out, double rho,
                      double* alpha_i, double* coef) {
+    lbfgs_flush(ctx);
     const unsigned nb = red_blocks(n);
     ProfScope ps(ctx, "dot_f64", 16.0 * n);
-    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kB), 0, ctx->stream, a, b, n, ctx->partials.as<double>());
-    final_coef(ctx, nb, dot_out, 1, rho, alpha_i, coef);
+    double* part = ctx->partials.as<double>();
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kB), 0, ctx->stream, a, b, n, part);
     MAFX_HIP(hipGetLastError());
+    set_pending(ctx, 1, part, static_cast<int>(nb), rho, alpha_i, coef, dot_out);
 }
 
 void lbfgs_pair(mafx_ctx* ctx, const double* m, const double* x_old, const double* g, const double* g_old,
                 double* s, double* y, int64_t n, double* out_dev) {
+    lbfgs_flush(ctx);
     const unsigned nb = red_blocks(n);
     ProfScope ps(ctx, "lbfgs_pair_f64", 48.0 * n);
     hipLaunchKernelGGL(k_lbfgs_pair, dim3(nb), dim3(kB), 0, ctx->stream, m, x_old, g, g_old, s, y, n,
@@ -560,6 +628,7 @@ void lbfgs_beta(mafx_ctx* ctx, double* coef, const double* alpha_i, const double
 }
 
 void read_device(mafx_ctx* ctx, const double* src_dev, int n, double* dst_host) {
+    lbfgs_flush(ctx);
     MAFX_HIP(hipMemcpyAsync(ctx->host_scal, src_dev, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
     MAFX_HIP(hipStreamSynchronize(ctx->stream));
     for (int i = 0; i < n; ++i) dst_host[i] = ctx->host_scal[i];

@@ -35,6 +35,10 @@ void read_device(mafx_ctx* ctx, const double* src_dev, int n, double* dst_host);
 // mode 1 / 2 also forms the next step's scalars from the dot product d = *out_dev, as lbfgs_alpha
 // (alpha_i = rho d, coef = -alpha_i) / lbfgs_beta (coef = alpha_i - rho d) would, in the same launch
 // as the reduction's last stage.  lbfgs_dot_alpha: *dot_out = a . b with dot()'s tree, then mode 1.
+// Launches the final reduction a two-loop step left pending (lbfgs_fused, lbfgs_dot_alpha), if any:
+// the next two-loop pass forms it itself when it uses that step's coefficient; every other user of
+// the partials buffer or of the step's scalars flushes it first.
+void lbfgs_flush(mafx_ctx* ctx);
 void lbfgs_fused(mafx_ctx* ctx, const double* q_in, const double* coef_dev, const double* v, double scale,
                  bool do_scale, const double* dnext, double* q_out, bool negate, int64_t n, double* out_dev,
                  int mode = 0, double rho = 0.0, double* alpha_i = nullptr, double* coef_out = nullptr);
