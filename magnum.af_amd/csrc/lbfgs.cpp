@@ -49,8 +49,7 @@ struct Lbfgs {
         evaluate(ctx, ea);
         const int ns = finish_energy_launch(ctx);
         double* sc = ctx->scal.as<double>();
-        dot_dev(ctx, g, s, n3, sc + ns);
-        max_abs_dev(ctx, g, n3, sc + ns + 1);
+        dot_max_dev(ctx, g, s, n3, sc + ns);
         MAFX_HIP(hipMemcpyAsync(ctx->host_scal, sc, sizeof(double) * (ns + 2), hipMemcpyDeviceToHost, ctx->stream));
         MAFX_HIP(hipStreamSynchronize(ctx->stream));
         double E = 0;

@@ -54,6 +54,24 @@ __global__ void __launch_bounds__(kB) k_dot(const double* __restrict__ a, const 
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// k_dot's a . b and k_maxabs' max |a| in one pass over a: partials [0][block] and [1][block], each
+// formed as the two kernels form theirs
+__global__ void __launch_bounds__(kB) k_dot_maxabs(const double* __restrict__ a, const double* __restrict__ b,
+                                                  int64_t n, double* __restrict__ part) {
+    double s = 0, mx = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * kB) {
+        const double ai = a[i];
+        s += ai * b[i];
+        mx = fmax(mx, fabs(ai));
+    }
+    s = block_sum(s);
+    mx = block_max(mx);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s;
+        part[gridDim.x + blockIdx.x] = mx;
+    }
+}
+
 __global__ void __launch_bounds__(kB) k_maxabs(const double* __restrict__ a, int64_t n, double* __restrict__ part) {
     double s = 0;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * kB)
@@ -399,6 +417,15 @@ void dot_dev(mafx_ctx* ctx, const double* a, const double* b, int64_t n, double*
     MAFX_HIP(hipGetLastError());
 }
 
+void dot_max_dev(mafx_ctx* ctx, const double* a, const double* b, int64_t n, double* out_dev) {
+    lbfgs_flush(ctx);
+    const unsigned nb = red_blocks(n);
+    ProfScope ps(ctx, "dot_max_f64", 16.0 * n);
+    hipLaunchKernelGGL(k_dot_maxabs, dim3(nb), dim3(kB), 0, ctx->stream, a, b, n, ctx->partials.as<double>());
+    hipLaunchKernelGGL(k_final, dim3(2), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(), static_cast<int>(nb),
+                       2, 1, out_dev);
+    MAFX_HIP(hipGetLastError());
+}
 void max_abs_dev(mafx_ctx* ctx, const double* a, int64_t n, double* out_dev) {
     lbfgs_flush(ctx);
     const unsigned nb = red_blocks(n);
