@@ -4,7 +4,9 @@
 // update.  Reference citations are relative to magnumaf/src.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <mutex>
+#include <type_traits>
 
 #include "mafx_internal.hpp"
 #include "vec_kernels.hpp"
@@ -196,54 +198,92 @@ struct ErrArgs {
 // RKF selects that shape.  NS is the stage count, so one element's stage loads are issued
 // together; stages whose b and e weights are both zero are not read.  The sums are FP64 whatever
 // the type K of the stored k_j.
-template <int NS, bool RKF, class K>
+template <int NS, bool RKF, class K, int P>
 __global__ void __launch_bounds__(kB) k_rk_error(const double* __restrict__ m, ErrArgs a, double* __restrict__ sumbk,
                                                 int64_t len, double* __restrict__ part) {
-    // U elements per lane and trip, all their loads issued before any is used (the grid is at
-    // most kMaxRedBlocks groups, so each lane walks many elements); a max does not depend on the
-    // order its elements are taken in
+    // Units of P consecutive elements (P = 2: 8-byte loads of float k_j, 16-byte loads of m and
+    // stores of sumbk), U units per lane and trip with all their loads issued before any is used
+    // (the grid is one resident round of groups, so each lane walks many units); the P - 1
+    // elements at most past the last whole unit go to the first lane.  A max does not depend on
+    // the order its elements are taken in.
     constexpr int U = NS <= 8 ? 2 : 1;
+    using KP = typename std::conditional<P == 2, typename std::conditional<sizeof(K) == 4, float2, double2>::type, K>::type;
+    using MP = typename std::conditional<P == 2, double2, double>::type;
+    const int64_t nunit = len / P;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * kB;
     double emax = 0;
-    for (int64_t i0 = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i0 < len; i0 += U * stride) {
-        K kv[U][NS + 1]; // held in the stored type until used
-        double mi[U];
+    // one element: sumbk and the error ratio from its stage derivatives
+    auto elem = [&](const K (&kv)[NS + 1], double mi) -> double {
+        double k[NS + 1];
+#pragma unroll
+        for (int j = 1; j <= NS; ++j) k[j] = static_cast<double>(kv[j]);
+        double sb = a.b[1] * k[1];
+#pragma unroll
+        for (int j = 2; j <= NS; ++j)
+            if (a.b[j] != 0.0) sb = sb + a.b[j] * k[j];
+        double re;
+        if constexpr (RKF) {
+            double t = a.e[1] * k[1];
+#pragma unroll
+            for (int j = 2; j <= NS; ++j)
+                if (a.e[j] != 0.0) t = t + a.e[j] * k[j];
+            re = sb - t;
+        } else {
+            re = a.e[1] * k[1];
+#pragma unroll
+            for (int j = 2; j <= NS; ++j)
+                if (a.e[j] != 0.0) re = re + a.e[j] * k[j];
+        }
+        const double mm = fmax(mi, mi + sb);
+        emax = fmax(emax, fabs(re / (a.atol + a.rtol * fabs(mm))));
+        return sb;
+    };
+    for (int64_t q0 = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; q0 < nunit; q0 += U * stride) {
+        K kv[U][P][NS + 1]; // held in the stored type until used
+        double mi[U][P];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t i = i0 + u * stride;
-            const bool in = i < len;
+            const int64_t q = q0 + u * stride;
+            const bool in = q < nunit;
 #pragma unroll
-            for (int j = 1; j <= NS; ++j)
-                kv[u][j] = in && (a.b[j] != 0.0 || a.e[j] != 0.0) ? static_cast<const K*>(a.k[j])[i] : K(0);
-            mi[u] = in ? m[i] : 0.0;
+            for (int j = 1; j <= NS; ++j) {
+                const bool rd = in && (a.b[j] != 0.0 || a.e[j] != 0.0);
+                if constexpr (P == 2) {
+                    const KP v = rd ? static_cast<const KP*>(a.k[j])[q] : KP{0, 0};
+                    kv[u][0][j] = v.x;
+                    kv[u][1][j] = v.y;
+                } else {
+                    kv[u][0][j] = rd ? static_cast<const K*>(a.k[j])[q] : K(0);
+                }
+            }
+            if constexpr (P == 2) {
+                const MP v = in ? reinterpret_cast<const MP*>(m)[q] : MP{0, 0};
+                mi[u][0] = v.x;
+                mi[u][1] = v.y;
+            } else {
+                mi[u][0] = in ? m[q] : 0.0;
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t i = i0 + u * stride;
-            if (i >= len) break;
-            double k[NS + 1];
+            const int64_t q = q0 + u * stride;
+            if (q >= nunit) break;
+            double sb[P];
 #pragma unroll
-            for (int j = 1; j <= NS; ++j) k[j] = static_cast<double>(kv[u][j]);
-            double sb = a.b[1] * k[1];
+            for (int p = 0; p < P; ++p) sb[p] = elem(kv[u][p], mi[u][p]);
+            if constexpr (P == 2)
+                reinterpret_cast<MP*>(sumbk)[q] = MP{sb[0], sb[1]};
+            else
+                sumbk[q] = sb[0];
+        }
+    }
+    if (P > 1 && blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int64_t i = nunit * P; i < len; ++i) {
+            K kv[NS + 1];
 #pragma unroll
-            for (int j = 2; j <= NS; ++j)
-                if (a.b[j] != 0.0) sb = sb + a.b[j] * k[j];
-            double re;
-            if constexpr (RKF) {
-                double t = a.e[1] * k[1];
-#pragma unroll
-                for (int j = 2; j <= NS; ++j)
-                    if (a.e[j] != 0.0) t = t + a.e[j] * k[j];
-                re = sb - t;
-            } else {
-                re = a.e[1] * k[1];
-#pragma unroll
-                for (int j = 2; j <= NS; ++j)
-                    if (a.e[j] != 0.0) re = re + a.e[j] * k[j];
-            }
-            sumbk[i] = sb;
-            const double mm = fmax(mi[u], mi[u] + sb);
-            emax = fmax(emax, fabs(re / (a.atol + a.rtol * fabs(mm))));
+            for (int j = 1; j <= NS; ++j)
+                kv[j] = (a.b[j] != 0.0 || a.e[j] != 0.0) ? static_cast<const K*>(a.k[j])[i] : K(0);
+            sumbk[i] = elem(kv, m[i]);
         }
     }
     emax = block_max(emax);
@@ -711,7 +751,7 @@ void line_point(mafx_ctx* ctx, double* x, const double* a, double s, const doubl
     MAFX_HIP(hipGetLastError());
 }
 
-template <class K>
+template <class K, int P>
 void rk_error_k(mafx_ctx* ctx, const double* m, const ErrArgs& a, int ns, bool rkf_form, double* sumbk, int64_t len,
                 unsigned nb) {
     double* part = ctx->partials.as<double>();
@@ -719,25 +759,25 @@ void rk_error_k(mafx_ctx* ctx, const double* m, const ErrArgs& a, int ns, bool r
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, bl, 0, ctx->stream, m, a, sumbk, len, part); };
     if (rkf_form) {
         switch (ns) {
-        case 1: go(k_rk_error<1, true, K>); break;
-        case 2: go(k_rk_error<2, true, K>); break;
-        case 3: go(k_rk_error<3, true, K>); break;
-        case 4: go(k_rk_error<4, true, K>); break;
-        case 5: go(k_rk_error<5, true, K>); break;
-        case 6: go(k_rk_error<6, true, K>); break;
-        case 7: go(k_rk_error<7, true, K>); break;
-        case 8: go(k_rk_error<8, true, K>); break;
-        default: go(k_rk_error<13, true, K>); break;
+        case 1: go(k_rk_error<1, true, K, P>); break;
+        case 2: go(k_rk_error<2, true, K, P>); break;
+        case 3: go(k_rk_error<3, true, K, P>); break;
+        case 4: go(k_rk_error<4, true, K, P>); break;
+        case 5: go(k_rk_error<5, true, K, P>); break;
+        case 6: go(k_rk_error<6, true, K, P>); break;
+        case 7: go(k_rk_error<7, true, K, P>); break;
+        case 8: go(k_rk_error<8, true, K, P>); break;
+        default: go(k_rk_error<13, true, K, P>); break;
         }
     } else {
         switch (ns) {
-        case 1: go(k_rk_error<1, false, K>); break;
-        case 2: go(k_rk_error<2, false, K>); break;
-        case 3: go(k_rk_error<3, false, K>); break;
-        case 4: go(k_rk_error<4, false, K>); break;
-        case 5: go(k_rk_error<5, false, K>); break;
-        case 6: go(k_rk_error<6, false, K>); break;
-        default: go(k_rk_error<7, false, K>); break;
+        case 1: go(k_rk_error<1, false, K, P>); break;
+        case 2: go(k_rk_error<2, false, K, P>); break;
+        case 3: go(k_rk_error<3, false, K, P>); break;
+        case 4: go(k_rk_error<4, false, K, P>); break;
+        case 5: go(k_rk_error<5, false, K, P>); break;
+        case 6: go(k_rk_error<6, false, K, P>); break;
+        default: go(k_rk_error<7, false, K, P>); break;
         }
     }
 }
@@ -754,20 +794,28 @@ double rk_error(mafx_ctx* ctx, const double* m, const void* const* k, const doub
     a.atol = atol;
     a.rtol = rtol;
     const int64_t len = 3 * ctx->N;
-    // one resident round of the kernel (6 groups per CU at its <= 80 VGPRs), each lane taking two
-    // elements per trip; never more partials than the context's buffer holds (50 red_blocks)
-    const unsigned nb = static_cast<unsigned>(std::max<int64_t>(
-        1, std::min<int64_t>({(len + 2 * kB - 1) / (2 * kB), int64_t{6} * ctx->num_cus, int64_t{50} * red_blocks(len)})));
     require((ns >= 1 && ns <= 8) || ns == 13, "rk_error: 1..8 or 13 stages");
+    // float stage derivatives in pairs when every array is aligned for it
+    bool pairs = k_f32 && reinterpret_cast<uintptr_t>(m) % 16 == 0 && reinterpret_cast<uintptr_t>(sumbk) % 16 == 0;
+    for (int j = 1; j <= ns && pairs; ++j) pairs = reinterpret_cast<uintptr_t>(k[j]) % 8 == 0;
+    // one resident round of the kernel (6 groups per CU at <= 80 VGPRs, 4 for the pair form at
+    // <= 128), each lane taking two units per trip; never more partials than the context's buffer
+    // holds (50 red_blocks)
+    const int64_t per_lane = pairs ? 4 : 2;
+    const unsigned nb = static_cast<unsigned>(std::max<int64_t>(
+        1, std::min<int64_t>({(len + per_lane * kB - 1) / (per_lane * kB), int64_t{pairs ? 4 : 6} * ctx->num_cus,
+                              int64_t{50} * red_blocks(len)})));
     require(ns <= 7 || rkf_form, "rk_error: more than 7 stages only in the sumbk - sum e k form");
     int nread = 0;
     for (int j = 1; j <= ns; ++j) nread += (b[j] != 0.0 || e[j] != 0.0);
     {
         ProfScope ps(ctx, k_f32 ? "rk_error_k32" : "rk_error_f64", (k_f32 ? 4.0 * nread + 16.0 : 8.0 * (nread + 2)) * len);
-        if (k_f32)
-            rk_error_k<float>(ctx, m, a, ns, rkf_form, sumbk, len, nb);
+        if (pairs)
+            rk_error_k<float, 2>(ctx, m, a, ns, rkf_form, sumbk, len, nb);
+        else if (k_f32)
+            rk_error_k<float, 1>(ctx, m, a, ns, rkf_form, sumbk, len, nb);
         else
-            rk_error_k<double>(ctx, m, a, ns, rkf_form, sumbk, len, nb);
+            rk_error_k<double, 1>(ctx, m, a, ns, rkf_form, sumbk, len, nb);
         hipLaunchKernelGGL(k_final, dim3(1), dim3(kB), 0, ctx->stream, ctx->partials.as<double>(),
                            static_cast<int>(nb), 1, 0, ctx->scal.as<double>());
     }

@@ -636,6 +636,25 @@ def test_fp32_stage_derivatives(monkeypatch):
     assert dev <= 1e-5
 
 
+@pytest.mark.parametrize("mesh", [(15, 5, 3, 2e-9, 2e-9, 2e-9), (256, 32, 4, 2e-9, 2e-9, 2e-9)])
+def test_fp32_stage_derivatives_one_step(mesh, monkeypatch):
+    """One DP45 step of the FP32-field path with FP32 stage derivatives against FP64 ones
+    (MAFX_RK_K64=1), on an odd number of vector elements (3 x 225: the error pass takes pairs and
+    a tail) and an even one: every element within 4 FP32 roundings of the step's increment."""
+    res = {}
+    for k64 in ["0", "1"]:
+        monkeypatch.setenv("MAFX_RK_K64", k64)
+        res[k64] = _film_steps(mesh, "f32", "DP45", 1)
+    m0, m1_32 = res["0"][0], res["0"][1]
+    m1_64 = res["1"][1]
+    inc = np.max(np.abs(m1_64 - m0))
+    d1 = np.max(np.abs(m1_32 - m1_64))
+    print(f"\n[k32 one step] mesh={mesh} increment {inc:.3e} k32 - k64 {d1:.3e} ({d1 / (inc * 2.0 ** -24):.2f} ulps)",
+          flush=True)
+    assert 0 < d1 <= 4 * 2.0 ** -24 * inc
+    assert np.all(np.isfinite(m1_32))
+
+
 def test_relax_matches_reference_loop():
     """LLGIntegrator::relax (llg_integrator.cpp:43-66) on the GPU against the same loop driven
     through the oracle: E_prev = 1e20; while |(E_prev - E) / E_prev| > precision: E_prev = E,
