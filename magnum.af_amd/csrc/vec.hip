@@ -431,20 +431,6 @@ __global__ void k_axpy_dev(double* __restrict__ y, const double* __restrict__ a,
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
     if (i < n) y[i] = y[i] + (*a) * x[i];
 }
-// LBFGS two-loop scalars (lbfgs_minimizer.cpp:86-108): first loop alpha_i = rho d, coef = -alpha_i;
-// second loop beta = rho d, coef = alpha_i - beta
-__global__ void k_lbfgs_alpha(double* alpha_i, double* coef, const double* d, double rho) {
-    const double a = rho * (*d);
-    *alpha_i = a;
-    *coef = -a;
-}
-__global__ void k_lbfgs_beta(double* coef, const double* alpha_i, const double* d, double rho) {
-    // beta is rounded before the subtraction, as the reference's host arithmetic does; a fused
-    // multiply-subtract changes the iterates by an ulp and, over thousands of iterations, the path
-#pragma clang fp contract(off)
-    const double beta = rho * (*d);
-    *coef = *alpha_i - beta;
-}
 } // namespace
 
 void dot_dev(mafx_ctx* ctx, const double* a, const double* b, int64_t n, double* out_dev) {
@@ -563,7 +549,8 @@ __global__ void __launch_bounds__(kB) k_lbfgs_pair(const double* __restrict__ m,
 }
 
 // k_final's one-slot sum, then (MODE 1) alpha_i = rho d, coef = -alpha_i or (MODE 2) coef =
-// alpha_i - rho d with beta = rho d rounded first, as k_lbfgs_alpha / k_lbfgs_beta do: the two-loop
+// alpha_i - rho d with beta = rho d rounded first (the reference's host arithmetic rounds beta before
+// the subtraction; a fused multiply-subtract changes the iterates by an ulp): the two-loop
 // step's scalars without a launch of their own
 template <int MODE>
 __global__ void __launch_bounds__(kB) k_final_coef(const double* __restrict__ part, int nb, double* __restrict__ dot_out,
@@ -681,16 +668,6 @@ void neg_copy(mafx_ctx* ctx, double* y, const double* x, int64_t n) {
 
 void axpy_dev(mafx_ctx* ctx, double* y, const double* a_dev, const double* x, int64_t n) {
     hipLaunchKernelGGL(k_axpy_dev, dim3(grid_for(n, kB)), dim3(kB), 0, ctx->stream, y, a_dev, x, n);
-    MAFX_HIP(hipGetLastError());
-}
-
-void lbfgs_alpha(mafx_ctx* ctx, double* alpha_i, double* coef, const double* d, double rho) {
-    hipLaunchKernelGGL(k_lbfgs_alpha, dim3(1), dim3(1), 0, ctx->stream, alpha_i, coef, d, rho);
-    MAFX_HIP(hipGetLastError());
-}
-
-void lbfgs_beta(mafx_ctx* ctx, double* coef, const double* alpha_i, const double* d, double rho) {
-    hipLaunchKernelGGL(k_lbfgs_beta, dim3(1), dim3(1), 0, ctx->stream, coef, alpha_i, d, rho);
     MAFX_HIP(hipGetLastError());
 }
 

@@ -19,16 +19,13 @@ void sub(mafx_ctx* ctx, double* out, const double* a, const double* b, int64_t n
 void copy(mafx_ctx* ctx, double* dst, const double* src, int64_t n);
 int64_t reduction_partials_needed(int64_t n);
 // Device-side scalars (no host synchronisation): *out_dev = a . b and max |a| with the kernels and
-// reduction tree of dot() / max_abs(), y += (*a_dev) x, the LBFGS two-loop recurrences
-// (alpha_i = rho d, coef = -alpha_i; coef = alpha_i - rho d), and one read of n device doubles.
+// reduction tree of dot() / max_abs(), y += (*a_dev) x, and one read of n device doubles.
 void dot_dev(mafx_ctx* ctx, const double* a, const double* b, int64_t n, double* out_dev);
 void max_abs_dev(mafx_ctx* ctx, const double* a, int64_t n, double* out_dev);
 // out_dev[0] = a . b and out_dev[1] = max |a| in one pass over a, each bit for bit as dot_dev /
 // max_abs_dev form it
 void dot_max_dev(mafx_ctx* ctx, const double* a, const double* b, int64_t n, double* out_dev);
 void axpy_dev(mafx_ctx* ctx, double* y, const double* a_dev, const double* x, int64_t n);
-void lbfgs_alpha(mafx_ctx* ctx, double* alpha_i, double* coef, const double* d, double rho);
-void lbfgs_beta(mafx_ctx* ctx, double* coef, const double* alpha_i, const double* d, double rho);
 void read_device(mafx_ctx* ctx, const double* src_dev, int n, double* dst_host);
 // LBFGS fused vector work, bit for bit the unfused sequence's results:
 // lbfgs_fused: q_out = (q_in + (*coef_dev) v) [* scale] (negated when `negate`), *out_dev = dnext . q
