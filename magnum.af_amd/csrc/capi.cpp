@@ -188,6 +188,10 @@ void set_term(mafx_ctx* ctx, TermDev& t, const mafx_term_desc& d) {
             upload(t.ridx, d.index0, sizeof(uint32_t) * N);
             t.has_ridx = true;
         }
+        if (stencils_f32(ctx->precision))
+            rkky_weights_f32(ctx, t);
+        else
+            t.rkw.release();
         break;
     default: throw Error(MAFX_ERR_ARG, "unknown term kind " + std::to_string(d.kind));
     }
@@ -322,8 +326,7 @@ int mafx_ctx_create(const mafx_mesh* mesh, double Ms, const double* Ms_field_hos
                                                  ctx->hbuf.bytes + ctx->spec.bytes + ctx->tensor.bytes +
                                                  ctx->hdem.bytes);
         for (auto& t : ctx->terms)
-            ctx->device_bytes += static_cast<int64_t>(t.zee.bytes + t.rkky.bytes + t.aex.bytes + t.ridx.bytes +
-                                                      t.pfield.bytes + t.axfield.bytes);
+            ctx->device_bytes += static_cast<int64_t>(term_bytes(t));
         *out = ctx.release();
     });
 }
@@ -367,11 +370,9 @@ int mafx_ctx_update_term(mafx_ctx* ctx, int index, const mafx_term_desc* desc) {
         // kernels of earlier calls may still read the arrays being replaced
         MAFX_HIP(hipStreamSynchronize(ctx->stream));
         TermDev& t = ctx->terms[index];
-        const int64_t before = static_cast<int64_t>(t.zee.bytes + t.rkky.bytes + t.aex.bytes + t.ridx.bytes +
-                                                    t.pfield.bytes + t.axfield.bytes);
+        const int64_t before = static_cast<int64_t>(term_bytes(t));
         set_term(ctx, t, *desc);
-        ctx->device_bytes += static_cast<int64_t>(t.zee.bytes + t.rkky.bytes + t.aex.bytes + t.ridx.bytes +
-                                                  t.pfield.bytes + t.axfield.bytes) - before;
+        ctx->device_bytes += static_cast<int64_t>(term_bytes(t)) - before;
     });
 }
 

@@ -41,6 +41,7 @@ struct EpiTerm {
     const double* rk;
     const double* aex;
     const uint32_t* ridx;
+    const float* rkw; // 6N RKKY weights per neighbour for the FP32 stencils, or null
     const double* pf; // N: D_constants (DMI) or Ku1_field (UNIAXIAL), or null
     const double* ax; // 3N: normalised Ku1_axis_field (UNIAXIAL), or null
 };
@@ -323,10 +324,30 @@ __device__ __forceinline__ void cell_terms(const EpiParams& P, int64_t x, int64_
         }
         case MAFX_RKKY: {
             // rkky_exchange_field.cpp:195-337 entries, applied as a stencil, then / Ms (:19-29)
-            const int64_t sxy = P.nx * P.ny;
-            const double Ai = tm.aex[i];
             double a0 = 0, a1 = 0, a2 = 0;
-            if (Ai != 0) {
+            bool pre = false;
+            if constexpr (sizeof(T) == 4) {
+                // FP32 stencils: the neighbour weights precomputed once per term (k_rkky_weights),
+                // summed in the order of the FP64 form below
+                if (tm.rkw) {
+                    pre = true;
+                    const uint32_t o4 = static_cast<uint32_t>(i) * 4u;
+                    constexpr int ord[6] = {1, 0, 3, 2, 5, 4};
+                    float w[6];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) w[k] = ldb(tm.rkw + ord[k] * N, o4);
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        const double wk = static_cast<double>(w[k]);
+                        a0 += wk * static_cast<double>(nb[ord[k]].x);
+                        a1 += wk * static_cast<double>(nb[ord[k]].y);
+                        a2 += wk * static_cast<double>(nb[ord[k]].z);
+                    }
+                }
+            }
+            const int64_t sxy = P.nx * P.ny;
+            const double Ai = pre ? 0.0 : tm.aex[i];
+            if (!pre && Ai != 0) {
                 auto inplane = [&](bool ok, int64_t j, int k, double pref) {
                     if (!ok) return;
                     const double Aj = tm.aex[j];
@@ -356,8 +377,10 @@ __device__ __forceinline__ void cell_terms(const EpiParams& P, int64_t x, int64_
                 a1 += w * static_cast<double>(nb[k].y);
                 a2 += w * static_cast<double>(nb[k].z);
             };
-            zc(z + 1 < P.nz, i + sxy, 5);
-            zc(z > 0, i - sxy, 4);
+            if (!pre) {
+                zc(z + 1 < P.nz, i + sxy, 5);
+                zc(z > 0, i - sxy, 4);
+            }
             if (!P.ms) {
                 h0 = static_cast<T>(a0 / P.Ms);
                 h1 = static_cast<T>(a1 / P.Ms);
@@ -703,6 +726,56 @@ __global__ void k_reduce_slots(const double* __restrict__ partials, int64_t nb, 
 
 template <class C> C* spec_of(mafx_ctx* ctx) { return ctx->spec.as<C>(); }
 
+// RKKY stencil weights of cell i towards its six neighbours (x-, x+, y-, y+, z-, z+; 0 where the
+// neighbour is outside the mesh or the entry is absent), formed in FP64 exactly as the per-cell
+// FP64 form does (rkky_exchange_field.cpp:195-337) and stored as FP32 for the FP32 stencils.
+__global__ void __launch_bounds__(256) k_rkky_weights(const double* __restrict__ rk, const double* __restrict__ aex,
+                                                      const uint32_t* __restrict__ ridx, int64_t nx, int64_t ny,
+                                                      int64_t nz, double pxy, double py, double pz,
+                                                      float* __restrict__ w) {
+    const int64_t N = nx * ny * nz, sxy = nx * ny;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= N) return;
+    const int64_t z = i / sxy, r = i - z * sxy, y = r / nx, x = r - y * nx;
+    const double Ai = aex[i];
+    auto inplane = [&](bool ok, int64_t j, double pref) -> double {
+        if (!ok || Ai == 0) return 0.0;
+        const double Aj = aex[j];
+        return (2. * Ai) / pref * 2. * Aj / (Aj + Ai);
+    };
+    auto zc = [&](bool ok, int64_t j) -> double {
+        if (!ok) return 0.0;
+        const double Ri = rk[i], Rj = rk[j];
+        const uint32_t Ii = ridx ? ridx[i] : 0u, Ij = ridx ? ridx[j] : 0u;
+        if ((Ii == Ij) && (Ri != 0) && (Rj != 0)) return (2. * Ri) / pz;
+        if (Ai == 0) return 0.0;
+        const double Aj = aex[j];
+        return (2. * Ai) / pz * 2. * Aj / (Aj + Ai);
+    };
+    const double wk[6] = {inplane(x > 0, i - 1, pxy),      inplane(x + 1 < nx, i + 1, pxy),
+                          inplane(y > 0, i - nx, py),      inplane(y + 1 < ny, i + nx, py),
+                          zc(z > 0, i - sxy),              zc(z + 1 < nz, i + sxy)};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k * N + i] = static_cast<float>(wk[k]);
+}
+
+} // namespace
+
+void rkky_weights_f32(mafx_ctx* ctx, TermDev& t) {
+    const mafx_mesh& g = ctx->mesh;
+    const int64_t N = ctx->N;
+    t.rkw.alloc(sizeof(float) * 6 * N);
+    // the same prefactors as make_params: mu0 dx^2 couples x AND y neighbours, mu0 dz^2 the layers
+    const double pxy = kMu0 * std::pow(g.dx, 2), pz = kMu0 * std::pow(g.dz, 2);
+    hipLaunchKernelGGL(k_rkky_weights, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, t.rkky.as<double>(),
+                       t.aex.as<double>(), t.has_ridx ? t.ridx.as<uint32_t>() : nullptr, g.nx, g.ny, g.nz, pxy, pxy,
+                       pz, t.rkw.as<float>());
+    MAFX_HIP(hipGetLastError());
+    MAFX_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+namespace {
+
 EpiParams make_params(mafx_ctx* ctx, const EvalArgs& a, int64_t nblocks) {
     EpiParams P{};
     const mafx_mesh& g = ctx->mesh;
@@ -738,6 +811,7 @@ EpiParams make_params(mafx_ctx* ctx, const EvalArgs& a, int64_t nblocks) {
         e.rk = td.rkky.as<double>();
         e.aex = td.aex.as<double>();
         e.ridx = td.has_ridx ? td.ridx.as<uint32_t>() : nullptr;
+        e.rkw = td.rkw.p ? td.rkw.as<float>() : nullptr;
         e.pf = td.pfield.p ? td.pfield.as<double>() : nullptr;
         e.ax = td.axfield.p ? td.axfield.as<double>() : nullptr;
         if (e.pf || e.ax) P.has_fields = 1;

@@ -108,6 +108,7 @@ struct TermDev {
     DevBuf zee;       // 3N f64 when the external field is an array or a full-field callback
     DevBuf rkky, aex; // N f64
     DevBuf ridx;      // N u32 (optional)
+    DevBuf rkw;       // 6N f32: RKKY stencil weights per neighbour (x-, x+, y-, y+, z-, z+) of the FP32 stencils
     DevBuf pfield;    // N f64: D_constants (DMI) or Ku1_field (UNIAXIAL)
     DevBuf axfield;   // 3N f64: normalised Ku1_axis_field (UNIAXIAL)
     bool has_zee_array{false}, has_ridx{false};
@@ -119,12 +120,18 @@ struct TermDev {
 };
 // Uploads desc's parameters into t (construction and mafx_ctx_update_term).
 void set_term(mafx_ctx* ctx, TermDev& t, const mafx_term_desc& desc);
+// Fills t.rkw from t's RKKY arrays and the context's mesh (FP32 stencil precisions).
+void rkky_weights_f32(mafx_ctx* ctx, TermDev& t);
+inline size_t term_bytes(const TermDev& t) {
+    return t.zee.bytes + t.rkky.bytes + t.aex.bytes + t.ridx.bytes + t.rkw.bytes + t.pfield.bytes + t.axfield.bytes;
+}
 
 struct Workspace; // precision-specific demag buffers
 
 // The demag / dipole FFT convolution runs in FP64 (complex reference tensor) for MAFX_F64 and
 // MAFX_F32_STENCILS, in FP32 (symmetric real tensor) for MAFX_F32_FIELD and MAFX_F32_DEMAG.
 inline bool demag_f64(int precision) { return precision == MAFX_F64 || precision == MAFX_F32_STENCILS; }
+inline bool stencils_f32(int precision) { return precision == MAFX_F32_FIELD || precision == MAFX_F32_STENCILS; }
 // FP32 demag on the complex half spectrum of the whole tensor (MAFX_TERM_FULL_TENSOR) instead of
 // the real symmetric quarter
 inline bool demag_full_c32(int precision, int demag_flags) {

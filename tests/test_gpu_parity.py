@@ -440,6 +440,43 @@ def test_rkky_layers_with_indices():
     assert normwise(terms[0].H_in_Apm(state), sim.heff()) <= FP64_TOL
 
 
+@pytest.mark.parametrize("mesh", [(6, 5, 8, 2e-9, 2e-9, 1e-9), (256, 2, 8, 2e-9, 3e-9, 2e-9)])
+def test_rkky_fp32_weights_layers(mesh):
+    """FP32 stencils take the RKKY neighbour weights precomputed per term (FP64 formula, stored as
+    FP32): eight layers with layer indices, RKKY entries on some layers only, a non-uniform exchange
+    constant with zeros, on the generic per-cell kernel (nx = 6) and the z-column one (nx = 256).
+    A relative FP32 rounding of each weight is within the FP32 tolerance; the FP64 path keeps its
+    in-kernel weights at 1e-12.  Then new arrays assigned to the same term object: the cached
+    context uploads them (mafx_ctx_update_term) and its weights follow."""
+    nx, ny, nz = mesh[:3]
+    print(f"\n[rkky fp32 weights] mesh={mesh}", flush=True)
+    rng = np.random.default_rng(21)
+    rk = np.zeros((nx, ny, nz))
+    rk[:, :, 1:6] = -1e-3 * mesh[5] / 2 * (1 + rng.random((nx, ny, 5)))
+    A = 1e-11 * (1 + rng.random((nx, ny, nz)))
+    A[0, :, 3] = 0.0
+    idx = np.zeros((nx, ny, nz), dtype=np.uint32)
+    idx[:, :, 3:] = 1
+    idx[:, :, 6:] = 2
+    m = rand_m((nx, ny, nz), 22)
+    M = mf()
+    for prec, tol in [("f64", FP64_TOL), ("f32", FP32_TOL)]:
+        state, terms, sim = build(mesh, 1e6, m, [("rkky", {"rk": rk, "A": A, "idx": idx})])
+        h = M.LLGIntegrator(0.05, terms, precision=prec).H_in_Apm(state)
+        assert normwise(h, sim.heff()) <= tol, (prec, normwise(h, sim.heff()))
+    llg = M.LLGIntegrator(0.05, terms, precision="f32")
+    rk2 = np.full((nx, ny, nz), -2e-3 * mesh[5] / 2)
+    A2 = np.full((nx, ny, nz), 1.3e-11)
+    g = M.Mesh(*mesh)
+    terms[0].rk = M._scalar_flat(rk2, g)
+    terms[0].A = M._scalar_flat(A2, g)
+    terms[0].idx = None
+    h = llg.H_in_Apm(state)
+    _, _, sim2 = build(mesh, 1e6, m, [("rkky", {"rk": rk2, "A": A2})])
+    assert normwise(h, sim2.heff()) <= FP32_TOL, normwise(h, sim2.heff())
+    assert normwise(h, sim.heff()) > 1e-2  # the old arrays' field differs
+
+
 @pytest.mark.parametrize("precision,tol", [("f64", FP64_TOL), ("f32", 1e-5)])
 def test_rkky_multilayer_rows(precision, tol):
     """The C5 term list (demag, RKKY between layers, DMI, uniaxial, Zeeman) and RKKY alone on
