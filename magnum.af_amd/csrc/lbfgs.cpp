@@ -40,26 +40,48 @@ struct Lbfgs {
     }
     // The line search's trial: energy and gradient, plus g . s and max |g| reduced on the device and
     // read back with the energy partials in one transfer (one host synchronisation per trial).
-    double energy_gradient_dg(const double* m, double* g, const double* s, double& dg, double& gmax) {
+    // launch_trial() queues the device work and the transfer; after the stream is synchronised,
+    // read_trial() reads the values back and count_trial() makes them the search's latest trial.
+    struct Trial {
+        double stp, f, dg, gmax, torque, hmax;
+    };
+    int trial_ns{0};
+    void launch_trial(const double* m, double* g, const double* s) {
         EvalArgs ea;
         ea.m = m;
         ea.mode = OutMode::GRAD;
         ea.out = g;
         ea.want_energy = true;
         evaluate(ctx, ea);
-        const int ns = finish_energy_launch(ctx);
+        trial_ns = finish_energy_launch(ctx);
         double* sc = ctx->scal.as<double>();
-        dot_max_dev(ctx, g, s, n3, sc + ns);
-        MAFX_HIP(hipMemcpyAsync(ctx->host_scal, sc, sizeof(double) * (ns + 2), hipMemcpyDeviceToHost, ctx->stream));
-        MAFX_HIP(hipStreamSynchronize(ctx->stream));
-        double E = 0;
-        finish_energy_combine(ctx, ~0u, nullptr, &E);
-        dg = ctx->host_scal[ns];
-        gmax = ctx->host_scal[ns + 1];
-        last_torque = ctx->last_max_torque;
-        last_hmax = ctx->last_max_h;
+        dot_max_dev(ctx, g, s, n3, sc + trial_ns);
+        MAFX_HIP(hipMemcpyAsync(ctx->host_scal, sc, sizeof(double) * (trial_ns + 2), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    }
+    Trial read_trial(double stp) {
+        Trial t{};
+        t.stp = stp;
+        finish_energy_combine(ctx, ~0u, nullptr, &t.f);
+        t.dg = ctx->host_scal[trial_ns];
+        t.gmax = ctx->host_scal[trial_ns + 1];
+        t.torque = ctx->last_max_torque;
+        t.hmax = ctx->last_max_h;
+        return t;
+    }
+    void count_trial(const Trial& t) {
+        last_torque = t.torque;
+        last_hmax = t.hmax;
         ++evals;
-        return E;
+    }
+    double energy_gradient_dg(const double* m, double* g, const double* s, double& dg, double& gmax) {
+        launch_trial(m, g, s);
+        MAFX_HIP(hipStreamSynchronize(ctx->stream));
+        const Trial t = read_trial(0.0);
+        count_trial(t);
+        dg = t.dg;
+        gmax = t.gmax;
+        return t.f;
     }
     double mydot(const double* a, const double* b) { return dot(ctx, a, b, n3); }
     double mynorm(const double* a) { return std::sqrt(mydot(a, a)); }
@@ -174,7 +196,7 @@ int cstep(double& stx, double& fx, double& dx, double& sty, double& fy, double& 
 // dginit = g . s is handed in by the caller (the two-loop's phiPrime0, the same dot product with
 // the sign flipped, exactly); gmax returns max |g| of the last trial's gradient.
 int cvsrch(Lbfgs& L, double* m, const double* wa, double& f, double* g, double& stp, const double* s,
-           const double tolf, const double dginit, double& gmax) {
+           const double tolf, const double dginit, double& gmax, const Lbfgs::Trial* pre = nullptr) {
     int info = 0;
     int infoc = 1;
     const double xtol = 1e-15;
@@ -210,9 +232,17 @@ int cvsrch(Lbfgs& L, double* m, const double* wa, double& f, double* g, double& 
         if ((brackt && ((stp <= stmin) | (stp >= stmax))) | (nfev >= maxfev - 1) | (infoc == 0) |
             (brackt & (stmax - stmin <= xtol * stmax)))
             stp = stx;
-        line_point(L.ctx, m, wa, stp, s);
         double dg = 0;
-        f = L.energy_gradient_dg(m, g, s, dg, gmax);
+        if (pre && nfev == 0 && stp == pre->stp) {
+            // the first trial, evaluated before the search began (m and g hold it)
+            L.count_trial(*pre);
+            f = pre->f;
+            dg = pre->dg;
+            gmax = pre->gmax;
+        } else {
+            line_point(L.ctx, m, wa, stp, s);
+            f = L.energy_gradient_dg(m, g, s, dg, gmax);
+        }
         nfev++;
         const double ftest1 = finit + stp * dgtest;
         const double ftest2 = finit + eps * std::fabs(finit);
@@ -264,8 +294,8 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
     // slots are pointers that change roles, so no vector is copied.  mcur / gcur hold the current
     // iterate and gradient, mold / gold the previous ones; the history is a ring of hist + 1 slots
     // whose spare slot receives the next pair.
-    DevBuf xbuf, gbuf0, gbuf1, q, dir;
-    for (auto* b : {&xbuf, &gbuf0, &gbuf1, &q, &dir}) b->alloc(bytes);
+    DevBuf xbuf, gbuf0, gbuf1, q, dir, xspare, gspare;
+    for (auto* b : {&xbuf, &gbuf0, &gbuf1, &q, &dir, &xspare, &gspare}) b->alloc(bytes);
     std::array<DevBuf, hist + 1> sB, yB;
     for (size_t i = 0; i <= hist; ++i) {
         sB[i].alloc(bytes); // read only after being written (k <= iter)
@@ -280,6 +310,10 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
     double* mold = xbuf.as<double>();
     double* gcur = gbuf0.as<double>();
     double* gold = gbuf1.as<double>();
+    // the line search's first trial is evaluated into these before the search starts
+    double* mspec = xspare.as<double>();
+    double* gspec = gspare.as<double>();
+    constexpr int kPhiSlot = 32; // host_scal slot of the search direction's slope
     // rho_i = 1 / (s_i . y_i) of the stored pairs: the reference recomputes the dot product at every
     // use (lbfgs_minimizer.cpp:88,98); it is the y . s computed when the pair was stored (the dot
     // is commutative and deterministic), kept here.  Device scalars of the two-loop recursion:
@@ -370,20 +404,33 @@ int lbfgs_minimize(mafx_ctx* ctx, double* m, double tolerance, int64_t max_iter,
                             last ? 0.0 : rho[i + 1], last ? nullptr : ds + (i + 1), last ? nullptr : d_coef);
             }
         }
-        double phi;
-        read_device(ctx, d_dot, 1, &phi);
-        const double phiPrime0 = -phi;
+        // cvsrch's first trial is at step length 1 whenever dir is a descent direction (rate = 1
+        // and the step bounds, lbfgs_minimizer.cpp:205-206,253-282): it is
+        // evaluated now, into the spare buffers, and read back with the slope phi in one transfer.
+        // A direction that is not a descent direction discards it.
+        double rate = 1.0;
+        line_point(ctx, mspec, mold, rate, dir.as<double>());
+        L.launch_trial(mspec, gspec, dir.as<double>());
+        MAFX_HIP(hipMemcpyAsync(ctx->host_scal + kPhiSlot, d_dot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        MAFX_HIP(hipStreamSynchronize(ctx->stream));
+        const double phiPrime0 = -ctx->host_scal[kPhiSlot];
         // the line search's g . dir for dir = -q is phiPrime0 exactly (every product and partial
         // sum only changes sign); after the reset to q = grad it is computed
         double dginit = phiPrime0;
-        if (phiPrime0 > 0) {
-            neg_copy(ctx, dir.as<double>(), gold, n3);
-            iter = 0;
-            dginit = L.mydot(gold, dir.as<double>());
-        }
-        double rate = 1.0;
         double gmax = -1.0;
-        cvsrch(L, mcur, mold, f, gcur, rate, dir.as<double>(), tolerance, dginit, gmax);
+        if (phiPrime0 < 0) {
+            const Lbfgs::Trial first = L.read_trial(rate);
+            std::swap(mcur, mspec);
+            std::swap(gcur, gspec);
+            cvsrch(L, mcur, mold, f, gcur, rate, dir.as<double>(), tolerance, dginit, gmax, &first);
+        } else {
+            if (phiPrime0 > 0) {
+                neg_copy(ctx, dir.as<double>(), gold, n3);
+                iter = 0;
+                dginit = L.mydot(gold, dir.as<double>());
+            }
+            cvsrch(L, mcur, mold, f, gcur, rate, dir.as<double>(), tolerance, dginit, gmax);
+        }
         const double f1 = 1 + std::fabs(f);
         gradNorm = gmax >= 0.0 ? gmax : L.maxabs(gcur);
         if (verbose > 1) {
