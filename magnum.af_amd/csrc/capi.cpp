@@ -188,10 +188,7 @@ void set_term(mafx_ctx* ctx, TermDev& t, const mafx_term_desc& d) {
             upload(t.ridx, d.index0, sizeof(uint32_t) * N);
             t.has_ridx = true;
         }
-        if (stencils_f32(ctx->precision))
-            rkky_weights_f32(ctx, t);
-        else
-            t.rkw.release();
+        rkky_weights(ctx, t);
         break;
     default: throw Error(MAFX_ERR_ARG, "unknown term kind " + std::to_string(d.kind));
     }

@@ -41,7 +41,7 @@ struct EpiTerm {
     const double* rk;
     const double* aex;
     const uint32_t* ridx;
-    const float* rkw; // 6N RKKY weights per neighbour for the FP32 stencils, or null
+    const void* rkw; // 6N RKKY weights per neighbour in the stencils' type T, or null
     const double* pf; // N: D_constants (DMI) or Ku1_field (UNIAXIAL), or null
     const double* ax; // 3N: normalised Ku1_axis_field (UNIAXIAL), or null
 };
@@ -326,23 +326,22 @@ __device__ __forceinline__ void cell_terms(const EpiParams& P, int64_t x, int64_
             // rkky_exchange_field.cpp:195-337 entries, applied as a stencil, then / Ms (:19-29)
             double a0 = 0, a1 = 0, a2 = 0;
             bool pre = false;
-            if constexpr (sizeof(T) == 4) {
-                // FP32 stencils: the neighbour weights precomputed once per term (k_rkky_weights),
-                // summed in the order of the FP64 form below
-                if (tm.rkw) {
-                    pre = true;
-                    const uint32_t o4 = static_cast<uint32_t>(i) * 4u;
-                    constexpr int ord[6] = {1, 0, 3, 2, 5, 4};
-                    float w[6];
+            // the neighbour weights precomputed once per term (k_rkky_weights), summed in the
+            // order of the in-kernel form below (an absent neighbour's weight is 0)
+            if (tm.rkw) {
+                pre = true;
+                const T* wp = static_cast<const T*>(tm.rkw);
+                const uint32_t ot = static_cast<uint32_t>(i) * static_cast<uint32_t>(sizeof(T));
+                constexpr int ord[6] = {1, 0, 3, 2, 5, 4};
+                T w[6];
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) w[k] = ldb(tm.rkw + ord[k] * N, o4);
+                for (int k = 0; k < 6; ++k) w[k] = ldb(wp + ord[k] * N, ot);
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) {
-                        const double wk = static_cast<double>(w[k]);
-                        a0 += wk * static_cast<double>(nb[ord[k]].x);
-                        a1 += wk * static_cast<double>(nb[ord[k]].y);
-                        a2 += wk * static_cast<double>(nb[ord[k]].z);
-                    }
+                for (int k = 0; k < 6; ++k) {
+                    const double wk = static_cast<double>(w[k]);
+                    a0 += wk * static_cast<double>(nb[ord[k]].x);
+                    a1 += wk * static_cast<double>(nb[ord[k]].y);
+                    a2 += wk * static_cast<double>(nb[ord[k]].z);
                 }
             }
             const int64_t sxy = P.nx * P.ny;
@@ -728,11 +727,12 @@ template <class C> C* spec_of(mafx_ctx* ctx) { return ctx->spec.as<C>(); }
 
 // RKKY stencil weights of cell i towards its six neighbours (x-, x+, y-, y+, z-, z+; 0 where the
 // neighbour is outside the mesh or the entry is absent), formed in FP64 exactly as the per-cell
-// FP64 form does (rkky_exchange_field.cpp:195-337) and stored as FP32 for the FP32 stencils.
+// in-kernel form does (rkky_exchange_field.cpp:195-337) and stored in the stencils' type W.
+template <class W>
 __global__ void __launch_bounds__(256) k_rkky_weights(const double* __restrict__ rk, const double* __restrict__ aex,
                                                       const uint32_t* __restrict__ ridx, int64_t nx, int64_t ny,
                                                       int64_t nz, double pxy, double py, double pz,
-                                                      float* __restrict__ w) {
+                                                      W* __restrict__ w) {
     const int64_t N = nx * ny * nz, sxy = nx * ny;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= N) return;
@@ -756,20 +756,27 @@ __global__ void __launch_bounds__(256) k_rkky_weights(const double* __restrict__
                           inplane(y > 0, i - nx, py),      inplane(y + 1 < ny, i + nx, py),
                           zc(z > 0, i - sxy),              zc(z + 1 < nz, i + sxy)};
 #pragma unroll
-    for (int k = 0; k < 6; ++k) w[k * N + i] = static_cast<float>(wk[k]);
+    for (int k = 0; k < 6; ++k) w[k * N + i] = static_cast<W>(wk[k]);
 }
 
 } // namespace
 
-void rkky_weights_f32(mafx_ctx* ctx, TermDev& t) {
+void rkky_weights(mafx_ctx* ctx, TermDev& t) {
     const mafx_mesh& g = ctx->mesh;
     const int64_t N = ctx->N;
-    t.rkw.alloc(sizeof(float) * 6 * N);
+    const bool f32 = stencils_f32(ctx->precision);
+    t.rkw.alloc((f32 ? sizeof(float) : sizeof(double)) * 6 * N);
     // the same prefactors as make_params: mu0 dx^2 couples x AND y neighbours, mu0 dz^2 the layers
     const double pxy = kMu0 * std::pow(g.dx, 2), pz = kMu0 * std::pow(g.dz, 2);
-    hipLaunchKernelGGL(k_rkky_weights, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, t.rkky.as<double>(),
-                       t.aex.as<double>(), t.has_ridx ? t.ridx.as<uint32_t>() : nullptr, g.nx, g.ny, g.nz, pxy, pxy,
-                       pz, t.rkw.as<float>());
+    const uint32_t* ridx = t.has_ridx ? t.ridx.as<uint32_t>() : nullptr;
+    if (f32)
+        hipLaunchKernelGGL(k_rkky_weights<float>, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream,
+                           t.rkky.as<double>(), t.aex.as<double>(), ridx, g.nx, g.ny, g.nz, pxy, pxy, pz,
+                           t.rkw.as<float>());
+    else
+        hipLaunchKernelGGL(k_rkky_weights<double>, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream,
+                           t.rkky.as<double>(), t.aex.as<double>(), ridx, g.nx, g.ny, g.nz, pxy, pxy, pz,
+                           t.rkw.as<double>());
     MAFX_HIP(hipGetLastError());
     MAFX_HIP(hipStreamSynchronize(ctx->stream));
 }
@@ -811,7 +818,7 @@ EpiParams make_params(mafx_ctx* ctx, const EvalArgs& a, int64_t nblocks) {
         e.rk = td.rkky.as<double>();
         e.aex = td.aex.as<double>();
         e.ridx = td.has_ridx ? td.ridx.as<uint32_t>() : nullptr;
-        e.rkw = td.rkw.p ? td.rkw.as<float>() : nullptr;
+        e.rkw = td.rkw.p;
         e.pf = td.pfield.p ? td.pfield.as<double>() : nullptr;
         e.ax = td.axfield.p ? td.axfield.as<double>() : nullptr;
         if (e.pf || e.ax) P.has_fields = 1;

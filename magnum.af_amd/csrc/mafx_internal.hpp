@@ -108,7 +108,7 @@ struct TermDev {
     DevBuf zee;       // 3N f64 when the external field is an array or a full-field callback
     DevBuf rkky, aex; // N f64
     DevBuf ridx;      // N u32 (optional)
-    DevBuf rkw;       // 6N f32: RKKY stencil weights per neighbour (x-, x+, y-, y+, z-, z+) of the FP32 stencils
+    DevBuf rkw;       // 6N RKKY stencil weights per neighbour (x-, x+, y-, y+, z-, z+), in the stencils' type
     DevBuf pfield;    // N f64: D_constants (DMI) or Ku1_field (UNIAXIAL)
     DevBuf axfield;   // 3N f64: normalised Ku1_axis_field (UNIAXIAL)
     bool has_zee_array{false}, has_ridx{false};
@@ -120,8 +120,8 @@ struct TermDev {
 };
 // Uploads desc's parameters into t (construction and mafx_ctx_update_term).
 void set_term(mafx_ctx* ctx, TermDev& t, const mafx_term_desc& desc);
-// Fills t.rkw from t's RKKY arrays and the context's mesh (FP32 stencil precisions).
-void rkky_weights_f32(mafx_ctx* ctx, TermDev& t);
+// Fills t.rkw from t's RKKY arrays and the context's mesh, in the stencils' type of the precision.
+void rkky_weights(mafx_ctx* ctx, TermDev& t);
 inline size_t term_bytes(const TermDev& t) {
     return t.zee.bytes + t.rkky.bytes + t.aex.bytes + t.ridx.bytes + t.rkw.bytes + t.pfield.bytes + t.axfield.bytes;
 }

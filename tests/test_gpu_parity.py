@@ -441,15 +441,15 @@ def test_rkky_layers_with_indices():
 
 
 @pytest.mark.parametrize("mesh", [(6, 5, 8, 2e-9, 2e-9, 1e-9), (256, 2, 8, 2e-9, 3e-9, 2e-9)])
-def test_rkky_fp32_weights_layers(mesh):
-    """FP32 stencils take the RKKY neighbour weights precomputed per term (FP64 formula, stored as
-    FP32): eight layers with layer indices, RKKY entries on some layers only, a non-uniform exchange
-    constant with zeros, on the generic per-cell kernel (nx = 6) and the z-column one (nx = 256).
-    A relative FP32 rounding of each weight is within the FP32 tolerance; the FP64 path keeps its
-    in-kernel weights at 1e-12.  Then new arrays assigned to the same term object: the cached
+def test_rkky_precomputed_weights_layers(mesh):
+    """The RKKY neighbour weights are precomputed per term (FP64 expressions, stored in the
+    stencils' type): eight layers with layer indices, RKKY entries on some layers only, a
+    non-uniform exchange constant with zeros, on the generic per-cell kernel (nx = 6) and the
+    z-column one (nx = 256).  FP64 weights hold the 1e-12 bound; the FP32 rounding of each weight
+    is within the FP32 tolerance.  Then new arrays assigned to the same term object: the cached
     context uploads them (mafx_ctx_update_term) and its weights follow."""
     nx, ny, nz = mesh[:3]
-    print(f"\n[rkky fp32 weights] mesh={mesh}", flush=True)
+    print(f"\n[rkky weights] mesh={mesh}", flush=True)
     rng = np.random.default_rng(21)
     rk = np.zeros((nx, ny, nz))
     rk[:, :, 1:6] = -1e-3 * mesh[5] / 2 * (1 + rng.random((nx, ny, 5)))
